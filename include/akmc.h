@@ -1,0 +1,151 @@
+/*
+ * libakmc -- MI355X-native vacancy-hop AKMC engine, C ABI.
+ *
+ * Plain pointers and sizes only.  The engine owns device memory; the caller
+ * owns every host buffer it passes.  Every call returns an int status
+ * (AKMC_OK or a negative AKMC_ERR_*); akmc_last_error() gives the message of
+ * the last failure on the calling thread.  Nothing in the library calls
+ * exit() or abort().  One engine may be used by one host thread at a time;
+ * distinct engines may be created and used from different threads.
+ *
+ * Site encoding used by akmc_upload/akmc_download: one uint8 per lattice site,
+ * index x + 2*nx*(y + ny*z) with x doubled (x even = corner site, x odd = body
+ * site), bits: 1 = V, 2 = Cu, 4 = Mn, 8 = Ni, none = Fe -- membership in the
+ * reference's four occupancy hash containers (src/lattice/lattices_list.h:
+ * 329-333); a site reads as the first present of V, Cu, Mn, Ni, else Fe.
+ */
+#ifndef AKMC_H
+#define AKMC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AKMC_OK 0
+#define AKMC_ERR_INVALID (-1)   /* bad argument or state */
+#define AKMC_ERR_HIP (-2)       /* a HIP runtime call failed */
+#define AKMC_ERR_CAPACITY (-3)  /* a sector/replica exceeds a kernel capacity */
+#define AKMC_ERR_NO_EVENT (-4)  /* u * sum >= sum: the reference asserts here (kmc.cpp:245,393) */
+#define AKMC_ERR_CONFIG (-5)    /* config.yaml could not be parsed */
+#define AKMC_ERR_BUDGET (-6)    /* HBM admission budget exceeded */
+
+/* Rate precision. */
+#define AKMC_PREC_FP64 0  /* bit-exact with the reference CPU build */
+#define AKMC_PREC_FP32 1  /* FP32 bond energies and exp, FP64 sums, search and dt */
+
+/* Why akmc_run returned. */
+#define AKMC_STOP_TIME 1    /* the time loop finished: ceil(physics_time / T) steps */
+#define AKMC_STOP_EVENTS 2  /* the event cap was reached (mid-sector stop) */
+
+/* LatticeTypes::lat_type values (reference src/lattice/lattice_types.h:20-36)
+ * used for the alloy list. */
+#define AKMC_TYPE_FE 0x0001
+#define AKMC_TYPE_CU 0x0002
+#define AKMC_TYPE_NI 0x0004
+#define AKMC_TYPE_MN 0x0008
+
+#define AKMC_MAX_ALLOY 32
+
+/* Simulation parameters: the fields of the reference's config.yaml that the
+ * vacancy-hop path reads (profiles/config/config_values.h:15-70), plus the
+ * sector time threshold that the reference hard-codes to 3e-7 s
+ * (src/simulation.h:73-74). */
+typedef struct akmc_params {
+  int64_t box[3];            /* box.size, unit cells (x not doubled) */
+  double lattice_const;      /* box.lattice_const */
+  double cutoff_radius;      /* box.cutoff_radius; ghost width = ceil(cutoff / a0) */
+  int32_t n_alloy;           /* create.random.alloy, in document order */
+  int32_t alloy_type[AKMC_MAX_ALLOY]; /* lat_type value of each alloy key */
+  int64_t alloy_ratio[AKMC_MAX_ALLOY];
+  int64_t va_count;          /* create.random.va_count */
+  int64_t seed_create_types; /* seeds.* (0 = std::random_device, as the reference) */
+  int64_t seed_create_vacancy;
+  int64_t seed_event_selection; /* parsed but unused, as in the reference */
+  int64_t seed_time_inc;
+  double temperature;        /* simulation.temperature, K */
+  double physics_time;       /* simulation.physics_time, s */
+  double attempt_freq;       /* simulation.attempt_freq, 1/s */
+  double sector_time;        /* sublattice threshold T per sector cycle, s (default 3e-7) */
+  int32_t precision;         /* AKMC_PREC_* */
+  int32_t reserved;
+} akmc_params;
+
+typedef struct akmc_stats {
+  int64_t events;            /* events performed since creation */
+  int64_t step;              /* current time step */
+  int32_t sector_pos;        /* ring position 0..7 of the current sector (ring 0,7,2,5,3,4,1,6) */
+  int32_t in_sector;         /* 1 if a sector is in progress (event-cap stop) */
+  double evolution[8];       /* evolution time of each sector id */
+  double sector_threshold;   /* threshold of the sector in progress */
+  double sector_time;        /* partial time of the sector in progress */
+  int32_t stop;              /* AKMC_STOP_* of the last akmc_run */
+  int32_t reserved;
+  double seconds;            /* wall time of the last akmc_run */
+  double kernel_seconds;     /* device time of the event-loop launches of the last akmc_run */
+  int64_t launches;          /* event-loop launches of the last akmc_run */
+} akmc_stats;
+
+typedef struct akmc_engine akmc_engine;
+
+/* Defaults of the reference example config (examples/config.yaml). */
+int akmc_params_default(akmc_params* p);
+
+/* Reads the reference's config.yaml schema (examples/config.yaml; parser
+ * profiles/config/config_parsing.cpp:54-230).  Replaces
+ * ConfigParsing::newInstance/parse. */
+int akmc_params_from_yaml(const char* path, akmc_params* p);
+
+/* Creates an engine on HIP device `device`.  Replaces initialize_gpu and
+ * gpu_prepare (reference gpu/gpu_simulate.h:58-60,
+ * gpu/gpu_simulate.hip:1764-1831). */
+int akmc_create(const akmc_params* p, int device, akmc_engine** out);
+
+/* Initial random occupancy exactly as creation::createRandom
+ * (reference src/creation.cpp:96-164), then uploads it.  Replaces the host
+ * hash construction and the initInfo gather (gpu_simulate.hip:1874-2040). */
+int akmc_init_random(akmc_engine* e);
+
+/* The same initial occupancy written to a host array of 2*nx*ny*nz sites,
+ * without a device. */
+int akmc_random_sites(const akmc_params* p, uint8_t* sites, int64_t n);
+
+/* Site arrays (see the encoding above); n must equal akmc_num_sites(). */
+int64_t akmc_num_sites(const akmc_engine* e);
+int akmc_upload(akmc_engine* e, const uint8_t* sites, int64_t n);
+int akmc_download(akmc_engine* e, uint8_t* sites, int64_t n);
+
+/* Runs the sublattice time loop on the device until the loop ends or
+ * max_events more events have been performed.  Replaces
+ * firstSectLocalToGpu, first_calculate_GPU, calculate_GPU,
+ * selectAndPerformEventGPU, calculateNextAdvRegion and sector_final
+ * (gpu/gpu_simulate.h:31-54) together with SubLattice::startTimeLoop
+ * (src/algorithms/sl/sublattice.inl:25-190).  stats may be NULL. */
+int akmc_run(akmc_engine* e, int64_t max_events, akmc_stats* stats);
+
+int akmc_get_stats(akmc_engine* e, akmc_stats* stats);
+
+/* Parity hook: evaluates every vacancy of sector `sector` (sector id 0..7,
+ * bit0 x high, bit1 y high, bit2 z high) as ABVIModel::calcRates does
+ * (models/abvi/kmc.cpp:68-115, including its side effect on stale species
+ * tags) and returns their site indices in sweep order, their 8 rates each
+ * and the sector total.  *n receives the vacancy count (may exceed cap). */
+int akmc_sector_rates(akmc_engine* e, int sector, int64_t cap, int64_t* sites, double* rates, double* total,
+                      int64_t* n);
+
+/* Event log: when enabled, akmc_run records every event's (from, to) site
+ * indices, for trajectory comparison.  capacity 0 disables it. */
+int akmc_set_event_log(akmc_engine* e, int64_t capacity);
+int akmc_get_event_log(akmc_engine* e, int64_t* from, int64_t* to, int64_t cap, int64_t* n);
+
+/* Frees the engine.  Replaces gpu_final (gpu/gpu_simulate.h:56). */
+void akmc_destroy(akmc_engine* e);
+
+const char* akmc_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AKMC_H */

@@ -1,0 +1,212 @@
+"""Python binding of libakmc (include/akmc.h) over ctypes.
+
+The binding is thin: every call goes to the C ABI and raises AkmcError on a
+non-zero status.  Loading fails loudly when the HIP library has not been
+built; there is no fallback implementation.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "lib", "libakmc.so")
+
+AKMC_OK = 0
+AKMC_PREC_FP64 = 0
+AKMC_PREC_FP32 = 1
+AKMC_STOP_TIME = 1
+AKMC_STOP_EVENTS = 2
+MAX_ALLOY = 32
+
+# LatticeTypes::lat_type values (reference src/lattice/lattice_types.h:20-36).
+LAT_TYPES = {"V": 0, "Fe": 1, "Cu": 2, "Ni": 4, "Mn": 8,
+             "FeFe": 0x0101, "FeCu": 0x0102, "FeNi": 0x0104, "FeMn": 0x0108,
+             "CuCu": 0x0202, "CuNi": 0x0204, "CuMn": 0x0208,
+             "NiNi": 0x0404, "NiMn": 0x0408, "MnMn": 0x0808}
+
+# Site flag bits (one uint8 per site).
+BIT_V, BIT_CU, BIT_MN, BIT_NI = 1, 2, 4, 8
+
+
+class AkmcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("akmc error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("box", ctypes.c_int64 * 3), ("lattice_const", ctypes.c_double),
+                ("cutoff_radius", ctypes.c_double), ("n_alloy", ctypes.c_int32),
+                ("alloy_type", ctypes.c_int32 * MAX_ALLOY), ("alloy_ratio", ctypes.c_int64 * MAX_ALLOY),
+                ("va_count", ctypes.c_int64), ("seed_create_types", ctypes.c_int64),
+                ("seed_create_vacancy", ctypes.c_int64), ("seed_event_selection", ctypes.c_int64),
+                ("seed_time_inc", ctypes.c_int64), ("temperature", ctypes.c_double),
+                ("physics_time", ctypes.c_double), ("attempt_freq", ctypes.c_double),
+                ("sector_time", ctypes.c_double), ("precision", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("events", ctypes.c_int64), ("step", ctypes.c_int64), ("sector_pos", ctypes.c_int32),
+                ("in_sector", ctypes.c_int32), ("evolution", ctypes.c_double * 8),
+                ("sector_threshold", ctypes.c_double), ("sector_time", ctypes.c_double),
+                ("stop", ctypes.c_int32), ("reserved", ctypes.c_int32), ("seconds", ctypes.c_double),
+                ("kernel_seconds", ctypes.c_double), ("launches", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {"events": self.events, "step": self.step, "sector_pos": self.sector_pos,
+                "in_sector": self.in_sector, "evolution": np.array(self.evolution[:]),
+                "sector_threshold": self.sector_threshold, "sector_time": self.sector_time,
+                "stop": self.stop, "seconds": self.seconds, "kernel_seconds": self.kernel_seconds,
+                "launches": self.launches}
+
+
+_lib = None
+
+
+def lib():
+    """Loads libakmc.so; raises if it is missing (build it with __graft_entry__.build())."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libakmc.so not built at %s; run __graft_entry__.build()" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i64, i32, dbl = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
+    P = ctypes.POINTER
+    L.akmc_last_error.restype = ctypes.c_char_p
+    L.akmc_params_default.argtypes = [P(Params)]
+    L.akmc_params_from_yaml.argtypes = [ctypes.c_char_p, P(Params)]
+    L.akmc_create.argtypes = [P(Params), i32, P(vp)]
+    L.akmc_init_random.argtypes = [vp]
+    L.akmc_random_sites.argtypes = [P(Params), vp, i64]
+    L.akmc_num_sites.restype = i64
+    L.akmc_num_sites.argtypes = [vp]
+    L.akmc_upload.argtypes = [vp, vp, i64]
+    L.akmc_download.argtypes = [vp, vp, i64]
+    L.akmc_run.argtypes = [vp, i64, P(Stats)]
+    L.akmc_get_stats.argtypes = [vp, P(Stats)]
+    L.akmc_sector_rates.argtypes = [vp, i32, i64, vp, vp, P(dbl), P(i64)]
+    L.akmc_set_event_log.argtypes = [vp, i64]
+    L.akmc_get_event_log.argtypes = [vp, vp, vp, i64, P(i64)]
+    L.akmc_destroy.argtypes = [vp]
+    L.akmc_destroy.restype = None
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != AKMC_OK:
+        raise AkmcError(rc, lib().akmc_last_error().decode())
+
+
+def params_from_config(cfg):
+    """Builds akmc_params from a dict (same keys as the reference config.yaml
+    sections, flattened): box, alloy [(name, ratio), ...] in document order,
+    va_count, seeds{create_types, create_vacancy, event_selection, time_inc},
+    temperature, physics_time, attempt_freq, sector_T, lattice_const,
+    cutoff_radius, precision."""
+    p = Params()
+    _check(lib().akmc_params_default(ctypes.byref(p)))
+    for d in range(3):
+        p.box[d] = int(cfg["box"][d])
+    p.lattice_const = float(cfg.get("lattice_const", 2.85532))
+    p.cutoff_radius = float(cfg.get("cutoff_radius", 5.6))
+    alloy = list(cfg["alloy"])
+    p.n_alloy = len(alloy)
+    for i, (name, ratio) in enumerate(alloy):
+        p.alloy_type[i] = LAT_TYPES[name]
+        p.alloy_ratio[i] = int(ratio)
+    p.va_count = int(cfg["va_count"])
+    seeds = cfg["seeds"]
+    p.seed_create_types = int(seeds["create_types"])
+    p.seed_create_vacancy = int(seeds["create_vacancy"])
+    p.seed_event_selection = int(seeds.get("event_selection", 0))
+    p.seed_time_inc = int(seeds["time_inc"])
+    p.temperature = float(cfg["temperature"])
+    p.physics_time = float(cfg["physics_time"])
+    p.attempt_freq = float(cfg["attempt_freq"])
+    p.sector_time = float(cfg.get("sector_T", 3.0e-7))
+    p.precision = int(cfg.get("precision", AKMC_PREC_FP64))
+    return p
+
+
+def random_sites(params):
+    """Initial occupancy of creation::createRandom computed on the host."""
+    if isinstance(params, dict):
+        params = params_from_config(params)
+    n = 2 * params.box[0] * params.box[1] * params.box[2]
+    out = np.zeros(n, dtype=np.uint8)
+    _check(lib().akmc_random_sites(ctypes.byref(params), out.ctypes.data, n))
+    return out
+
+
+def params_from_yaml(path):
+    p = Params()
+    _check(lib().akmc_params_from_yaml(path.encode(), ctypes.byref(p)))
+    return p
+
+
+class Engine:
+    """One lattice resident on one GPU, driven through the C ABI."""
+
+    def __init__(self, params, device=0, init_random=True):
+        L = lib()
+        if isinstance(params, dict):
+            params = params_from_config(params)
+        self._p = params
+        h = ctypes.c_void_p()
+        _check(L.akmc_create(ctypes.byref(params), int(device), ctypes.byref(h)))
+        self.h = h
+        self.n_sites = L.akmc_num_sites(self.h)
+        if init_random:
+            _check(L.akmc_init_random(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().akmc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def upload(self, sites):
+        sites = np.ascontiguousarray(sites, dtype=np.uint8)
+        _check(lib().akmc_upload(self.h, sites.ctypes.data, sites.size))
+
+    def download(self):
+        out = np.zeros(self.n_sites, dtype=np.uint8)
+        _check(lib().akmc_download(self.h, out.ctypes.data, out.size))
+        return out
+
+    def run(self, max_events):
+        st = Stats()
+        _check(lib().akmc_run(self.h, int(max_events), ctypes.byref(st)))
+        return st.as_dict()
+
+    def stats(self):
+        st = Stats()
+        _check(lib().akmc_get_stats(self.h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def sector_rates(self, sector, cap=4096):
+        sites = np.zeros(cap, dtype=np.int64)
+        rates = np.zeros((cap, 8), dtype=np.float64)
+        tot = ctypes.c_double()
+        n = ctypes.c_int64()
+        _check(lib().akmc_sector_rates(self.h, int(sector), cap, sites.ctypes.data, rates.ctypes.data,
+                                       ctypes.byref(tot), ctypes.byref(n)))
+        m = min(n.value, cap)
+        return sites[:m], rates[:m], tot.value
+
+    def set_event_log(self, capacity):
+        _check(lib().akmc_set_event_log(self.h, int(capacity)))
+
+    def event_log(self, cap):
+        f = np.zeros(cap, dtype=np.int64)
+        t = np.zeros(cap, dtype=np.int64)
+        n = ctypes.c_int64()
+        _check(lib().akmc_get_event_log(self.h, f.ctypes.data, t.ctypes.data, cap, ctypes.byref(n)))
+        m = min(n.value, cap)
+        return f[:m], t[:m]

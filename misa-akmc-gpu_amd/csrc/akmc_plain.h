@@ -1,0 +1,70 @@
+// Host/device interface of the plain FP64 event-loop kernel.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "akmc_device.h"
+
+// Why a launch of the event loop returned.
+enum {
+  AKMC_RUN_OK = 0,
+  AKMC_RUN_TIME_DONE = 1,      // all ceil(physics_time / T) steps finished
+  AKMC_RUN_EVENT_CAP = 2,      // max_events events performed (mid-sector stop)
+  AKMC_RUN_NEED_RNG = 3,       // the uploaded random-number chunk is used up
+  AKMC_RUN_YIELD = 4,          // sector-visit budget of one launch used up; relaunch
+  AKMC_RUN_ERR_NO_EVENT = -10, // u * sum >= sum: no event above the target
+  AKMC_RUN_ERR_CAPACITY = -11  // a sector holds more vacancies than the kernel's capacity
+};
+
+namespace akmc {
+
+// Sublattice time-loop state (SubLattice::startTimeLoop locals and the
+// per-sector evolution_time of SectorMeta, reference sublattice.inl:25-143).
+struct PlainState {
+  double evo[8];      // evolution time by sector id
+  double thr;         // threshold of the sector in progress
+  double t;           // partial sector time of the sector in progress
+  int64_t step;       // time step
+  int64_t events;     // events performed so far
+  int32_t sect;       // ring position 0..7 (ring 0,7,2,5,3,4,1,6)
+  int32_t in_sector;  // 1 while a sector is in progress
+  int32_t status;     // AKMC_RUN_* of the last launch
+  int32_t pad;
+};
+
+struct PlainArgs {
+  uint8_t* sites;        // one flag byte per site, x + X*(y + Y*z)
+  uint32_t* vac_key;     // every vacancy's key
+  int32_t nvac;
+  BoxDims dims;
+  PlainState* state;
+  const double* u_stream;  // u_i = mt19937() * (1 / 4294967295) for event i
+  const double* l_stream;  // l_i = -log(r_i), r_i the time stream draw of event i
+  int64_t rng_base;        // event index of u_stream[0]
+  int64_t rng_end;         // first event index not covered by the streams
+  int64_t max_events;      // absolute event cap
+  int64_t time_steps;      // ceil(physics_time / T)
+  double sector_T;
+  double nu;
+  double kT;               // kB * T
+  uint32_t* log;           // optional (from, to) keys per event
+  int64_t log_base;
+  int64_t log_cap;
+  // Parity hook: when rates_sector >= 0 the kernel only evaluates that
+  // sector (calcRates with its side effect) and writes the sorted keys,
+  // rates and total, then returns.
+  int32_t rates_sector;
+  int32_t pad;
+  uint32_t* out_keys;
+  double* out_rates;
+  double* out_total;
+  int32_t* out_n;
+  int32_t out_cap;
+  int32_t pad2;
+};
+
+}  // namespace akmc
+
+extern "C" int akmc_plain_launch(const akmc::PlainArgs* args, hipStream_t stream);
+extern "C" int akmc_plain_set_stencil(const akmc::StencilTables* t);
+extern "C" int akmc_plain_vcap();

@@ -1,0 +1,331 @@
+// Plain FP64 vacancy-hop event loop for one lattice on one GPU.
+//
+// One wavefront runs the reference's sublattice time loop
+// (src/algorithms/sl/sublattice.inl:61-179) on the device: for each step the
+// 8 sectors in ring order, and inside a sector the event loop
+//   calcRates -> (sum == 0 ? jump to the threshold : select, perform, dt).
+// The chain is serial by construction, so the loop is written for latency
+// (akmc_wave.h): the sector's vacancy list, rates and running sums stay in
+// LDS; after a hop only the vacancies whose 55-site stencil holds one of the
+// two changed sites are re-evaluated, and the running sums are repaired
+// exactly (block fold + tail shift) instead of re-folded.
+//
+// Evaluation points match the reference.  The reference re-evaluates every
+// vacancy of the sector at the start of every event (ABVIModel::calcRates,
+// models/abvi/kmc.cpp:68-115), and that evaluation has one side effect
+// (VacRatesSolver::deltaE moves the target's species tag to the vacancy and
+// back, which drops a stale copy held by the vacancy,
+// vacancy_rates_solver.cpp:81-163).  Here every vacancy of the sector is
+// evaluated at the sector's first event and at every launch start (always an
+// event start); afterwards, at the start of each further event of the sector,
+// exactly the vacancies whose stencil changed are evaluated.  The others
+// would reproduce their stored rates and an already-applied, idempotent side
+// effect.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "akmc_exp_table.h"
+#include "akmc_plain.h"
+#include "akmc_wave.h"
+
+namespace akmc {
+
+__constant__ StencilTables c_sten;
+__constant__ uint64_t c_exp_tab[256] = AKMC_EXP_TAB_INIT;
+__constant__ double c_e1[25] = AKMC_E1_TABLE;
+__constant__ double c_e2[25] = AKMC_E2_TABLE;
+__constant__ double c_e0[5] = AKMC_E0_TABLE;
+
+static constexpr int PLAIN_VCAP = 1024;
+static const int kRing[8] = {0, 7, 2, 5, 3, 4, 1, 6};  // SectorMeta ring, sublattice.h:21-30
+
+using PlainSector = SectorLDS<PLAIN_VCAP, double>;
+
+__device__ void stage_tables(SharedTables& T) {
+  const int l = lane_id();
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(&c_sten);
+  uint32_t* dst = reinterpret_cast<uint32_t*>(&T.st);
+  for (int i = l; i < (int)(sizeof(StencilTables) / 4); i += 64) dst[i] = src[i];
+  for (int i = l; i < 25; i += 64) { T.e1[i] = c_e1[i]; T.e2[i] = c_e2[i]; }
+  for (int i = l; i < 5; i += 64) T.e0[i] = c_e0[i];
+  for (int i = l; i < 256; i += 64) T.exp_tab[i] = c_exp_tab[i];
+  wave_sync();
+}
+
+// Collects the vacancies of sector s from the global list and sorts them by
+// key (sweep order).  Returns the count, or -1 if it exceeds the capacity.
+__device__ int collect_sector(PlainSector& S, const LatticeView& L, int s) {
+  const int l = lane_id();
+  int n = 0;
+  for (int b = 0; b < L.nvac; b += 64) {
+    const int j = b + l;
+    const uint32_t k = j < L.nvac ? L.vac_key[j] : 0u;
+    const bool in = j < L.nvac && key_sector(L.d, k) == s;
+    const uint64_t m = ballot(in);
+    const int r = n + __popcll(m & ((1ull << l) - 1));
+    if (in && r < PLAIN_VCAP) { S.key[r] = k; S.slot[r] = j; }
+    n += __popcll(m);
+  }
+  if (n > PLAIN_VCAP) return -1;
+  int n2 = 1;
+  while (n2 < n) n2 <<= 1;
+  for (int i = n + l; i < n2; i += 64) { S.key[i] = 0xFFFFFFFFu; S.slot[i] = -1; }
+  wave_sync();
+  for (int k = 2; k <= n2; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = l; i < n2; i += 64) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const bool up = (i & k) == 0;
+          const uint32_t a = S.key[i], b = S.key[ixj];
+          if ((a > b) == up) {
+            S.key[i] = b; S.key[ixj] = a;
+            const int32_t t = S.slot[i]; S.slot[i] = S.slot[ixj]; S.slot[ixj] = t;
+          }
+        }
+      }
+      wave_sync();
+    }
+  }
+  return n;
+}
+
+// Moves entry `from_i` to position `to_i` (shifting the entries between), or
+// deletes it when to_i < 0.  pvac is not moved (it is repaired afterwards).
+__device__ void move_entry(PlainSector& S, int nv, int from_i, int to_i) {
+  const int l = lane_id();
+  int lo, hi, delta;
+  if (to_i < 0) { lo = from_i + 1; hi = nv - 1; delta = -1; }
+  else if (to_i >= from_i) { lo = from_i + 1; hi = to_i; delta = -1; }
+  else { lo = to_i; hi = from_i - 1; delta = 1; }
+  const int cnt = hi - lo + 1;
+  if (cnt <= 0) return;
+  const int rounds = (cnt + 63) >> 6;
+  for (int rr = 0; rr < rounds; rr++) {
+    // Moving down: process from the low end; moving up: from the high end.
+    const int r = delta < 0 ? rr : rounds - 1 - rr;
+    const int v = lo + r * 64 + l;
+    const bool act = v <= hi;
+    uint32_t k = 0; int32_t sl = 0; uint8_t o = 0; uint64_t tm = 0;
+    double rt[8]; uint8_t nb[8];
+#pragma unroll
+    for (int b = 0; b < 8; b++) { rt[b] = 0.0; nb[b] = 0; }
+    if (act) {
+      k = S.key[v]; sl = S.slot[v]; o = S.own[v]; tm = S.tmask[v];
+#pragma unroll
+      for (int b = 0; b < 8; b++) { rt[b] = S.rate[8 * v + b]; nb[b] = S.nb1[8 * v + b]; }
+    }
+    wave_sync();
+    if (act) {
+      const int w = v + delta;
+      S.key[w] = k; S.slot[w] = sl; S.own[w] = o; S.tmask[w] = tm;
+#pragma unroll
+      for (int b = 0; b < 8; b++) { S.rate[8 * w + b] = rt[b]; S.nb1[8 * w + b] = nb[b]; }
+    }
+    wave_sync();
+  }
+}
+
+__global__ __launch_bounds__(64) void plain_kernel(PlainArgs a) {
+  __shared__ PlainSector S;
+  __shared__ SharedTables T;
+  const int l = lane_id();
+  stage_tables(T);
+  const LatticeView L{a.sites, a.vac_key, a.nvac, a.dims, a.nu, a.kT};
+  const Patch nopatch{0xFFFFFFFFu, 0xFFFFFFFFu, 0, 0, -1};
+  __shared__ double evo[8];  // sector clocks (indexed by the running sector id)
+  PlainState st = *a.state;  // every lane keeps an identical copy
+  if (l < 8) evo[l] = a.state->evo[l];
+  wave_sync();
+  bool rebuild = true;
+  double sum = 0.0;
+  int nv = 0;
+  int s = kRing[st.sect];
+  int32_t status = AKMC_RUN_OK;
+  int64_t visits = 0;  // sector visits this launch (bounded so a launch always returns)
+  // Random numbers of the next event, loaded one event ahead.
+  double u_next = 0.0, l_next = 0.0;
+  if (a.rates_sector < 0 && a.u_stream && a.l_stream && st.events >= a.rng_base && st.events < a.rng_end) {
+    u_next = a.u_stream[st.events - a.rng_base];
+    l_next = a.l_stream[st.events - a.rng_base];
+  }
+
+  if (a.rates_sector >= 0) {
+    // Parity hook: evaluate one sector as that sector's first event would.
+    st.in_sector = 1;
+    st.t = 0.0;
+    st.thr = 1.0;
+    st.events = 0;
+    for (int i = 0; i < 8; i++)
+      if (kRing[i] == a.rates_sector) st.sect = i;
+  }
+  for (;;) {
+    // Make this wave's earlier global writes visible to its later reads.
+    __threadfence_block();
+    if (!st.in_sector) {
+      if (st.step >= a.time_steps) { status = AKMC_RUN_TIME_DONE; break; }
+      if (++visits > (int64_t)1 << 16) { status = AKMC_RUN_YIELD; break; }
+      s = kRing[st.sect];
+      // step_threshold_time (sublattice.inl:72)
+      st.thr = static_cast<double>(st.step + 1) * a.sector_T - evo[s];
+      st.t = 0.0;
+      st.in_sector = 1;
+      rebuild = true;
+    }
+    s = kRing[st.sect];
+    if (!(st.t < st.thr)) {
+      if (l == 0) evo[s] += st.t;  // sublattice.inl:140
+      wave_sync();
+      st.in_sector = 0;
+      if (++st.sect == 8) { st.sect = 0; st.step++; }
+      continue;
+    }
+    if (st.events >= a.max_events) { status = AKMC_RUN_EVENT_CAP; break; }
+    if (st.events >= a.rng_end) { status = AKMC_RUN_NEED_RNG; break; }
+
+    if (rebuild) {
+      nv = collect_sector(S, L, s);
+      if (nv < 0) { status = AKMC_RUN_ERR_CAPACITY; break; }
+      eval_list<PLAIN_VCAP, double, false>(S, T, L, nullptr, 0, nv, nopatch, false);
+      sum = fold_range<PLAIN_VCAP, double>(S, 0, nv, 0.0);
+      rebuild = false;
+      if (a.rates_sector >= 0) {
+        for (int i = l; i < nv && i < a.out_cap; i += 64) {
+          a.out_keys[i] = S.key[i];
+          for (int b = 0; b < 8; b++) a.out_rates[8 * i + b] = S.rate[8 * i + b];
+        }
+        if (l == 0) { *a.out_total = sum; *a.out_n = nv; }
+        status = AKMC_RUN_OK;
+        break;
+      }
+    }
+    // sum_rates == 0 or below epsilon: jump to the threshold (sublattice.inl:108-112).
+    if (sum == 0.0 || fabs(sum) < 2.220446049250313e-16) {
+      st.t = st.thr;
+      continue;
+    }
+
+    // ---- select: first (vacancy, dir) whose running sum exceeds u * sum
+    // (ModelAdapter::selectAndPerform, model_adapter.h:95-101; ABVIModel::select, kmc.cpp:242-309).
+    const double target = u_next * sum;
+    const double lg = l_next;
+    const int vsel = first_above(S.pvac, nv, target);
+    if (vsel >= nv) { status = AKMC_RUN_ERR_NO_EVENT; break; }
+    int bsel = -1;
+    {
+      double p = vsel == 0 ? 0.0 : S.pvac[vsel - 1];
+      for (int b = 0; b < 8; b++) {
+        p += S.rate[vsel * 8 + b];
+        if (p > target) { bsel = b; break; }
+      }
+    }
+    if (bsel < 0) { status = AKMC_RUN_ERR_NO_EVENT; break; }
+
+    // ---- perform (ABVIModel::perform, kmc.cpp:358-396): the vacancy moves to
+    // the target; the target's first species tag of Cu, Mn, Ni moves back.
+    const uint32_t from = S.key[vsel];
+    const int par = (int)(key_x(from) & 1);
+    int ox, oy, oz;
+    nn1_offset(par, bsel, &ox, &oy, &oz);
+    const uint32_t to = key_add(L.d, from, ox, oy, oz);
+    const uint8_t ff = S.own[vsel];
+    const uint8_t gt = S.nb1[vsel * 8 + bsel];
+    const uint8_t xb = (gt & BIT_CU) ? BIT_CU : (gt & BIT_MN) ? BIT_MN : (gt & BIT_NI) ? BIT_NI : 0;
+    const uint8_t nf_from = (uint8_t)((ff & ~BIT_V) | xb);
+    const uint8_t nf_to = (uint8_t)((gt & ~xb) | BIT_V);
+    const int32_t vslot = S.slot[vsel];
+    if (l == 0) {
+      L.sites[key_index(L.d, from)] = nf_from;
+      L.sites[key_index(L.d, to)] = nf_to;
+      L.vac_key[vslot] = to;
+      const int64_t li = st.events - a.log_base;
+      if (a.log && li >= 0 && li < a.log_cap) { a.log[2 * li] = from; a.log[2 * li + 1] = to; }
+    }
+    // ---- residence time (sublattice.inl:118-120): dt = -log(r) / sum.
+    st.t += lg / sum;
+    st.events++;
+    if (st.events < a.rng_end) {
+      u_next = a.u_stream[st.events - a.rng_base];
+      l_next = a.l_stream[st.events - a.rng_base];
+    }
+    if (!(st.t < st.thr) || st.events >= a.max_events || st.events >= a.rng_end) {
+      rebuild = true;
+      continue;
+    }
+
+    // ---- the next event's calcRates, incrementally.
+    const bool stays = key_sector(L.d, to) == s;
+    int knew = -1;
+    if (stays) {
+      const int lb = lower_bound_key<PLAIN_VCAP>(S.key, nv, to);
+      knew = lb > vsel ? lb - 1 : lb;  // rank of `to` among the other keys
+    }
+    move_entry(S, nv, vsel, stays ? knew : -1);
+    if (stays) {
+      if (l == 0) { S.key[knew] = to; S.slot[knew] = vslot; }
+    } else {
+      nv -= 1;
+    }
+    wave_sync();
+    // Candidates: sorted keys within 3 z-planes of the hop (the stencil reaches
+    // 2 planes; `to` is one plane from `from`), unwrapped since a sector never
+    // wraps around the box.
+    const int zf = (int)key_z(from);
+    const int zlo = min(zf, zf + oz) - 2, zhi = max(zf, zf + oz) + 2;
+    const int ilo = zlo <= 0 ? 0 : lower_bound_key<PLAIN_VCAP>(S.key, nv, make_key(0, 0, (uint32_t)zlo));
+    const int ihi =
+        zhi + 1 >= L.d.Z ? nv : lower_bound_key<PLAIN_VCAP>(S.key, nv, make_key(0, 0, (uint32_t)(zhi + 1)));
+    int amin = 0x7FFFFFFF, amax = -1;
+    const Patch pt{from, to, nf_from, nf_to, stays ? knew : -1};
+    for (int b0 = ilo; b0 < ihi; b0 += 64) {
+      const int v = b0 + l;
+      bool hit = false;
+      if (v < ihi) {
+        const uint32_t w = S.key[v];
+        hit = in_stencil(L.d, T.st, w, from) || in_stencil(L.d, T.st, w, to);
+      }
+      const uint64_t m = ballot(hit);
+      if (m) {
+        const int r = __popcll(m & ((1ull << l) - 1));
+        if (hit) S.list[r] = v;
+        amin = min(amin, b0 + first_set(m));
+        amax = max(amax, b0 + 63 - (int)__clzll(m));
+        wave_sync();
+        eval_list<PLAIN_VCAP, double, false>(S, T, L, S.list, 0, __popcll(m), pt, true);
+      }
+    }
+    // ---- running sums.
+    if (nv == 0) {
+      sum = 0.0;
+    } else if (!stays) {
+      const int c_lo = min(min(vsel, amin), nv);
+      sum = c_lo < nv ? fold_range<PLAIN_VCAP, double>(S, c_lo, nv, c_lo > 0 ? S.pvac[c_lo - 1] : 0.0)
+                      : S.pvac[nv - 1];
+    } else {
+      const int c_lo = min(min(vsel, knew), amin);
+      const int c_hi = max(max(vsel, knew), amax);
+      const double Po = S.pvac[c_hi];
+      const double Pn = fold_range<PLAIN_VCAP, double>(S, c_lo, c_hi + 1, c_lo > 0 ? S.pvac[c_lo - 1] : 0.0);
+      sum = c_hi + 1 < nv ? tail_update<PLAIN_VCAP, double>(S, c_hi + 1, nv, Pn, Po) : Pn;
+    }
+  }
+  if (l == 0 && a.rates_sector < 0) {
+    st.status = status;
+    for (int i = 0; i < 8; i++) st.evo[i] = evo[i];
+    *a.state = st;
+  }
+  if (l == 0 && a.rates_sector >= 0 && status != AKMC_RUN_OK) *a.out_n = status;
+}
+
+}  // namespace akmc
+
+extern "C" int akmc_plain_launch(const akmc::PlainArgs* args, hipStream_t stream) {
+  hipLaunchKernelGGL(akmc::plain_kernel, dim3(1), dim3(64), 0, stream, *args);
+  return (int)hipGetLastError();
+}
+
+extern "C" int akmc_plain_set_stencil(const akmc::StencilTables* t) {
+  return (int)hipMemcpyToSymbol(HIP_SYMBOL(akmc::c_sten), t, sizeof(*t));
+}
+
+extern "C" int akmc_plain_vcap() { return akmc::PLAIN_VCAP; }

@@ -1,0 +1,566 @@
+// refcpu -- CPU restatement of the reference's vacancy-hop AKMC CPU build.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the oracle: only tests/, the smoke
+// check in __graft_entry__.py and bench.py's cpu_baseline leg may load it, and
+// only as the checker or the timed CPU baseline.  The product library never
+// links or calls it.
+//
+// It restates the reference CPU build (-DKMC_HIP_ENABLE_FLAG=OFF, KMC_RAND=MT,
+// one MPI rank) with the reference's own data structures: occupancy lives in
+// four hash containers (vac/cu/mn/ni), every event re-sweeps the whole sector
+// through those hashes, and the C library's exp/log are used.  Each function
+// cites the reference lines it follows.  Single-rank ghost sites are periodic
+// images of the box (ghost init + per-sector sync of src/pack/*); here that is
+// expressed as an index wrap on the ghost-extended coordinates, see wrap().
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <random>
+#include <string>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+
+namespace refcpu {
+
+// LatticeTypes::lat_type values (src/lattice/lattice_types.h:20-36).
+enum LatType : int { V = 0, Fe = 1, Cu = 2, Ni = 4, Mn = 8 };
+
+// PairBond::makeBond key (models/abvi/rate/bonds/pair_bond.hpp:58-65).
+static inline int make_bond(int a, int b) {
+  if (a > b) { int t = a; a = b; b = t; }
+  return (a << 8) | b;
+}
+
+// Bond tables (models/abvi/rate/bonds/bonds_counter.cpp:13-33), in eV.
+static double bond1(int key) {
+  switch (key) {
+  case 0x0101: return -0.778; case 0x0000: return 0.315; case 0x0202: return -0.581;
+  case 0x0404: return -0.793; case 0x0808: return -0.438;
+  case 0x0001: return -0.161; case 0x0102: return -0.609; case 0x0104: return -0.821;
+  case 0x0108: return -0.648;
+  case 0x0002: return -0.103; case 0x0004: return -0.234; case 0x0008: return -0.151;
+  case 0x0204: return -0.692; case 0x0208: return -0.519; case 0x0408: return -0.831;
+  }
+  fprintf(stderr, "refcpu: bad 1nn bond key %x\n", key);
+  abort();
+}
+static double bond2(int key) {
+  switch (key) {
+  case 0x0101: return -0.389; case 0x0000: return -0.214; case 0x0202: return -0.389;
+  case 0x0404: return -0.389; case 0x0808: return -0.389;
+  case 0x0001: return -0.161; case 0x0102: return -0.344; case 0x0104: return -0.399;
+  case 0x0108: return -0.364;
+  case 0x0002: return -0.206; case 0x0004: return -0.351; case 0x0008: return -0.206;
+  case 0x0204: return -0.344; case 0x0208: return -0.249; case 0x0408: return -0.464;
+  }
+  fprintf(stderr, "refcpu: bad 2nn bond key %x\n", key);
+  abort();
+}
+
+// VacRatesSolver::e0 (models/abvi/rate/vacancy_rates_solver.cpp:14-27).
+static double e0(int ghost_atom) {
+  switch (ghost_atom) {
+  case Fe: return 0.62;
+  case Cu: return 0.54;
+  case Ni: return 0.68;
+  case Mn: return 1.03;
+  default: return 0;
+  }
+}
+
+// models/abvi/rate/rates_types.h:16
+static const double kBoltzmann = 8.617330350E-5;
+
+struct VacancyHash {  // models/abvi/defect/vac_hash.h:11-20
+  double nn_rates[8];
+  unsigned char avail_trans_dir;
+};
+
+struct Params {
+  int64_t box[3];           // cells (x not doubled)
+  int64_t ghost;            // ghost width in cells = ceil(cutoff / a0)
+  std::vector<int> types;   // alloy keys in YAML order (lat_type values)
+  std::vector<int64_t> ratio;
+  int64_t va_count;
+  int64_t seed_create_types, seed_create_vacancy, seed_time_inc;
+  double temperature, attempt_freq, sector_T, physics_time;
+  uint32_t mask_2nn;        // 0x2F in the periodic lattice list
+};
+
+class RefCpu {
+ public:
+  explicit RefCpu(const Params& p) : P(p) {
+    // LatListMeta (src/lattice/lattice_list_meta.cpp:9-16), BCC_DBX = 2.
+    box_x = 2 * p.box[0]; box_y = p.box[1]; box_z = p.box[2];
+    ghost_x = 2 * p.ghost; ghost_y = p.ghost; ghost_z = p.ghost;
+    size_x = box_x + 2 * ghost_x; size_y = box_y + 2 * ghost_y; size_z = box_z + 2 * ghost_z;
+  }
+
+  // LatListMeta::getLId (src/lattice/lattice_list_meta.h:103-105).
+  int64_t getId(int64_t x, int64_t y, int64_t z) const { return x + size_x * (y + z * size_y); }
+  void coord(int64_t id, int64_t* x, int64_t* y, int64_t* z) const {
+    *x = id % size_x; id /= size_x; *y = id % size_y; *z = id / size_y;
+  }
+  // Ghost coordinates resolve to their periodic image inside the box.
+  static int64_t wrap1(int64_t c, int64_t g, int64_t n) {
+    int64_t r = (c - g) % n;
+    if (r < 0) r += n;
+    return r + g;
+  }
+  int64_t wrapId(int64_t x, int64_t y, int64_t z) const {
+    return getId(wrap1(x, ghost_x, box_x), wrap1(y, ghost_y, box_y), wrap1(z, ghost_z, box_z));
+  }
+
+  // PeriodLatticeList::getnnlattice readout order V > Cu > Mn > Ni > Fe
+  // (src/lattice/period_lattice_list.cpp:307-324).
+  int readout(int64_t id) const {
+    if (vac_hash.find(id) != vac_hash.end()) return V;
+    if (cu_hash.find(id) != cu_hash.end()) return Cu;
+    if (mn_hash.find(id) != mn_hash.end()) return Mn;
+    if (ni_hash.find(id) != ni_hash.end()) return Ni;
+    return Fe;
+  }
+
+  // 1nn ids of (x,y,z), PeriodLatticeList::get1nn2 (period_lattice_list.cpp:45-100).
+  void nn1(int64_t x, int64_t y, int64_t z, int64_t out[8]) const {
+    if (x % 2 == 0) {
+      out[0] = wrapId(x - 1, y - 1, z - 1); out[1] = wrapId(x - 1, y - 1, z);
+      out[2] = wrapId(x - 1, y, z - 1);     out[3] = wrapId(x - 1, y, z);
+      out[4] = wrapId(x + 1, y - 1, z - 1); out[5] = wrapId(x + 1, y - 1, z);
+      out[6] = wrapId(x + 1, y, z - 1);     out[7] = wrapId(x + 1, y, z);
+    } else {
+      out[0] = wrapId(x - 1, y, z);         out[1] = wrapId(x - 1, y, z + 1);
+      out[2] = wrapId(x - 1, y + 1, z);     out[3] = wrapId(x - 1, y + 1, z + 1);
+      out[4] = wrapId(x + 1, y, z);         out[5] = wrapId(x + 1, y, z + 1);
+      out[6] = wrapId(x + 1, y + 1, z);     out[7] = wrapId(x + 1, y + 1, z + 1);
+    }
+  }
+  // 2nn ids, PeriodLatticeList::get2nn2 (period_lattice_list.cpp:234-256).
+  void nn2(int64_t x, int64_t y, int64_t z, int64_t out[6]) const {
+    out[0] = wrapId(x - 2, y, z); out[1] = wrapId(x, y - 1, z); out[2] = wrapId(x, y, z - 1);
+    out[3] = wrapId(x, y, z + 1); out[4] = wrapId(x, y + 1, z); out[5] = wrapId(x + 2, y, z);
+  }
+
+  // BondsCounter::count (models/abvi/rate/bonds/bonds_counter.cpp:35-81) and
+  // count_exchange (:83-129, get1nn3/get2nn3 substitute `atom` at source_id).
+  double count_impl(int64_t now_id, int src_atom_type, int64_t source_id, int atom, bool exchange) const {
+    int64_t x, y, z;
+    coord(now_id, &x, &y, &z);
+    int64_t n1[8], n2[6];
+    nn1(x, y, z, n1);
+    double energy = 0.0;
+    for (int b = 0; b < 8; b++) {  // get1nnStatus = 0xFF
+      int t = (exchange && n1[b] == source_id) ? atom : readout(n1[b]);
+      energy += bond1(make_bond(t, src_atom_type));
+    }
+    nn2(x, y, z, n2);
+    for (int b = 0; b < 6; b++) {
+      if ((P.mask_2nn >> b) & 1) {
+        int t = (exchange && n2[b] == source_id) ? atom : readout(n2[b]);
+        energy += bond2(make_bond(t, src_atom_type));
+      }
+    }
+    return energy;
+  }
+  double count(int64_t id, int type) const { return count_impl(id, type, -1, 0, false); }
+  double count_exchange(int64_t now_id, int64_t source_id, int atom, int src_atom_type) const {
+    return count_impl(now_id, src_atom_type, source_id, atom, true);
+  }
+
+  std::unordered_set<int64_t>* species_hash(int t) {
+    switch (t) { case Cu: return &cu_hash; case Mn: return &mn_hash; case Ni: return &ni_hash; }
+    return nullptr;
+  }
+
+  // VacRatesSolver::deltaE (models/abvi/rate/vacancy_rates_solver.cpp:29-172),
+  // including the temporary hash exchange and exchange back.
+  double deltaE(int64_t source_id, int64_t target_id, int target_type, int ghost_atom) {
+    double e_before = 0;
+    {
+      double e_src = count(source_id, V);
+      double e_des = count(target_id, target_type);
+      e_before = e_src + e_des;
+    }
+    vac_hash.emplace(target_id, VacancyHash{});  // source keeps its V entry
+    std::unordered_set<int64_t>* h = species_hash(ghost_atom);
+    if (h) {
+      auto it = h->find(target_id);
+      if (it == h->end()) { fprintf(stderr, "refcpu: deltaE missing species\n"); abort(); }
+      h->erase(it);
+      h->emplace(source_id);
+    }
+    double e_after = 0;
+    {
+      double e_src = count(source_id, ghost_atom);
+      double e_des = count_exchange(target_id, source_id, ghost_atom, V);
+      e_after = e_src + e_des;
+    }
+    vac_hash.erase(vac_hash.find(target_id));
+    if (h) {
+      auto it = h->find(source_id);
+      if (it == h->end()) { fprintf(stderr, "refcpu: deltaE missing species (back)\n"); abort(); }
+      h->erase(it);
+      h->emplace(target_id);
+    }
+    return e_after - e_before;
+  }
+
+  // RatesSolver::rate and arrhenius (models/abvi/rate/rates_solver.cpp:13-26).
+  double rate(int64_t source_id, int64_t target_id, int ghost_atom) {
+    const double active_energy = e0(ghost_atom) + deltaE(source_id, target_id, ghost_atom, ghost_atom) / 2;
+    return P.attempt_freq * exp(-active_energy / (kBoltzmann * P.temperature));
+  }
+
+  // Sector region: local_sector_region[s] of the colored domain (low half
+  // floor(n/2) cells, bit0 = x high, bit1 = y high, bit2 = z high, shifted by
+  // the ghost width; described at src/simulation.cpp:91-110).
+  void region(int s, int64_t lo[3], int64_t hi[3]) const {
+    const int64_t n[3] = {P.box[0], P.box[1], P.box[2]};
+    for (int d = 0; d < 3; d++) {
+      const int64_t half = n[d] / 2;
+      const bool high = (s >> d) & 1;
+      lo[d] = P.ghost + (high ? half : 0);
+      hi[d] = P.ghost + (high ? n[d] : half);
+    }
+  }
+
+  // ABVIModel::calcRates (models/abvi/kmc.cpp:68-115) with
+  // VacancyHash::beforeRatesUpdate2/updateRates2 (vac_hash.cpp:7-46).
+  double calcRates(int s) {
+    int64_t lo[3], hi[3];
+    region(s, lo, hi);
+    double sum_rates = 0;
+    for (int64_t z = lo[2]; z < hi[2]; z++)
+      for (int64_t y = lo[1]; y < hi[1]; y++)
+        for (int64_t x = 2 * lo[0]; x < 2 * hi[0]; x++) {
+          const int64_t id = getId(x, y, z);
+          auto it = vac_hash.find(id);
+          if (it == vac_hash.end()) continue;
+          VacancyHash& vh = it->second;  // node reference survives rehash
+          int64_t n1[8];
+          int t1[8];
+          nn1(x, y, z, n1);
+          for (int b = 0; b < 8; b++) t1[b] = readout(n1[b]);
+          for (int b = 0; b < 8; b++) vh.nn_rates[b] = 0;
+          unsigned char avail = 0;
+          for (int b = 0; b < 8; b++)
+            if (t1[b] != V) avail |= (unsigned char)(1u << b);
+          vh.avail_trans_dir = avail;
+          for (int b = 0; b < 8; b++) {
+            if ((avail >> b) & 1) {
+              const double r = rate(id, n1[b], t1[b]);
+              sum_rates += r;
+              vh.nn_rates[b] = r;
+            }
+          }
+        }
+    sum_rates += 0.0;  // defectGenRate(): env::global_env.defect_gen_rate == 0
+    return sum_rates;
+  }
+
+  // ABVIModel::select (models/abvi/kmc.cpp:242-309): first (vacancy, dir) in
+  // sweep order whose running sum is > excepted_rate.
+  bool select(int s, double excepted_rate, int64_t* from, int64_t* to, int* dir) {
+    int64_t lo[3], hi[3];
+    region(s, lo, hi);
+    double acc = 0.0;
+    for (int64_t z = lo[2]; z < hi[2]; z++)
+      for (int64_t y = lo[1]; y < hi[1]; y++)
+        for (int64_t x = 2 * lo[0]; x < 2 * hi[0]; x++) {
+          const int64_t id = getId(x, y, z);
+          auto it = vac_hash.find(id);
+          if (it == vac_hash.end()) continue;
+          for (int b = 0; b < 8; b++) {
+            acc += it->second.nn_rates[b];
+            if (acc > excepted_rate) {
+              int64_t n1[8];
+              nn1(x, y, z, n1);  // LatListMeta::getIdBy1nnOffset
+              *from = id; *to = n1[b]; *dir = b;
+              return true;
+            }
+          }
+        }
+    return false;  // the reference returns event::DefectGen, which perform() asserts on
+  }
+
+  // ABVIModel::perform (models/abvi/kmc.cpp:358-396), vacancy branch.
+  void perform(int64_t from, int64_t to) {
+    auto it = vac_hash.find(from);
+    if (it == vac_hash.end()) { fprintf(stderr, "refcpu: perform without vacancy\n"); abort(); }
+    vac_hash.erase(it);
+    vac_hash.emplace(to, VacancyHash{});
+    auto ic = cu_hash.find(to), im = mn_hash.find(to), in = ni_hash.find(to);
+    if (ic != cu_hash.end()) { cu_hash.erase(ic); cu_hash.emplace(from); }
+    else if (im != mn_hash.end()) { mn_hash.erase(im); mn_hash.emplace(from); }
+    else if (in != ni_hash.end()) { ni_hash.erase(in); ni_hash.emplace(from); }
+  }
+
+  // LatticeTypes::randomAtomsType (src/lattice/lattice_types.cpp:9-19).
+  int randomAtomsType(int64_t hit) const {
+    int64_t rank_local = 0;
+    for (size_t i = 0; i < P.types.size(); i++) {
+      rank_local += P.ratio[i];
+      if (rank_local >= hit) return P.types[i];
+    }
+    return Fe;
+  }
+
+  // creation::createAtomsRandom + createRandom (src/creation.cpp:18-164).  One
+  // type draw per ghost-extended site in z,y,x order; ghost sites are later
+  // overwritten by their periodic images, so only real sites are stored.
+  void createRandom() {
+    int64_t ratio_total = 0;
+    for (size_t i = 0; i < P.types.size(); i++) ratio_total += P.ratio[i];
+    std::uniform_int_distribution<> types_dis(1, (int)ratio_total);
+    std::mt19937 types_rng((std::mt19937::result_type)P.seed_create_types);
+    for (int64_t z = 0; z < size_z; z++)
+      for (int64_t y = 0; y < size_y; y++)
+        for (int64_t x = 0; x < size_x; x++) {
+          const int64_t hit = types_dis(types_rng);
+          const int t = randomAtomsType(hit);
+          const bool real = x >= ghost_x && x < ghost_x + box_x && y >= ghost_y && y < ghost_y + box_y &&
+                            z >= ghost_z && z < ghost_z + box_z;
+          if (!real) continue;
+          const int64_t id = getId(x, y, z);
+          if (t == Cu) cu_hash.emplace(id);
+          else if (t == Ni) ni_hash.emplace(id);
+          else if (t == Mn) mn_hash.emplace(id);
+        }
+    const int64_t max_lattice_size = box_x * box_y * box_z;
+    std::uniform_int_distribution<long long> va_dis(0, max_lattice_size - 1);
+    std::mt19937 va_rng((std::mt19937::result_type)P.seed_create_vacancy);
+    for (int64_t va_i = 0; va_i < P.va_count;) {
+      int64_t local_id = va_dis(va_rng);
+      const int64_t x = local_id % box_x;
+      local_id = local_id / box_x;
+      const int64_t y = local_id % box_y;
+      const int64_t z = local_id / box_y;
+      const int64_t id = getId(x + ghost_x, y + ghost_y, z + ghost_z);
+      if (vac_hash.find(id) == vac_hash.end()) {
+        auto ic = cu_hash.find(id);
+        if (ic != cu_hash.end()) cu_hash.erase(ic);
+        vac_hash.emplace(id, VacancyHash{});
+        va_i++;
+      }
+    }
+  }
+
+  // SubLattice::startTimeLoop (src/algorithms/sl/sublattice.inl:25-190),
+  // sector ring order 0,7,2,5,3,4,1,6 (sublattice.h:21-30), time stream from
+  // SubLattice's constructor (sublattice.cpp:10-18).  Runs until the time
+  // loop ends or max_events events have been performed (mid-sector stop).
+  int run(int64_t max_events) {
+    static const int ring[8] = {0, 7, 2, 5, 3, 4, 1, 6};
+    const int64_t time_steps = (int64_t)ceil(P.physics_time / P.sector_T);
+    for (; step < time_steps; step++) {
+      for (; sect < 8; sect++) {
+        const int s = ring[sect];
+        if (!in_sector) {
+          thr = static_cast<double>(step + 1) * P.sector_T - evolution[s];
+          sector_time = 0.0;
+          in_sector = true;
+        }
+        while (sector_time < thr) {
+          if (events >= max_events) return 1;
+          const double total_rates = calcRates(s);
+          if (total_rates == 0.0 || std::abs(total_rates) < std::numeric_limits<double>::epsilon()) {
+            sector_time = thr;
+          } else {
+            // ModelAdapter::selectAndPerform (src/models/model_adapter.h:95-101),
+            // r::random() = rand32() * (1.0 / mt19937::max()) (random.cpp:35-41).
+            const double u = sel_rng() * (1.0 / std::mt19937::max());
+            int64_t from, to;
+            int dir;
+            if (!select(s, u * total_rates, &from, &to, &dir)) return -2;
+            perform(from, to);
+            if (log_from) { log_from->push_back(from); log_to->push_back(to); }
+            const double rnd = time_inc_dis(time_inc_rng);
+            const double delta_t = -std::log(rnd) / total_rates;
+            sector_time += delta_t;
+            events++;
+          }
+        }
+        evolution[s] += sector_time;
+        in_sector = false;
+      }
+      sect = 0;
+    }
+    return 0;
+  }
+
+  void seed_time() { time_inc_rng.seed((std::mt19937::result_type)P.seed_time_inc); }
+
+  Params P;
+  int64_t box_x, box_y, box_z, ghost_x, ghost_y, ghost_z, size_x, size_y, size_z;
+  std::unordered_map<int64_t, VacancyHash> vac_hash;
+  std::unordered_set<int64_t> cu_hash, mn_hash, ni_hash;
+  std::mt19937 sel_rng;  // r::rng: default-constructed, never seeded (rand_generators.cpp:8)
+  std::mt19937 time_inc_rng;
+  std::uniform_real_distribution<double> time_inc_dis{0, 1};
+  double evolution[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int64_t step = 0;
+  int sect = 0;
+  bool in_sector = false;
+  double thr = 0, sector_time = 0;
+  int64_t events = 0;
+  std::vector<int64_t>* log_from = nullptr;
+  std::vector<int64_t>* log_to = nullptr;
+};
+
+}  // namespace refcpu
+
+// ---------------------------------------------------------------- C ABI
+// Flag byte written by ref_get_sites(): bit0 V, bit1 Cu, bit2 Mn, bit3 Ni
+// (membership in vac/cu/mn/ni hashes); one byte per real site in
+// x + 2N*(y + N*z) order with x doubled.
+extern "C" {
+
+struct ref_params {
+  int64_t box[3];
+  int64_t ghost;
+  int32_t n_types;
+  int32_t types[32];
+  int64_t ratio[32];
+  int64_t va_count;
+  int64_t seed_create_types, seed_create_vacancy, seed_time_inc;
+  double temperature, attempt_freq, sector_T, physics_time;
+  uint32_t mask_2nn;
+};
+
+void* ref_create(const ref_params* p) {
+  refcpu::Params P;
+  for (int d = 0; d < 3; d++) P.box[d] = p->box[d];
+  P.ghost = p->ghost;
+  for (int i = 0; i < p->n_types; i++) { P.types.push_back(p->types[i]); P.ratio.push_back(p->ratio[i]); }
+  P.va_count = p->va_count;
+  P.seed_create_types = p->seed_create_types;
+  P.seed_create_vacancy = p->seed_create_vacancy;
+  P.seed_time_inc = p->seed_time_inc;
+  P.temperature = p->temperature;
+  P.attempt_freq = p->attempt_freq;
+  P.sector_T = p->sector_T;
+  P.physics_time = p->physics_time;
+  P.mask_2nn = p->mask_2nn;
+  auto* r = new refcpu::RefCpu(P);
+  r->seed_time();
+  return r;
+}
+
+void ref_destroy(void* h) { delete (refcpu::RefCpu*)h; }
+void ref_create_random(void* h) { ((refcpu::RefCpu*)h)->createRandom(); }
+
+int64_t ref_n_sites(void* h) {
+  auto* r = (refcpu::RefCpu*)h;
+  return r->box_x * r->box_y * r->box_z;
+}
+
+static int64_t real_to_local(refcpu::RefCpu* r, int64_t i) {
+  const int64_t x = i % r->box_x, y = (i / r->box_x) % r->box_y, z = i / (r->box_x * r->box_y);
+  return r->getId(x + r->ghost_x, y + r->ghost_y, z + r->ghost_z);
+}
+static int64_t local_to_real(refcpu::RefCpu* r, int64_t id) {
+  int64_t x, y, z;
+  r->coord(id, &x, &y, &z);
+  return (x - r->ghost_x) + r->box_x * ((y - r->ghost_y) + r->box_y * (z - r->ghost_z));
+}
+
+void ref_get_sites(void* h, uint8_t* out) {
+  auto* r = (refcpu::RefCpu*)h;
+  const int64_t n = ref_n_sites(h);
+  memset(out, 0, (size_t)n);
+  for (auto& kv : r->vac_hash) out[local_to_real(r, kv.first)] |= 1;
+  for (auto id : r->cu_hash) out[local_to_real(r, id)] |= 2;
+  for (auto id : r->mn_hash) out[local_to_real(r, id)] |= 4;
+  for (auto id : r->ni_hash) out[local_to_real(r, id)] |= 8;
+}
+
+void ref_set_sites(void* h, const uint8_t* in) {
+  auto* r = (refcpu::RefCpu*)h;
+  r->vac_hash.clear(); r->cu_hash.clear(); r->mn_hash.clear(); r->ni_hash.clear();
+  const int64_t n = ref_n_sites(h);
+  for (int64_t i = 0; i < n; i++) {
+    if (!in[i]) continue;
+    const int64_t id = real_to_local(r, i);
+    if (in[i] & 1) r->vac_hash.emplace(id, refcpu::VacancyHash{});
+    if (in[i] & 2) r->cu_hash.emplace(id);
+    if (in[i] & 4) r->mn_hash.emplace(id);
+    if (in[i] & 8) r->ni_hash.emplace(id);
+  }
+}
+
+// Runs up to max_events; logs (from, to) real-site indices when log arrays
+// are given (each of capacity max_events).  Returns 0 when the time loop
+// ended, 1 when the event cap stopped it, -2 when no event was found
+// (u * sum >= sum, the reference's DefectGen assert).
+int ref_run(void* h, int64_t max_events, int64_t* log_from, int64_t* log_to) {
+  auto* r = (refcpu::RefCpu*)h;
+  std::vector<int64_t> lf, lt;
+  if (log_from) { r->log_from = &lf; r->log_to = &lt; }
+  const int rc = r->run(r->events + max_events);
+  if (log_from) {
+    for (size_t i = 0; i < lf.size(); i++) { log_from[i] = local_to_real(r, lf[i]); log_to[i] = local_to_real(r, lt[i]); }
+  }
+  r->log_from = r->log_to = nullptr;
+  return rc;
+}
+
+// Clocks: evolution[8] by sector id, then step, ring position, in-sector flag,
+// current threshold and partial sector time, and the event count.
+void ref_state(void* h, double evolution[8], int64_t* step, int32_t* sect, int32_t* in_sector, double* thr,
+               double* sector_time, int64_t* events) {
+  auto* r = (refcpu::RefCpu*)h;
+  for (int i = 0; i < 8; i++) evolution[i] = r->evolution[i];
+  *step = r->step; *sect = r->sect; *in_sector = r->in_sector ? 1 : 0;
+  *thr = r->thr; *sector_time = r->sector_time; *events = r->events;
+}
+
+// Rates of every vacancy of sector s as computed by calcRates (which applies
+// its side effects); out holds 8 doubles per vacancy in sweep order, sites
+// the vacancy real-site indices.  Returns the vacancy count (capacity cap)
+// and writes the sector total.
+int64_t ref_sector_rates(void* h, int s, int64_t cap, int64_t* sites, double* out, double* total) {
+  auto* r = (refcpu::RefCpu*)h;
+  *total = r->calcRates(s);
+  int64_t lo[3], hi[3];
+  r->region(s, lo, hi);
+  int64_t n = 0;
+  for (int64_t z = lo[2]; z < hi[2]; z++)
+    for (int64_t y = lo[1]; y < hi[1]; y++)
+      for (int64_t x = 2 * lo[0]; x < 2 * hi[0]; x++) {
+        const int64_t id = r->getId(x, y, z);
+        auto it = r->vac_hash.find(id);
+        if (it == r->vac_hash.end()) continue;
+        if (n < cap) {
+          sites[n] = local_to_real(r, id);
+          for (int b = 0; b < 8; b++) out[8 * n + b] = it->second.nn_rates[b];
+        }
+        n++;
+      }
+  return n;
+}
+
+// deltaE of one (source vacancy, direction) pair through the hash path.
+double ref_delta_e(void* h, int64_t real_src, int dir) {
+  auto* r = (refcpu::RefCpu*)h;
+  const int64_t src = real_to_local(r, real_src);
+  int64_t x, y, z, n1[8];
+  r->coord(src, &x, &y, &z);
+  r->nn1(x, y, z, n1);
+  const int t = r->readout(n1[dir]);
+  return r->deltaE(src, n1[dir], t, t);
+}
+
+double ref_rate(void* h, int64_t real_src, int dir) {
+  auto* r = (refcpu::RefCpu*)h;
+  const int64_t src = real_to_local(r, real_src);
+  int64_t x, y, z, n1[8];
+  r->coord(src, &x, &y, &z);
+  r->nn1(x, y, z, n1);
+  const int t = r->readout(n1[dir]);
+  return r->rate(src, n1[dir], t);
+}
+
+}  // extern "C"

@@ -1,0 +1,159 @@
+"""ctypes wrapper of the oracle libraries (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, and only as the checker or the timed CPU baseline.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_BUILD = os.path.join(_HERE, "_build")
+
+# LatticeTypes::lat_type values (reference src/lattice/lattice_types.h:20-36).
+LAT_TYPES = {"V": 0, "Fe": 1, "Cu": 2, "Ni": 4, "Mn": 8,
+             "FeFe": 0x0101, "FeCu": 0x0102, "FeNi": 0x0104, "FeMn": 0x0108,
+             "CuCu": 0x0202, "CuNi": 0x0204, "CuMn": 0x0208,
+             "NiNi": 0x0404, "NiMn": 0x0408, "MnMn": 0x0808}
+
+
+class RefParams(ctypes.Structure):
+    _fields_ = [("box", ctypes.c_int64 * 3), ("ghost", ctypes.c_int64),
+                ("n_types", ctypes.c_int32), ("types", ctypes.c_int32 * 32),
+                ("ratio", ctypes.c_int64 * 32), ("va_count", ctypes.c_int64),
+                ("seed_create_types", ctypes.c_int64), ("seed_create_vacancy", ctypes.c_int64),
+                ("seed_time_inc", ctypes.c_int64),
+                ("temperature", ctypes.c_double), ("attempt_freq", ctypes.c_double),
+                ("sector_T", ctypes.c_double), ("physics_time", ctypes.c_double),
+                ("mask_2nn", ctypes.c_uint32)]
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _load(name):
+    path = os.path.join(_BUILD, name)
+    if not os.path.exists(path):
+        build()
+    return ctypes.CDLL(path)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = _load("librefcpu.so")
+        vp = ctypes.c_void_p
+        L.ref_create.restype = vp
+        L.ref_create.argtypes = [ctypes.POINTER(RefParams)]
+        L.ref_destroy.argtypes = [vp]
+        L.ref_create_random.argtypes = [vp]
+        L.ref_n_sites.restype = ctypes.c_int64
+        L.ref_n_sites.argtypes = [vp]
+        L.ref_get_sites.argtypes = [vp, ctypes.c_void_p]
+        L.ref_set_sites.argtypes = [vp, ctypes.c_void_p]
+        L.ref_run.restype = ctypes.c_int
+        L.ref_run.argtypes = [vp, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
+        L.ref_state.argtypes = [vp, ctypes.c_void_p] + [ctypes.c_void_p] * 6
+        L.ref_sector_rates.restype = ctypes.c_int64
+        L.ref_sector_rates.argtypes = [vp, ctypes.c_int, ctypes.c_int64, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_void_p]
+        L.ref_delta_e.restype = ctypes.c_double
+        L.ref_delta_e.argtypes = [vp, ctypes.c_int64, ctypes.c_int]
+        L.ref_rate.restype = ctypes.c_double
+        L.ref_rate.argtypes = [vp, ctypes.c_int64, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def make_params(cfg):
+    """cfg: dict with box, alloy (ordered list of (name, ratio)), va_count,
+    seeds (create_types, create_vacancy, time_inc), temperature,
+    attempt_freq, sector_T, physics_time, optional ghost / mask_2nn."""
+    p = RefParams()
+    for d in range(3):
+        p.box[d] = int(cfg["box"][d])
+    p.ghost = int(cfg.get("ghost", 2))
+    alloy = list(cfg["alloy"])
+    p.n_types = len(alloy)
+    for i, (name, ratio) in enumerate(alloy):
+        p.types[i] = LAT_TYPES[name]
+        p.ratio[i] = int(ratio)
+    p.va_count = int(cfg["va_count"])
+    p.seed_create_types = int(cfg["seeds"]["create_types"])
+    p.seed_create_vacancy = int(cfg["seeds"]["create_vacancy"])
+    p.seed_time_inc = int(cfg["seeds"]["time_inc"])
+    p.temperature = float(cfg["temperature"])
+    p.attempt_freq = float(cfg["attempt_freq"])
+    p.sector_T = float(cfg.get("sector_T", 3.0e-7))
+    p.physics_time = float(cfg["physics_time"])
+    p.mask_2nn = int(cfg.get("mask_2nn", 0x2F))
+    return p
+
+
+class RefCpu:
+    """The reference CPU build's event loop (hash containers, full sweeps)."""
+
+    def __init__(self, cfg, create=True):
+        self.L = lib()
+        self._p = make_params(cfg)
+        self.h = self.L.ref_create(ctypes.byref(self._p))
+        self.n_sites = self.L.ref_n_sites(self.h)
+        if create:
+            self.L.ref_create_random(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.ref_destroy(self.h)
+            self.h = None
+
+    def sites(self):
+        out = np.zeros(self.n_sites, dtype=np.uint8)
+        self.L.ref_get_sites(self.h, out.ctypes.data)
+        return out
+
+    def set_sites(self, sites):
+        sites = np.ascontiguousarray(sites, dtype=np.uint8)
+        assert sites.size == self.n_sites
+        self.L.ref_set_sites(self.h, sites.ctypes.data)
+
+    def run(self, max_events, log=False):
+        if log:
+            lf = np.zeros(max_events, dtype=np.int64)
+            lt = np.zeros(max_events, dtype=np.int64)
+            rc = self.L.ref_run(self.h, max_events, lf.ctypes.data, lt.ctypes.data)
+            n = self.state()["events"]
+            return rc, lf, lt
+        return self.L.ref_run(self.h, max_events, None, None), None, None
+
+    def state(self):
+        evo = np.zeros(8, dtype=np.float64)
+        step = ctypes.c_int64()
+        sect = ctypes.c_int32()
+        ins = ctypes.c_int32()
+        thr = ctypes.c_double()
+        st = ctypes.c_double()
+        ev = ctypes.c_int64()
+        self.L.ref_state(self.h, evo.ctypes.data, ctypes.byref(step), ctypes.byref(sect),
+                         ctypes.byref(ins), ctypes.byref(thr), ctypes.byref(st), ctypes.byref(ev))
+        return {"evolution": evo, "step": step.value, "sect": sect.value, "in_sector": ins.value,
+                "thr": thr.value, "sector_time": st.value, "events": ev.value}
+
+    def sector_rates(self, s, cap=1 << 16):
+        sites = np.zeros(cap, dtype=np.int64)
+        rates = np.zeros((cap, 8), dtype=np.float64)
+        tot = ctypes.c_double()
+        n = self.L.ref_sector_rates(self.h, s, cap, sites.ctypes.data, rates.ctypes.data, ctypes.byref(tot))
+        assert n <= cap
+        return sites[:n], rates[:n], tot.value
+
+    def delta_e(self, site, d):
+        return self.L.ref_delta_e(self.h, int(site), int(d))
+
+    def rate(self, site, d):
+        return self.L.ref_rate(self.h, int(site), int(d))
