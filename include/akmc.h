@@ -85,6 +85,9 @@ typedef struct akmc_stats {
   double seconds;            /* wall time of the last akmc_run */
   double kernel_seconds;     /* device time of the event-loop launches of the last akmc_run */
   int64_t launches;          /* event-loop launches of the last akmc_run */
+  int64_t n_eval;            /* vacancy evaluations since creation (55-site stencil gathers) */
+  int64_t n_fold;            /* vacancies whose running sum was folded from their 8 rates */
+  int64_t n_shift;           /* vacancies whose running sum was shifted by an exact delta */
 } akmc_stats;
 
 typedef struct akmc_engine akmc_engine;
@@ -138,6 +141,33 @@ int akmc_sector_rates(akmc_engine* e, int sector, int64_t cap, int64_t* sites, d
  * indices, for trajectory comparison.  capacity 0 disables it. */
 int akmc_set_event_log(akmc_engine* e, int64_t capacity);
 int akmc_get_event_log(akmc_engine* e, int64_t* from, int64_t* to, int64_t cap, int64_t* n);
+
+/* Events per random-number chunk (default and maximum 2^20).  A launch of
+ * the event loop consumes at most one chunk, so this sets the launch
+ * granularity.  Only before the first akmc_run. */
+int akmc_set_rng_chunk(akmc_engine* e, int64_t events);
+
+/* Events [first, first + n) of the two random streams akmc_run consumes, one
+ * pair per event: u = r::random() of a default-seeded mt19937
+ * (src/utils/random/random.cpp:35-41) and l = -log(r), r drawn by
+ * SubLattice's uniform_real_distribution on mt19937(seed_time_inc)
+ * (src/algorithms/sl/sublattice.inl:118-120). */
+int akmc_rng_streams(int64_t seed_time_inc, int64_t first, int64_t n, double* u, double* l);
+
+/* The exp of the rate kernel over an array, on the host and on a device:
+ * the table-driven algorithm that reproduces the C library's exp bit for bit
+ * (the reference's arrhenius, models/abvi/rate/rates_solver.cpp:25). */
+int akmc_exp_host(const double* x, double* y, int64_t n);
+int akmc_exp_device(int device, const double* x, double* y, int64_t n);
+
+/* Waits for all work on a device (benchmark bracketing). */
+int akmc_sync(int device);
+
+/* Copies `bytes` between two device buffers `iters` times with a plain
+ * streaming kernel and returns the average seconds per copy: a transfer of
+ * known size for calibrating the profiler's HBM byte counters.  No
+ * counterpart in the reference. */
+int akmc_copy_bench(int device, int64_t bytes, int iters, double* seconds);
 
 /* Frees the engine.  Replaces gpu_final (gpu/gpu_simulate.h:56). */
 void akmc_destroy(akmc_engine* e);

@@ -52,14 +52,16 @@ class Stats(ctypes.Structure):
                 ("in_sector", ctypes.c_int32), ("evolution", ctypes.c_double * 8),
                 ("sector_threshold", ctypes.c_double), ("sector_time", ctypes.c_double),
                 ("stop", ctypes.c_int32), ("reserved", ctypes.c_int32), ("seconds", ctypes.c_double),
-                ("kernel_seconds", ctypes.c_double), ("launches", ctypes.c_int64)]
+                ("kernel_seconds", ctypes.c_double), ("launches", ctypes.c_int64),
+                ("n_eval", ctypes.c_int64), ("n_fold", ctypes.c_int64), ("n_shift", ctypes.c_int64)]
 
     def as_dict(self):
         return {"events": self.events, "step": self.step, "sector_pos": self.sector_pos,
                 "in_sector": self.in_sector, "evolution": np.array(self.evolution[:]),
                 "sector_threshold": self.sector_threshold, "sector_time": self.sector_time,
                 "stop": self.stop, "seconds": self.seconds, "kernel_seconds": self.kernel_seconds,
-                "launches": self.launches}
+                "launches": self.launches, "n_eval": self.n_eval, "n_fold": self.n_fold,
+                "n_shift": self.n_shift}
 
 
 _lib = None
@@ -90,10 +92,51 @@ def lib():
     L.akmc_sector_rates.argtypes = [vp, i32, i64, vp, vp, P(dbl), P(i64)]
     L.akmc_set_event_log.argtypes = [vp, i64]
     L.akmc_get_event_log.argtypes = [vp, vp, vp, i64, P(i64)]
+    L.akmc_set_rng_chunk.argtypes = [vp, i64]
+    L.akmc_rng_streams.argtypes = [i64, i64, i64, vp, vp]
+    L.akmc_sync.argtypes = [i32]
+    L.akmc_copy_bench.argtypes = [i32, i64, i32, P(dbl)]
+    L.akmc_exp_host.argtypes = [vp, vp, i64]
+    L.akmc_exp_device.argtypes = [i32, vp, vp, i64]
     L.akmc_destroy.argtypes = [vp]
     L.akmc_destroy.restype = None
     _lib = L
     return L
+
+
+def rng_streams(seed_time_inc, first, n):
+    """(u, l) of events [first, first + n), as akmc_run consumes them."""
+    u = np.zeros(n, dtype=np.float64)
+    l = np.zeros(n, dtype=np.float64)
+    _check(lib().akmc_rng_streams(int(seed_time_inc), int(first), int(n), u.ctypes.data, l.ctypes.data))
+    return u, l
+
+
+def sync(device=0):
+    _check(lib().akmc_sync(int(device)))
+
+
+def copy_bench(nbytes, iters=10, device=0):
+    """Seconds per copy of nbytes with the library's streaming copy kernel."""
+    sec = ctypes.c_double()
+    _check(lib().akmc_copy_bench(int(device), int(nbytes), int(iters), ctypes.byref(sec)))
+    return sec.value
+
+
+def exp_host(x):
+    """The rate kernel's exp evaluated on the host (same algorithm as the device)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    _check(lib().akmc_exp_host(x.ctypes.data, y.ctypes.data, x.size))
+    return y
+
+
+def exp_device(x, device=0):
+    """The rate kernel's exp evaluated by a HIP kernel."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    _check(lib().akmc_exp_device(int(device), x.ctypes.data, y.ctypes.data, x.size))
+    return y
 
 
 def _check(rc):
@@ -199,6 +242,9 @@ class Engine:
                                        ctypes.byref(tot), ctypes.byref(n)))
         m = min(n.value, cap)
         return sites[:m], rates[:m], tot.value
+
+    def set_rng_chunk(self, events):
+        _check(lib().akmc_set_rng_chunk(self.h, int(events)))
 
     def set_event_log(self, capacity):
         _check(lib().akmc_set_event_log(self.h, int(capacity)))

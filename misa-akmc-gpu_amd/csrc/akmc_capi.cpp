@@ -24,6 +24,8 @@
 #include <vector>
 
 #include "../../include/akmc.h"
+#include "akmc_exp.h"
+#include "akmc_exp_table.h"
 #include "akmc_model.h"
 #include "akmc_plain.h"
 #include "akmc_yaml.h"
@@ -48,7 +50,7 @@ int fail(int code, const char* fmt, ...) {
     if (_e != hipSuccess) return fail(AKMC_ERR_HIP, "%s: %s", #expr, hipGetErrorString(_e)); \
   } while (0)
 
-constexpr int64_t kChunk = int64_t(1) << 20;  // events per random-number chunk
+constexpr int64_t kChunkMax = int64_t(1) << 20;  // events per random-number chunk (default and maximum)
 
 // Builds the 55-site rate stencil tables (see akmc_model.h).
 void build_stencil(akmc::StencilTables* S) {
@@ -134,6 +136,7 @@ struct akmc_engine {
   double* d_l[2] = {nullptr, nullptr};
   int64_t chunk_base[2] = {-1, -1};  // first event of each buffered chunk
   int64_t generated = 0;             // events whose random numbers were generated
+  int64_t chunk = kChunkMax;         // events per chunk (one launch consumes at most one chunk)
   int cur = 0;
   // event log
   uint32_t* d_log = nullptr;
@@ -149,15 +152,15 @@ int generate_chunk(akmc_engine* e, int buf) {
   const int64_t base = e->generated;
   double* u = e->h_u[buf];
   double* l = e->h_l[buf];
-  for (int64_t i = 0; i < kChunk; i++) {
+  for (int64_t i = 0; i < e->chunk; i++) {
     u[i] = e->sel_rng() * (1.0 / std::mt19937::max());
     const double r = e->time_dis(e->time_rng);
     l[i] = -std::log(r);
   }
   e->chunk_base[buf] = base;
-  e->generated = base + kChunk;
-  HIP_TRY(hipMemcpyAsync(e->d_u[buf], u, kChunk * sizeof(double), hipMemcpyHostToDevice, e->stream));
-  HIP_TRY(hipMemcpyAsync(e->d_l[buf], l, kChunk * sizeof(double), hipMemcpyHostToDevice, e->stream));
+  e->generated = base + e->chunk;
+  HIP_TRY(hipMemcpyAsync(e->d_u[buf], u, e->chunk * sizeof(double), hipMemcpyHostToDevice, e->stream));
+  HIP_TRY(hipMemcpyAsync(e->d_l[buf], l, e->chunk * sizeof(double), hipMemcpyHostToDevice, e->stream));
   return AKMC_OK;
 }
 
@@ -193,6 +196,9 @@ void fill_stats(akmc_engine* e, akmc_stats* s) {
   for (int i = 0; i < 8; i++) s->evolution[i] = st.evo[i];
   s->sector_threshold = st.thr;
   s->sector_time = st.t;
+  s->n_eval = st.n_eval;
+  s->n_fold = st.n_fold;
+  s->n_shift = st.n_shift;
 }
 
 akmc::PlainArgs base_args(akmc_engine* e) {
@@ -332,10 +338,10 @@ int akmc_create(const akmc_params* p, int device, akmc_engine** out) {
   HIP_TRY(hipMemset(e->d_sites, 0, (size_t)e->nsites));
   HIP_TRY(hipMalloc(&e->d_state, sizeof(akmc::PlainState)));
   for (int b = 0; b < 2; b++) {
-    HIP_TRY(hipHostMalloc(&e->h_u[b], kChunk * sizeof(double), hipHostMallocDefault));
-    HIP_TRY(hipHostMalloc(&e->h_l[b], kChunk * sizeof(double), hipHostMallocDefault));
-    HIP_TRY(hipMalloc(&e->d_u[b], kChunk * sizeof(double)));
-    HIP_TRY(hipMalloc(&e->d_l[b], kChunk * sizeof(double)));
+    HIP_TRY(hipHostMalloc(&e->h_u[b], kChunkMax * sizeof(double), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc(&e->h_l[b], kChunkMax * sizeof(double), hipHostMallocDefault));
+    HIP_TRY(hipMalloc(&e->d_u[b], kChunkMax * sizeof(double)));
+    HIP_TRY(hipMalloc(&e->d_l[b], kChunkMax * sizeof(double)));
   }
   memset(&e->h_state, 0, sizeof(e->h_state));
   HIP_TRY(hipMemcpy(e->d_state, &e->h_state, sizeof(e->h_state), hipMemcpyHostToDevice));
@@ -502,7 +508,7 @@ int akmc_run(akmc_engine* e, int64_t max_events, akmc_stats* stats) {
     // Make sure the chunk holding event `ev` is on the device.
     int buf = -1;
     for (int b = 0; b < 2; b++)
-      if (e->chunk_base[b] >= 0 && ev >= e->chunk_base[b] && ev < e->chunk_base[b] + kChunk) buf = b;
+      if (e->chunk_base[b] >= 0 && ev >= e->chunk_base[b] && ev < e->chunk_base[b] + e->chunk) buf = b;
     if (buf < 0) {
       buf = e->chunk_base[0] < 0 ? 0 : (e->chunk_base[0] < e->chunk_base[1] ? 0 : 1);
       int rc = generate_chunk(e, buf);
@@ -512,7 +518,7 @@ int akmc_run(akmc_engine* e, int64_t max_events, akmc_stats* stats) {
     a.u_stream = e->d_u[buf];
     a.l_stream = e->d_l[buf];
     a.rng_base = e->chunk_base[buf];
-    a.rng_end = e->chunk_base[buf] + kChunk;
+    a.rng_end = e->chunk_base[buf] + e->chunk;
     a.max_events = cap;
     a.log = e->d_log;
     a.log_base = e->log_base;
@@ -524,7 +530,7 @@ int akmc_run(akmc_engine* e, int64_t max_events, akmc_stats* stats) {
     // Overlap: produce the next chunk while this one is consumed.
     const int other = buf ^ 1;
     const bool need_next = e->chunk_base[other] < e->chunk_base[buf] || e->chunk_base[other] < 0;
-    if (need_next && e->generated == e->chunk_base[buf] + kChunk && cap > e->chunk_base[buf] + kChunk) {
+    if (need_next && e->generated == e->chunk_base[buf] + e->chunk && cap > e->chunk_base[buf] + e->chunk) {
       int rc = generate_chunk(e, other);
       if (rc) return rc;
     }
@@ -553,6 +559,86 @@ int akmc_run(akmc_engine* e, int64_t max_events, akmc_stats* stats) {
   e->last.launches = launches;
   if (stats) *stats = e->last;
   return AKMC_OK;
+}
+
+int akmc_set_rng_chunk(akmc_engine* e, int64_t events) {
+  if (!e) return fail(AKMC_ERR_INVALID, "null engine");
+  if (events < 1 || events > kChunkMax)
+    return fail(AKMC_ERR_INVALID, "chunk must be in [1, %lld]", (long long)kChunkMax);
+  if (e->generated != 0) return fail(AKMC_ERR_INVALID, "the chunk size can only be set before the first run");
+  e->chunk = events;
+  return AKMC_OK;
+}
+
+int akmc_rng_streams(int64_t seed_time_inc, int64_t first, int64_t n, double* u, double* l) {
+  if (first < 0 || n < 0 || ((!u || !l) && n > 0)) return fail(AKMC_ERR_INVALID, "bad argument");
+  std::mt19937 sel;
+  std::mt19937 tr((std::mt19937::result_type)(int64_t)(double)seed_time_inc);
+  std::uniform_real_distribution<double> dis(0.0, 1.0);
+  for (int64_t i = 0; i < first; i++) {
+    (void)sel();
+    (void)dis(tr);
+  }
+  for (int64_t i = 0; i < n; i++) {
+    u[i] = sel() * (1.0 / std::mt19937::max());
+    l[i] = -std::log(dis(tr));
+  }
+  return AKMC_OK;
+}
+
+int akmc_sync(int device) {
+  HIP_TRY(hipSetDevice(device));
+  HIP_TRY(hipDeviceSynchronize());
+  return AKMC_OK;
+}
+
+int akmc_copy_bench(int device, int64_t bytes, int iters, double* seconds) {
+  if (bytes < 16 || iters < 1) return fail(AKMC_ERR_INVALID, "bad argument");
+  HIP_TRY(hipSetDevice(device));
+  void *a = nullptr, *b = nullptr;
+  HIP_TRY(hipMalloc(&a, (size_t)bytes));
+  HIP_TRY(hipMalloc(&b, (size_t)bytes));
+  HIP_TRY(hipMemset(a, 1, (size_t)bytes));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  int rc = AKMC_OK;
+  HIP_TRY(hipEventRecord(e0, nullptr));
+  for (int i = 0; i < iters && rc == AKMC_OK; i++)
+    if (akmc_copy_launch(a, b, bytes, nullptr) != 0) rc = fail(AKMC_ERR_HIP, "copy launch failed");
+  HIP_TRY(hipEventRecord(e1, nullptr));
+  HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  if (seconds) *seconds = ms * 1e-3 / iters;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return rc;
+}
+
+int akmc_exp_host(const double* x, double* y, int64_t n) {
+  if ((!x || !y) && n > 0) return fail(AKMC_ERR_INVALID, "null argument");
+  for (int64_t i = 0; i < n; i++) y[i] = akmc::exp_glibc(x[i], akmc_exp_tab_init);
+  return AKMC_OK;
+}
+
+int akmc_exp_device(int device, const double* x, double* y, int64_t n) {
+  if ((!x || !y) && n > 0) return fail(AKMC_ERR_INVALID, "null argument");
+  if (n <= 0) return AKMC_OK;
+  HIP_TRY(hipSetDevice(device));
+  double *dx = nullptr, *dy = nullptr;
+  HIP_TRY(hipMalloc(&dx, (size_t)n * sizeof(double)));
+  HIP_TRY(hipMalloc(&dy, (size_t)n * sizeof(double)));
+  int rc = AKMC_OK;
+  if (hipMemcpy(dx, x, (size_t)n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      akmc_exp_launch(dx, dy, n, nullptr) != 0 || hipDeviceSynchronize() != hipSuccess ||
+      hipMemcpy(y, dy, (size_t)n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(AKMC_ERR_HIP, "device exp failed");
+  (void)hipFree(dx);
+  (void)hipFree(dy);
+  return rc;
 }
 
 int akmc_get_stats(akmc_engine* e, akmc_stats* stats) {

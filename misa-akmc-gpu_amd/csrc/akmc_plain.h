@@ -30,6 +30,10 @@ struct PlainState {
   int32_t in_sector;  // 1 while a sector is in progress
   int32_t status;     // AKMC_RUN_* of the last launch
   int32_t pad;
+  // Work counters (algorithmic traffic bookkeeping).
+  int64_t n_eval;     // vacancy evaluations (55-site stencil gathers)
+  int64_t n_fold;     // vacancies whose running sum was folded from their rates
+  int64_t n_shift;    // vacancies whose running sum was shifted
 };
 
 struct PlainArgs {
@@ -68,3 +72,5 @@ struct PlainArgs {
 extern "C" int akmc_plain_launch(const akmc::PlainArgs* args, hipStream_t stream);
 extern "C" int akmc_plain_set_stencil(const akmc::StencilTables* t);
 extern "C" int akmc_plain_vcap();
+extern "C" int akmc_exp_launch(const double* d_x, double* d_y, int64_t n, hipStream_t stream);
+extern "C" int akmc_copy_launch(const void* src, void* dst, int64_t bytes, hipStream_t stream);

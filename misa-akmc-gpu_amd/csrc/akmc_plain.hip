@@ -187,6 +187,8 @@ __global__ __launch_bounds__(64) void plain_kernel(PlainArgs a) {
       nv = collect_sector(S, L, s);
       if (nv < 0) { status = AKMC_RUN_ERR_CAPACITY; break; }
       eval_list<PLAIN_VCAP, double, false>(S, T, L, nullptr, 0, nv, nopatch, false);
+      st.n_eval += nv;
+      st.n_fold += nv;
       sum = fold_range<PLAIN_VCAP, double>(S, 0, nv, 0.0);
       rebuild = false;
       if (a.rates_sector >= 0) {
@@ -292,6 +294,7 @@ __global__ __launch_bounds__(64) void plain_kernel(PlainArgs a) {
         amax = max(amax, b0 + 63 - (int)__clzll(m));
         wave_sync();
         eval_list<PLAIN_VCAP, double, false>(S, T, L, S.list, 0, __popcll(m), pt, true);
+        st.n_eval += __popcll(m);
       }
     }
     // ---- running sums.
@@ -299,6 +302,7 @@ __global__ __launch_bounds__(64) void plain_kernel(PlainArgs a) {
       sum = 0.0;
     } else if (!stays) {
       const int c_lo = min(min(vsel, amin), nv);
+      st.n_fold += nv - c_lo;
       sum = c_lo < nv ? fold_range<PLAIN_VCAP, double>(S, c_lo, nv, c_lo > 0 ? S.pvac[c_lo - 1] : 0.0)
                       : S.pvac[nv - 1];
     } else {
@@ -306,6 +310,8 @@ __global__ __launch_bounds__(64) void plain_kernel(PlainArgs a) {
       const int c_hi = max(max(vsel, knew), amax);
       const double Po = S.pvac[c_hi];
       const double Pn = fold_range<PLAIN_VCAP, double>(S, c_lo, c_hi + 1, c_lo > 0 ? S.pvac[c_lo - 1] : 0.0);
+      st.n_fold += c_hi + 1 - c_lo;
+      st.n_shift += nv - (c_hi + 1);
       sum = c_hi + 1 < nv ? tail_update<PLAIN_VCAP, double>(S, c_hi + 1, nv, Pn, Po) : Pn;
     }
   }
@@ -329,3 +335,32 @@ extern "C" int akmc_plain_set_stencil(const akmc::StencilTables* t) {
 }
 
 extern "C" int akmc_plain_vcap() { return akmc::PLAIN_VCAP; }
+
+namespace akmc {
+// The device exp of the rate kernel over an array (verification hook).
+__global__ void exp_kernel(const double* x, double* y, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = exp_glibc(x[i], c_exp_tab);
+}
+}  // namespace akmc
+
+namespace akmc {
+// Plain streaming copy (HBM traffic calibration for the profiler counters).
+__global__ void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+}  // namespace akmc
+
+extern "C" int akmc_copy_launch(const void* src, void* dst, int64_t bytes, hipStream_t stream) {
+  const int64_t n = bytes / 16;
+  hipLaunchKernelGGL(akmc::copy_kernel, dim3(2048), dim3(256), 0, stream, (const uint4*)src, (uint4*)dst, n);
+  return (int)hipGetLastError();
+}
+
+extern "C" int akmc_exp_launch(const double* d_x, double* d_y, int64_t n, hipStream_t stream) {
+  if (n <= 0) return 0;
+  const int64_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(akmc::exp_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, d_x, d_y, n);
+  return (int)hipGetLastError();
+}

@@ -553,6 +553,43 @@ double ref_delta_e(void* h, int64_t real_src, int dir) {
   return r->deltaE(src, n1[dir], t, t);
 }
 
+// BondsCounter::count of a real site holding `type` (lat_type value).
+double ref_count(void* h, int64_t real_site, int type) {
+  auto* r = (refcpu::RefCpu*)h;
+  return r->count(real_to_local(r, real_site), type);
+}
+
+// BondsCounter::count_exchange: neighbours of now_site, with source_site read
+// as `atom`, bonded to src_atom_type.
+double ref_count_exchange(void* h, int64_t now_site, int64_t source_site, int atom, int src_atom_type) {
+  auto* r = (refcpu::RefCpu*)h;
+  return r->count_exchange(real_to_local(r, now_site), real_to_local(r, source_site), atom, src_atom_type);
+}
+
+int ref_random_atoms_type(void* h, int64_t hit) { return ((refcpu::RefCpu*)h)->randomAtomsType(hit); }
+int ref_make_bond(int a, int b) { return refcpu::make_bond(a, b); }
+double ref_bond1(int key) { return refcpu::bond1(key); }
+double ref_bond2(int key) { return refcpu::bond2(key); }
+
+// The C library's exp over an array (what RatesSolver::arrhenius calls).
+void ref_exp_array(const double* x, double* y, int64_t n) {
+  for (int64_t i = 0; i < n; i++) y[i] = exp(x[i]);
+}
+
+// The two random numbers the time loop consumes per event, in order:
+// u = r::random() on a default-constructed mt19937 (random.cpp:35-41) and
+// l = -log(r) with r from uniform_real_distribution<double>(0,1) on
+// mt19937(seed_time_inc) (sublattice.cpp:12-17, sublattice.inl:118-120).
+void ref_streams(int64_t seed_time_inc, int64_t n, double* u, double* l) {
+  std::mt19937 sel;
+  std::mt19937 tr((std::mt19937::result_type)seed_time_inc);
+  std::uniform_real_distribution<double> dis(0, 1);
+  for (int64_t i = 0; i < n; i++) {
+    u[i] = sel() * (1.0 / std::mt19937::max());
+    l[i] = -std::log(dis(tr));
+  }
+}
+
 double ref_rate(void* h, int64_t real_src, int dir) {
   auto* r = (refcpu::RefCpu*)h;
   const int64_t src = real_to_local(r, real_src);

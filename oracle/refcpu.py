@@ -67,6 +67,20 @@ def lib():
         L.ref_delta_e.argtypes = [vp, ctypes.c_int64, ctypes.c_int]
         L.ref_rate.restype = ctypes.c_double
         L.ref_rate.argtypes = [vp, ctypes.c_int64, ctypes.c_int]
+        L.ref_count.restype = ctypes.c_double
+        L.ref_count.argtypes = [vp, ctypes.c_int64, ctypes.c_int]
+        L.ref_count_exchange.restype = ctypes.c_double
+        L.ref_count_exchange.argtypes = [vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ctypes.c_int]
+        L.ref_random_atoms_type.argtypes = [vp, ctypes.c_int64]
+        L.ref_make_bond.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.ref_bond1.restype = ctypes.c_double
+        L.ref_bond1.argtypes = [ctypes.c_int]
+        L.ref_bond2.restype = ctypes.c_double
+        L.ref_bond2.argtypes = [ctypes.c_int]
+        L.ref_exp_array.argtypes = [vp, vp, ctypes.c_int64]
+        L.ref_exp_array.restype = None
+        L.ref_streams.argtypes = [ctypes.c_int64, ctypes.c_int64, vp, vp]
+        L.ref_streams.restype = None
         _lib = L
     return _lib
 
@@ -157,3 +171,60 @@ class RefCpu:
 
     def rate(self, site, d):
         return self.L.ref_rate(self.h, int(site), int(d))
+
+    def count(self, site, type_name):
+        return self.L.ref_count(self.h, int(site), LAT_TYPES[type_name])
+
+    def count_exchange(self, now_site, source_site, atom, src_atom_type):
+        return self.L.ref_count_exchange(self.h, int(now_site), int(source_site), LAT_TYPES[atom],
+                                         LAT_TYPES[src_atom_type])
+
+    def random_atoms_type(self, hit):
+        return self.L.ref_random_atoms_type(self.h, int(hit))
+
+    def site_index(self, x, y, z):
+        """Real-site index of doubled-x coordinates (periodic)."""
+        X, Y, Z = 2 * self._p.box[0], self._p.box[1], self._p.box[2]
+        return (x % X) + X * ((y % Y) + Y * (z % Z))
+
+
+def bond1(a, b):
+    L = lib()
+    return L.ref_bond1(L.ref_make_bond(LAT_TYPES[a], LAT_TYPES[b]))
+
+
+def bond2(a, b):
+    L = lib()
+    return L.ref_bond2(L.ref_make_bond(LAT_TYPES[a], LAT_TYPES[b]))
+
+
+_REF_SRC = os.path.join(_HERE, "_ref", "librefsrc.so")
+
+
+def refsrc():
+    """The reference's own lattice_types.cpp / pair_bond.hpp, compiled from
+    source (absent where the reference is not mounted at build time)."""
+    if not os.path.exists(_REF_SRC):
+        return None
+    L = ctypes.CDLL(_REF_SRC)
+    L.refsrc_random_atoms_type.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint, ctypes.c_int64]
+    L.refsrc_make_bond.argtypes = [ctypes.c_int, ctypes.c_int]
+    return L
+
+
+def exp_array(x):
+    """The C library's exp over an array."""
+    L = lib()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    L.ref_exp_array(x.ctypes.data, y.ctypes.data, ctypes.c_int64(x.size))
+    return y
+
+
+def streams(seed_time_inc, n):
+    """(u, -log r) of the first n events as the reference time loop draws them."""
+    L = lib()
+    u = np.zeros(n, dtype=np.float64)
+    l = np.zeros(n, dtype=np.float64)
+    L.ref_streams(ctypes.c_int64(seed_time_inc), ctypes.c_int64(n), u.ctypes.data, l.ctypes.data)
+    return u, l

@@ -1,0 +1,45 @@
+"""Writes the golden trajectory fixtures tests/golden/traj_*.npz from the
+oracle (oracle/refcpu.cpp).  Each fixture holds the configuration, the first
+events' (from, to) site indices, a SHA-256 of the final site array, the
+sector clocks and the partial sector time.  Run from the repository root:
+    python3 tests/golden/make_golden.py
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "..", "..", "oracle"))
+sys.path.insert(0, os.path.join(HERE, ".."))
+import refcpu  # noqa: E402
+from conftest import make_cfg  # noqa: E402
+
+CASES = {
+    "cfg0_40cube": (make_cfg(n=40, va=26, T=573.0, sector_T=3e-7), 3000),
+    "mnni_rich_16": (make_cfg(n=16, va=60, T=600.0, alloy=[("Fe", 6000), ("Mn", 2000), ("Ni", 1500), ("Cu", 500)],
+                              seeds=dict(create_types=7, create_vacancy=11, time_inc=13)), 3000),
+    "odd_box": (make_cfg(box=(17, 13, 11), va=20, T=520.0, sector_T=5e-7,
+                         seeds=dict(create_types=21, create_vacancy=22, time_inc=23)), 2000),
+}
+
+
+def main():
+    for name, (cfg, n) in CASES.items():
+        r = refcpu.RefCpu(cfg)
+        rc, lf, lt = r.run(n, log=True)
+        st = r.state()
+        sites = r.sites()
+        np.savez_compressed(os.path.join(HERE, "traj_%s.npz" % name),
+                            cfg=json.dumps(cfg), events=np.int64(st["events"]),
+                            from_=lf[:st["events"]].astype(np.int32), to=lt[:st["events"]].astype(np.int32),
+                            sites_sha256=hashlib.sha256(sites.tobytes()).hexdigest(),
+                            evolution=st["evolution"], sector_time=np.float64(st["sector_time"]),
+                            step=np.int64(st["step"]))
+        print(name, st["events"], "events")
+
+
+if __name__ == "__main__":
+    main()

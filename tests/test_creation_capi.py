@@ -1,0 +1,129 @@
+"""Host-side product logic on the CPU: initial occupancy (creation::createRandom,
+reference src/creation.cpp:18-164) bit-exact against the oracle, the
+config.yaml reader, and the C ABI surface (every symbol include/akmc.h
+declares is exported; errors come back as status codes)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import akmc
+import refcpu
+from conftest import ROOT, make_cfg
+
+CASES = [
+    dict(n=16, va=6),
+    dict(n=40, va=26),
+    dict(box=(17, 12, 9), va=11),
+    dict(n=24, va=500, alloy=[("Fe", 5000), ("Mn", 2500), ("Ni", 2500)]),
+    dict(n=20, va=40, alloy=[("Fe", 9598), ("Cu", 134), ("Mn", 134), ("Ni", 134), ("FeFe", 0), ("CuCu", 0)]),
+    dict(n=16, va=9, seeds=dict(create_types=1, create_vacancy=2, time_inc=3)),
+]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_random_sites_match_oracle(case):
+    cfg = make_cfg(**case)
+    a = akmc.random_sites(cfg)
+    b = refcpu.RefCpu(cfg).sites()
+    assert np.array_equal(a, b)
+    # invariants: exact vacancy count, no Cu tag under a vacancy
+    assert int((a & 1).sum()) == cfg["va_count"]
+    assert not np.any((a & 1) & ((a >> 1) & 1))
+
+
+def test_random_sites_composition():
+    cfg = make_cfg(n=48, va=100)
+    s = akmc.random_sites(cfg)
+    n = s.size
+    for bit, ppm in ((2, 134), (4, 130), (8, 70)):
+        frac = float(((s & bit) != 0).sum()) / n
+        assert abs(frac - ppm / 1e4) < 5 * np.sqrt(ppm / 1e4 / n) + 1e-4
+
+
+def test_stale_tags_exist_under_vacancies():
+    # Vacancy placement drops only a Cu tag (creation.cpp:149-158), so with a
+    # Mn/Ni-rich alloy some vacancies carry stale Mn/Ni tags.
+    cfg = make_cfg(n=24, va=500, alloy=[("Fe", 5000), ("Mn", 2500), ("Ni", 2500)])
+    s = akmc.random_sites(cfg)
+    v = s[(s & 1) != 0]
+    assert np.any(v & 4) and np.any(v & 8)
+
+
+def _declared_functions():
+    text = open(os.path.join(ROOT, "include", "akmc.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(akmc_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported():
+    L = akmc.lib()
+    names = _declared_functions()
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(L, name), name
+
+
+def test_yaml_config(tmp_path):
+    p = tmp_path / "config.yaml"
+    p.write_text("""# reference schema (examples/config.yaml)
+title: "parallel kMC configure file"
+box:
+  size: [40, 40, 40]
+  lattice_const: 2.855   # lattice constant
+  cutoff_radius: 5.6
+create:
+  random:
+    enable: true
+    va_count: 26
+    alloy:
+      Fe: 9666
+      Cu: 134
+      Mn: 130
+      Ni: 70
+      FeFe: 0
+  pipe:
+    enable: false
+seeds:
+  create_types: 100565
+  create_vacancy: 16650
+  event_selection: 563264
+  time_inc: 594689
+simulation:
+  temperature: 573
+  physics_time: 1e-3
+  attempt_freq: 6e12
+output:
+  dump:
+    interval: 0
+""")
+    prm = akmc.params_from_yaml(str(p))
+    assert list(prm.box) == [40, 40, 40]
+    assert prm.n_alloy == 5
+    assert [prm.alloy_type[i] for i in range(5)] == [1, 2, 8, 4, 0x101]
+    assert [prm.alloy_ratio[i] for i in range(5)] == [9666, 134, 130, 70, 0]
+    assert prm.va_count == 26 and prm.temperature == 573.0 and prm.physics_time == 1e-3
+    assert prm.seed_time_inc == 594689 and prm.sector_time == 3e-7
+    cfg = make_cfg(n=40, va=26, alloy=[("Fe", 9666), ("Cu", 134), ("Mn", 130), ("Ni", 70), ("FeFe", 0)])
+    cfg["lattice_const"] = 2.855
+    assert np.array_equal(akmc.random_sites(prm), refcpu.RefCpu(cfg).sites())
+
+
+def test_yaml_errors(tmp_path):
+    p = tmp_path / "bad.yaml"
+    p.write_text("box:\n  size: [4, 4]\n")
+    with pytest.raises(akmc.AkmcError) as e:
+        akmc.params_from_yaml(str(p))
+    assert e.value.code == -5
+    with pytest.raises(akmc.AkmcError):
+        akmc.params_from_yaml(str(tmp_path / "missing.yaml"))
+
+
+def test_invalid_arguments_return_status():
+    prm = akmc.params_from_config(make_cfg(n=16, va=2))
+    assert akmc.lib().akmc_random_sites(None, None, 0) == -1
+    assert b"null" in akmc.lib().akmc_last_error()
+    out = np.zeros(10, dtype=np.uint8)
+    rc = akmc.lib().akmc_random_sites(akmc.ctypes.byref(prm), out.ctypes.data, out.size)
+    assert rc == -1 and b"site count" in akmc.lib().akmc_last_error()

@@ -1,0 +1,191 @@
+"""Plain FP64 path on the GPU, through the C ABI, against the oracle.
+
+The bar is bit-exact: every rate, every event's (from, to), the final site
+array (including stale species tags), the 8 sector clocks and the partial
+sector time.  Full-size checks (the 256^3 benchmark box) compare a prefix of
+the trajectory with the oracle and the rest through invariants.
+"""
+import glob
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import akmc
+import refcpu
+from conftest import make_cfg
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
+
+
+def compare_run(cfg, n, chunk=None, pieces=1):
+    eng = akmc.Engine(cfg)
+    if chunk:
+        eng.set_rng_chunk(chunk)
+    eng.set_event_log(n)
+    ref = refcpu.RefCpu(cfg)
+    assert np.array_equal(eng.download(), ref.sites())
+    done = 0
+    for k in range(pieces):
+        m = n // pieces if k < pieces - 1 else n - done
+        st = eng.run(m)
+        done = st["events"]
+    rc, lf, lt = ref.run(n, log=True)
+    rs = ref.state()
+    gf, gt = eng.event_log(n)
+    assert st["events"] == rs["events"]
+    m = rs["events"]
+    assert np.array_equal(gf[:m], lf[:m]) and np.array_equal(gt[:m], lt[:m]), \
+        "first divergence at %s" % np.nonzero((gf[:m] != lf[:m]) | (gt[:m] != lt[:m]))[0][:3]
+    assert np.array_equal(eng.download(), ref.sites())
+    assert np.array_equal(_bits(st["evolution"]), _bits(rs["evolution"]))
+    assert st["sector_time"] == rs["sector_time"] and st["step"] == rs["step"]
+    assert st["sector_pos"] == rs["sect"] and st["in_sector"] == rs["in_sector"]
+    eng.close()
+    return st
+
+
+def test_device_exp_matches_libc():
+    say("test_device_exp_matches_libc")
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.uniform(-50, 0, 4_000_000), rng.uniform(-745.2, 709.8, 500_000)])
+    assert np.array_equal(_bits(akmc.exp_device(x)), _bits(refcpu.exp_array(x)))
+
+
+RATE_CASES = [
+    dict(n=16, va=40),
+    dict(n=24, va=300, alloy=[("Fe", 5000), ("Mn", 2500), ("Ni", 2500)]),
+    dict(box=(17, 12, 9), va=30, T=650.0),
+    dict(n=32, va=800, alloy=[("Fe", 8000), ("Cu", 1000), ("Mn", 500), ("Ni", 500)]),
+]
+
+
+@pytest.mark.parametrize("case", RATE_CASES)
+def test_sector_rates_bit_exact(case):
+    say("test_sector_rates_bit_exact", case)
+    cfg = make_cfg(**case)
+    eng = akmc.Engine(cfg)
+    ref = refcpu.RefCpu(cfg)
+    for s in range(8):
+        gs, gr, gt = eng.sector_rates(s)
+        rs, rr, rt = ref.sector_rates(s)
+        assert np.array_equal(gs, rs), s
+        assert np.array_equal(_bits(gr), _bits(rr)), s
+        assert _bits(np.array([gt]))[0] == _bits(np.array([rt]))[0], s
+    # the evaluation side effect (stale tags dropped) leaves identical lattices
+    assert np.array_equal(eng.download(), ref.sites())
+    eng.close()
+
+
+TRAJ_CASES = [
+    ("cfg0_40cube", make_cfg(n=40, va=26, T=573.0), 20000),
+    ("dense_24", make_cfg(n=24, va=200), 20000),
+    ("mnni_rich", make_cfg(n=16, va=60, T=600.0, alloy=[("Fe", 6000), ("Mn", 2000), ("Ni", 1500), ("Cu", 500)],
+                           seeds=dict(create_types=7, create_vacancy=11, time_inc=13)), 20000),
+    ("odd_box", make_cfg(box=(17, 13, 11), va=20, T=520.0, sector_T=5e-7,
+                         seeds=dict(create_types=21, create_vacancy=22, time_inc=23)), 10000),
+    ("hot_long_T", make_cfg(n=20, va=30, T=700.0, sector_T=2e-6), 20000),
+    ("cluster_heavy", make_cfg(n=16, va=400, T=560.0), 10000),
+]
+
+
+@pytest.mark.parametrize("name,cfg,n", TRAJ_CASES, ids=[c[0] for c in TRAJ_CASES])
+def test_trajectory_bit_exact(name, cfg, n):
+    say("test_trajectory_bit_exact", name, n)
+    compare_run(cfg, n)
+
+
+def test_resume_and_small_chunks():
+    say("test_resume_and_small_chunks")
+    cfg = make_cfg(n=24, va=120, T=590.0)
+    compare_run(cfg, 12000, pieces=7)       # event-cap stops mid-sector, then resumes
+    compare_run(cfg, 12000, chunk=97)      # a relaunch (full re-evaluation) every 97 events
+    compare_run(cfg, 3000, chunk=1)        # a relaunch at every event
+
+
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "traj_*.npz")))
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p) for p in GOLDEN])
+def test_golden_trajectories(path):
+    say("test_golden_trajectories", os.path.basename(path))
+    z = np.load(path, allow_pickle=False)
+    cfg = json.loads(str(z["cfg"]))
+    n = int(z["events"])
+    eng = akmc.Engine(cfg)
+    eng.set_event_log(n)
+    st = eng.run(n)
+    gf, gt = eng.event_log(n)
+    assert np.array_equal(gf, z["from_"]) and np.array_equal(gt, z["to"])
+    assert hashlib.sha256(eng.download().tobytes()).hexdigest() == str(z["sites_sha256"])
+    assert np.array_equal(_bits(st["evolution"]), _bits(z["evolution"]))
+    assert st["sector_time"] == float(z["sector_time"])
+
+
+def test_time_loop_end_and_empty_box():
+    say("test_time_loop_end_and_empty_box")
+    cfg = make_cfg(n=12, va=0, physics=3e-5, sector_T=3e-7)
+    eng = akmc.Engine(cfg)
+    st = eng.run(10)
+    assert st["stop"] == akmc.AKMC_STOP_TIME and st["events"] == 0 and st["step"] == 100
+    assert np.allclose(st["evolution"], 3e-5, rtol=1e-12)
+    ref = refcpu.RefCpu(cfg)
+    ref.run(10)
+    assert np.array_equal(_bits(st["evolution"]), _bits(ref.state()["evolution"]))
+    # a short physics time with vacancies: the loop ends before the cap
+    cfg = make_cfg(n=12, va=5, physics=2e-6, sector_T=3e-7)
+    eng = akmc.Engine(cfg)
+    st = eng.run(10 ** 7)
+    ref = refcpu.RefCpu(cfg)
+    ref.run(10 ** 7)
+    rs = ref.state()
+    assert st["stop"] == akmc.AKMC_STOP_TIME and st["events"] == rs["events"]
+    assert np.array_equal(_bits(st["evolution"]), _bits(rs["evolution"]))
+    assert np.array_equal(eng.download(), ref.sites())
+
+
+def test_sector_capacity_error():
+    say("test_sector_capacity_error")
+    cfg = make_cfg(n=32, va=12000)  # ~1500 vacancies per sector > 1024
+    eng = akmc.Engine(cfg)
+    with pytest.raises(akmc.AkmcError) as e:
+        eng.run(10)
+    assert e.value.code == -3
+
+
+def test_full_size_prefix_and_invariants():
+    """The 256^3 benchmark box: the first 150 events bit-exact against the
+    oracle, then 300k more events through invariants."""
+    say("test_full_size_prefix_and_invariants")
+    cfg = make_cfg(n=256, va=6711, T=573.0, sector_T=1e-6, physics=1.0)
+    eng = akmc.Engine(cfg)
+    eng.set_event_log(150)
+    s0 = eng.download()
+    ref = refcpu.RefCpu(cfg)
+    assert np.array_equal(s0, ref.sites())
+    st = eng.run(150)
+    rc, lf, lt = ref.run(150, log=True)
+    gf, gt = eng.event_log(150)
+    assert np.array_equal(gf, lf) and np.array_equal(gt, lt)
+    assert np.array_equal(eng.download(), ref.sites())
+    assert st["sector_time"] == ref.state()["sector_time"]
+    del ref
+    st = eng.run(300_000)
+    s1 = eng.download()
+    assert st["events"] == 300_150
+    assert (s1 & 1).sum() == (s0 & 1).sum() == 6711
+    assert ((s1 >> 1) & 1).sum() == ((s0 >> 1) & 1).sum()
+    assert s1.max() < 16
+    eng.close()
