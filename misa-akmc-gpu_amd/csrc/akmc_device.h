@@ -55,8 +55,10 @@ __device__ __forceinline__ bool in_stencil(const BoxDims& d, const StencilTables
   const int dx = pdiff((int)key_x(q), (int)key_x(w), d.X);
   const int dy = pdiff((int)key_y(q), (int)key_y(w), d.Y);
   const int dz = pdiff((int)key_z(q), (int)key_z(w), d.Z);
-  if (dx < -3 || dx > 3 || dy < -2 || dy > 2 || dz < -2 || dz > 2) return false;
-  return (S.reach[key_x(w) & 1][dz + 2][dy + 2] >> (dx + 3)) & 1;
+  // Branch-free: every lane evaluates the same instructions.
+  const bool in_box = (dx >= -3) & (dx <= 3) & (dy >= -2) & (dy <= 2) & (dz >= -2) & (dz <= 2);
+  const int iz = min(max(dz + 2, 0), 4), iy = min(max(dy + 2, 0), 4), ix = min(max(dx + 3, 0), 7);
+  return in_box & (((S.reach[key_x(w) & 1][iz][iy] >> ix) & 1) != 0);
 }
 
 // ---------------------------------------------------------------- rates

@@ -106,15 +106,15 @@ __device__ void move_entry(PlainSector& S, int nv, int from_i, int to_i) {
     const int r = delta < 0 ? rr : rounds - 1 - rr;
     const int v = lo + r * 64 + l;
     const bool act = v <= hi;
-    uint32_t k = 0; int32_t sl = 0; uint8_t o = 0; uint64_t tm = 0;
-    double rt[8]; uint8_t nb[8];
+    const int vc = min(v, hi);
+    const uint32_t k = S.key[vc];
+    const int32_t sl = S.slot[vc];
+    const uint8_t o = S.own[vc];
+    const uint64_t tm = S.tmask[vc];
+    double rt[8];
+    uint8_t nb[8];
 #pragma unroll
-    for (int b = 0; b < 8; b++) { rt[b] = 0.0; nb[b] = 0; }
-    if (act) {
-      k = S.key[v]; sl = S.slot[v]; o = S.own[v]; tm = S.tmask[v];
-#pragma unroll
-      for (int b = 0; b < 8; b++) { rt[b] = S.rate[8 * v + b]; nb[b] = S.nb1[8 * v + b]; }
-    }
+    for (int b = 0; b < 8; b++) { rt[b] = S.rate[8 * vc + b]; nb[b] = S.nb1[8 * vc + b]; }
     wave_sync();
     if (act) {
       const int w = v + delta;
@@ -160,8 +160,6 @@ __global__ __launch_bounds__(64) void plain_kernel(PlainArgs a) {
       if (kRing[i] == a.rates_sector) st.sect = i;
   }
   for (;;) {
-    // Make this wave's earlier global writes visible to its later reads.
-    __threadfence_block();
     if (!st.in_sector) {
       if (st.step >= a.time_steps) { status = AKMC_RUN_TIME_DONE; break; }
       if (++visits > (int64_t)1 << 16) { status = AKMC_RUN_YIELD; break; }
@@ -184,9 +182,13 @@ __global__ __launch_bounds__(64) void plain_kernel(PlainArgs a) {
     if (st.events >= a.rng_end) { status = AKMC_RUN_NEED_RNG; break; }
 
     if (rebuild) {
+      // The whole sector is re-read from HBM: make this wave's earlier
+      // writes visible first.  (Within a sector the engine never reads back
+      // a byte it wrote in the same event; those are patched from registers.)
+      __threadfence_block();
       nv = collect_sector(S, L, s);
       if (nv < 0) { status = AKMC_RUN_ERR_CAPACITY; break; }
-      eval_list<PLAIN_VCAP, double, false>(S, T, L, nullptr, 0, nv, nopatch, false);
+      eval_list<PLAIN_VCAP, double, false>(S, T, L, nullptr, 0, nv, nopatch, false, 0, -1);
       st.n_eval += nv;
       st.n_fold += nv;
       sum = fold_range<PLAIN_VCAP, double>(S, 0, nv, 0.0);
@@ -215,10 +217,14 @@ __global__ __launch_bounds__(64) void plain_kernel(PlainArgs a) {
     if (vsel >= nv) { status = AKMC_RUN_ERR_NO_EVENT; break; }
     int bsel = -1;
     {
+      double r8[8];
+#pragma unroll
+      for (int b = 0; b < 8; b++) r8[b] = S.rate[vsel * 8 + b];
       double p = vsel == 0 ? 0.0 : S.pvac[vsel - 1];
+#pragma unroll
       for (int b = 0; b < 8; b++) {
-        p += S.rate[vsel * 8 + b];
-        if (p > target) { bsel = b; break; }
+        p += r8[b];
+        if (bsel < 0 && p > target) bsel = b;
       }
     }
     if (bsel < 0) { status = AKMC_RUN_ERR_NO_EVENT; break; }
@@ -230,6 +236,16 @@ __global__ __launch_bounds__(64) void plain_kernel(PlainArgs a) {
     int ox, oy, oz;
     nn1_offset(par, bsel, &ox, &oy, &oz);
     const uint32_t to = key_add(L.d, from, ox, oy, oz);
+    // Gather the stencil around `to` now (lane = stencil site); the bytes are
+    // used when the hopped vacancy is re-evaluated below, after the global
+    // latency has been hidden behind the list update.
+    uint8_t pf = 0;
+    {
+      const int pt_par = par ^ 1;
+      if (l < T.st.n_sites)
+        pf = L.sites[key_index(L.d, key_add(L.d, to, T.st.off[pt_par][l][0], T.st.off[pt_par][l][1],
+                                             T.st.off[pt_par][l][2]))];
+    }
     const uint8_t ff = S.own[vsel];
     const uint8_t gt = S.nb1[vsel * 8 + bsel];
     const uint8_t xb = (gt & BIT_CU) ? BIT_CU : (gt & BIT_MN) ? BIT_MN : (gt & BIT_NI) ? BIT_NI : 0;
@@ -259,8 +275,20 @@ __global__ __launch_bounds__(64) void plain_kernel(PlainArgs a) {
     const bool stays = key_sector(L.d, to) == s;
     int knew = -1;
     if (stays) {
-      const int lb = lower_bound_key<PLAIN_VCAP>(S.key, nv, to);
-      knew = lb > vsel ? lb - 1 : lb;  // rank of `to` among the other keys
+      // Rank of `to` among the other keys; the hop moves at most one z-plane,
+      // so look in a window around the old position first.
+      const int w0 = max(0, min(vsel - 32, nv - 64));
+      const int wv = w0 + l;
+      const uint32_t kw = S.key[min(wv, nv - 1)];
+      const uint64_t below = ballot((wv < nv) & (wv != vsel) & (kw < to));
+      // Every key before the window is below `to` and every key after it above.
+      const bool inside = (w0 == 0 || S.key[w0 - 1] < to) && (w0 + 64 >= nv || S.key[w0 + 64] > to);
+      if (inside) {
+        knew = w0 + __popcll(below) - (vsel < w0 ? 1 : 0);
+      } else {
+        const int lb = lower_bound_key(S.key, nv, to);
+        knew = lb > vsel ? lb - 1 : lb;
+      }
     }
     move_entry(S, nv, vsel, stays ? knew : -1);
     if (stays) {
@@ -274,17 +302,29 @@ __global__ __launch_bounds__(64) void plain_kernel(PlainArgs a) {
     // wraps around the box.
     const int zf = (int)key_z(from);
     const int zlo = min(zf, zf + oz) - 2, zhi = max(zf, zf + oz) + 2;
-    const int ilo = zlo <= 0 ? 0 : lower_bound_key<PLAIN_VCAP>(S.key, nv, make_key(0, 0, (uint32_t)zlo));
-    const int ihi =
-        zhi + 1 >= L.d.Z ? nv : lower_bound_key<PLAIN_VCAP>(S.key, nv, make_key(0, 0, (uint32_t)(zhi + 1)));
+    int ilo, ihi;
+    {
+      // A 64-entry window around the hop usually holds every candidate.
+      const int pos = stays ? knew : vsel;
+      const int c0 = max(0, min(pos - 32, nv - 64));
+      const bool covers = (c0 == 0 || (int)key_z(S.key[c0 - 1]) < zlo) &&
+                          (c0 + 64 >= nv || (int)key_z(S.key[c0 + 64]) > zhi);
+      if (covers) {
+        ilo = c0;
+        ihi = min(nv, c0 + 64);
+      } else {
+        ilo = zlo <= 0 ? 0 : lower_bound_key(S.key, nv, make_key(0, 0, (uint32_t)zlo));
+        ihi = zhi + 1 >= L.d.Z ? nv : lower_bound_key(S.key, nv, make_key(0, 0, (uint32_t)(zhi + 1)));
+      }
+    }
     int amin = 0x7FFFFFFF, amax = -1;
     const Patch pt{from, to, nf_from, nf_to, stays ? knew : -1};
     for (int b0 = ilo; b0 < ihi; b0 += 64) {
       const int v = b0 + l;
       bool hit = false;
-      if (v < ihi) {
-        const uint32_t w = S.key[v];
-        hit = in_stencil(L.d, T.st, w, from) || in_stencil(L.d, T.st, w, to);
+      {
+        const uint32_t w = S.key[min(v, ihi - 1)];
+        hit = (v < ihi) & (in_stencil(L.d, T.st, w, from) | in_stencil(L.d, T.st, w, to));
       }
       const uint64_t m = ballot(hit);
       if (m) {
@@ -293,7 +333,7 @@ __global__ __launch_bounds__(64) void plain_kernel(PlainArgs a) {
         amin = min(amin, b0 + first_set(m));
         amax = max(amax, b0 + 63 - (int)__clzll(m));
         wave_sync();
-        eval_list<PLAIN_VCAP, double, false>(S, T, L, S.list, 0, __popcll(m), pt, true);
+        eval_list<PLAIN_VCAP, double, false>(S, T, L, S.list, 0, __popcll(m), pt, true, pf, stays ? knew : -1);
         st.n_eval += __popcll(m);
       }
     }

@@ -3,9 +3,9 @@
 //
 // The event chain of one lattice is serial (every hop changes the next
 // rates), so the engine is written for latency, not throughput: one wave, no
-// workgroup barriers, cross-lane work through ballots and shuffles, loads
-// issued together and consumed late.  The plain path runs one engine on the
-// whole GPU; the batched path runs one engine per replica, many per CU.
+// workgroup barriers, cross-lane work through ballots, DPP and readlane,
+// loads issued together and consumed late.  The plain path runs one engine on
+// the whole GPU; the batched path runs one engine per replica, many per CU.
 //
 // Exactness of the running sums.  The reference folds the sector's rates in
 // sweep order, sum_rates += rate (models/abvi/kmc.cpp:103), and select()
@@ -40,20 +40,38 @@ __device__ __forceinline__ void wave_sync() {
 }
 __device__ __forceinline__ uint64_t ballot(bool p) { return (uint64_t)__ballot(p); }
 __device__ __forceinline__ int first_set(uint64_t m) { return m ? (int)__ffsll((unsigned long long)m) - 1 : 64; }
-__device__ __forceinline__ double shfl_d(double v, int src) { return __shfl(v, src, 64); }
-__device__ __forceinline__ int shfl_i(int v, int src) { return __shfl(v, src, 64); }
+__device__ __forceinline__ int last_set(uint64_t m) { return m ? 63 - (int)__clzll((unsigned long long)m) : -1; }
 
+// Value of v in lane `lane` (a wave-uniform index).
+__device__ __forceinline__ double bcast_d(double v, int lane) {
+  const uint64_t b = as_u64(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, lane);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), lane);
+  return as_f64(((uint64_t)hi << 32) | lo);
+}
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_d(double v) {
+  const uint64_t b = as_u64(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, ROW_MASK, 0xF, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, ROW_MASK, 0xF, false);
+  return as_f64(((uint64_t)hi << 32) | lo);
+}
+
+// Inclusive scan across the wave (Hillis-Steele inside each 16-lane row with
+// row_shr 1/2/4/8, then row_bcast 15/31 across rows).  Exact for integers
+// whose totals stay below 2^53, which is all it is used for.
 __device__ __forceinline__ double wave_incl_scan(double v) {
-  const int l = lane_id();
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const double y = __shfl_up(v, o, 64);
-    if (l >= o) v += y;
-  }
+  v += dpp_d<0x111, 0xF>(v);  // row_shr:1
+  v += dpp_d<0x112, 0xF>(v);  // row_shr:2
+  v += dpp_d<0x114, 0xF>(v);  // row_shr:4
+  v += dpp_d<0x118, 0xF>(v);  // row_shr:8
+  v += dpp_d<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_d<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
   return v;
 }
 
-__device__ __forceinline__ int exponent_of(double p) {  // biased-free exponent of a positive normal
+__device__ __forceinline__ int exponent_of(double p) {  // unbiased exponent of a positive normal
   return (int)((as_u64(p) >> 52) & 0x7FF) - 1023;
 }
 
@@ -65,18 +83,18 @@ __device__ __forceinline__ int lsb_exponent(double a) {
   if (ex == 0) return -1074 + (int)__builtin_ctzll(m);
   return ex - 1075 + (int)__builtin_ctzll(m | (1ull << 52));
 }
-// Tie-mask bit of a rate: bit (lsb + 60) for lsb in [-60, 3].
-__device__ __forceinline__ uint64_t tie_bit(double a) {
-  if (!(a > 0.0)) return 0;
+// Tie-mask bit index of a rate: lsb + 60 for lsb in [-60, 3], else -1.
+__device__ __forceinline__ int tie_index(double a) {
+  if (!(a > 0.0)) return -1;
   const int L = lsb_exponent(a);
-  return (L >= -60 && L <= 3) ? (1ull << (L + 60)) : 0;
+  return ((L >= -60) & (L <= 3)) ? L + 60 : -1;
 }
 // True if folding a vacancy with this tie mask inside binade e cannot hit a
 // rounding tie (no rate is an odd multiple of 2^(e-53)).
 __device__ __forceinline__ bool tie_free(uint64_t mask, int e) {
   const int t = e - 53 + 60;
-  if (t < 0 || t > 63) return false;
-  return !((mask >> t) & 1);
+  const bool in_window = (t >= 0) & (t <= 63);
+  return in_window & !((mask >> (t & 63)) & 1);
 }
 
 // ---------------------------------------------------------------- state
@@ -93,6 +111,7 @@ struct SectorLDS {
   uint8_t nb1[VCAP * 8];   // flag bytes of its 8 1nn sites
   uint8_t sten[EVB][STENCIL_MAX];
   uint8_t cflag[EVB];      // gathered flag byte of each batch vacancy's own site
+  int8_t tieb[EVB][8];     // tie-mask bit index of each batch rate
   int32_t list[64];
   int32_t nv;
   int32_t pad;
@@ -124,16 +143,17 @@ struct Patch {
 
 // ---------------------------------------------------------------- searches
 // First index i in [0, n) with key[i] >= target, n if none (key sorted).
-template <int VCAP>
-__device__ int lower_bound_key(const uint32_t* key, int n, uint32_t target) {
+__device__ __forceinline__ int lower_bound_key(const uint32_t* key, int n, uint32_t target) {
   if (n <= 0) return 0;
   const int l = lane_id();
   const int S = (n + 63) >> 6;
   const int ci = min(n - 1, l * S + S - 1);
-  const int f = first_set(ballot(l * S < n && key[ci] >= target));
+  const uint32_t kc = key[ci];
+  const int f = first_set(ballot((l * S < n) & (kc >= target)));
   if (f == 64) return n;
   const int fi = f * S + l;
-  const int g = first_set(ballot(l < S && fi < n && key[min(fi, n - 1)] >= target));
+  const uint32_t kf = key[min(fi, n - 1)];
+  const int g = first_set(ballot((l < S) & (fi < n) & (kf >= target)));
   return g == 64 ? min(n, (f + 1) * S) : f * S + g;
 }
 
@@ -143,10 +163,12 @@ __device__ __forceinline__ int first_above(const double* pvac, int n, double tar
   const int l = lane_id();
   const int S = (n + 63) >> 6;
   const int ci = min(n - 1, l * S + S - 1);
-  const int f = first_set(ballot(l * S < n && pvac[ci] > target));
+  const double pc = pvac[ci];
+  const int f = first_set(ballot((l * S < n) & (pc > target)));
   if (f == 64) return n;
   const int fi = f * S + l;
-  const int g = first_set(ballot(l < S && fi < n && pvac[min(fi, n - 1)] > target));
+  const double pf = pvac[min(fi, n - 1)];
+  const int g = first_set(ballot((l < S) & (fi < n) & (pf > target)));
   return g == 64 ? min(n, (f + 1) * S) : f * S + g;
 }
 
@@ -158,11 +180,14 @@ __device__ double fold_range(SectorLDS<VCAP, RateT>& S, int v0, int v1, double P
   const int l = lane_id();
   int v = v0;
   while (v < v1) {
-    if (!(P >= 0x1p-969)) {
-      // Zero (or tiny) running sum: fold one vacancy serially.
+    if (v1 - v <= 2 || !(P >= 0x1p-969)) {
+      // Short remainder or zero/tiny running sum: one vacancy serially.
+      double a[8];
+#pragma unroll
+      for (int b = 0; b < 8; b++) a[b] = (double)S.rate[8 * v + b];
       double p = P;
 #pragma unroll
-      for (int b = 0; b < 8; b++) p += (double)S.rate[8 * v + b];
+      for (int b = 0; b < 8; b++) p += a[b];
       if (l == 0) S.pvac[v] = p;
       P = p;
       v++;
@@ -177,44 +202,42 @@ __device__ double fold_range(SectorLDS<VCAP, RateT>& S, int v0, int v1, double P
     const int vc = active ? vi : v;
     double a[8];
 #pragma unroll
-    for (int b = 0; b < 8; b++) a[b] = active ? (double)S.rate[8 * vc + b] : 0.0;
+    for (int b = 0; b < 8; b++) a[b] = (double)S.rate[8 * vc + b];
+#pragma unroll
+    for (int b = 0; b < 8; b++) a[b] = active ? a[b] : 0.0;
     double q[8];
-    double local = 0.0;
-    int bad = 8;
+    bool bad = false;
 #pragma unroll
     for (int b = 0; b < 8; b++) {
-      double r = 0.0;
-      if (!(a[b] < limit)) {
-        r = 0x1p53;
-        bad = min(bad, b);
-      } else {
-        const double f = a[b] * inv_u;
-        r = __builtin_rint(f);
-        const double fr = f - r;
-        if (fr == 0.5 || fr == -0.5) bad = min(bad, b);
-      }
-      q[b] = r;
-      local += r;
+      // A rate outside the binade forces the serial path; so does a tie.
+      const bool inb = a[b] < limit;
+      const double f = inb ? a[b] * inv_u : 0.0;
+      const double r = __builtin_rint(f);
+      const double fr = f - r;
+      bad = bad | !inb | (fr == 0.5) | (fr == -0.5);
+      q[b] = inb ? r : 0x1p53;
     }
+    // Integer sums below 2^53 are exact in any order: a depth-3 tree.
+    const double local = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7]));
     const double incl = wave_incl_scan(local);
-    double R = incl - local;
-#pragma unroll
-    for (int b = 0; b < 8; b++) {
-      R += q[b];
-      if (b < bad && !(P + R * u < limit)) bad = b;
-    }
-    const int fb = first_set(ballot(active && bad < 8));
-    if (active && l < fb) S.pvac[vi] = P + incl * u;
+    // The running sum grows monotonically, so the lane's last slot decides
+    // whether all of its slots stayed inside the binade.
+    bad = bad | !(P + incl * u < limit);
+    const int fb = first_set(ballot(active & bad));
+    if (active & (l < fb)) S.pvac[vi] = P + incl * u;
     if (fb == 64) {
       const int last = min(63, v1 - 1 - v);
-      P = P + shfl_d(incl, last) * u;
+      P = P + bcast_d(incl, last) * u;
       v += last + 1;
     } else {
-      const double excl = shfl_d(incl - local, fb);
+      const double excl = bcast_d(incl - local, fb);
       double p = P + excl * u;  // exact: every slot before this vacancy stayed in the binade
       const int vb = v + fb;
+      double ab[8];
 #pragma unroll
-      for (int b = 0; b < 8; b++) p += (double)S.rate[8 * vb + b];
+      for (int b = 0; b < 8; b++) ab[b] = (double)S.rate[8 * vb + b];
+#pragma unroll
+      for (int b = 0; b < 8; b++) p += ab[b];
       if (l == 0) S.pvac[vb] = p;
       P = p;
       v = vb + 1;
@@ -225,102 +248,125 @@ __device__ double fold_range(SectorLDS<VCAP, RateT>& S, int v0, int v1, double P
 }
 
 // Shifts the running sums of vacancies [v0, n) whose rates did not change
-// after the sums before them moved from Po to Pn (pvac[v0 - 1] new = Pn, old
-// = Po).  pvac[v] still holds old values on entry for v >= v0.  Returns the
-// new total (pvac[n - 1]).
+// after the sum before them moved from Po to Pn.  pvac[v] still holds old
+// values on entry for v >= v0.  Returns the new total (pvac[n - 1]).
+// Works on groups of 256 vacancies: all old sums of a group are loaded
+// before anything of it is written, so the checks need no cross-lane chain.
 template <int VCAP, typename RateT>
 __device__ double tail_update(SectorLDS<VCAP, RateT>& S, int v0, int n, double Pn, double Po) {
   const int l = lane_id();
-  double prev_old = Po, prev_new = Pn;
   double D = Pn - Po;
+  if (!(Po + D == Pn && Pn - D == Po)) return fold_range<VCAP, RateT>(S, v0, n, Pn);
   int v = v0;
-  if (!(Po + D == Pn && Pn - D == Po)) {
-    // The shift itself is not exact: fold the tail exactly.
-    return n > v0 ? fold_range<VCAP, RateT>(S, v0, n, Pn) : Pn;
-  }
-  while (v < n && D != 0.0) {
-    const int vi = v + l;
-    const bool active = vi < n;
-    const double Eo = active ? S.pvac[vi] : 0.0;
-    const uint64_t tm = active ? S.tmask[vi] : 0;
-    double So = __shfl_up(Eo, 1, 64);
-    if (l == 0) So = prev_old;
-    const double Sn = So + D, En = Eo + D;
-    const int e = exponent_of(So);
-    const bool ok = So >= 0x1p-969 && exponent_of(Eo) == e && exponent_of(Sn) == e && exponent_of(En) == e &&
-                    tie_free(tm, e);
-    const int fb = first_set(ballot(active && !ok));
-    if (active && l < fb) S.pvac[vi] = En;
-    if (fb == 64) {
-      const int last = min(63, n - 1 - v);
-      prev_old = shfl_d(Eo, last);
-      prev_new = shfl_d(En, last);
-      v += last + 1;
-    } else {
-      // Exact fold of the offending vacancy from its new start.
-      const int vb = v + fb;
-      double p = fb == 0 ? prev_new : shfl_d(En, fb - 1);
+  double Sold = Po;   // old running sum before v
+  double Pprev = Pn;  // new running sum before v
+  while (v < n) {
+    if (D == 0.0) {
+      wave_sync();
+      return S.pvac[n - 1];  // the rest is unchanged
+    }
+    constexpr int G = 4;
+    double Eo[G], So[G];
+    uint64_t tm[G];
 #pragma unroll
-      for (int b = 0; b < 8; b++) p += (double)S.rate[8 * vb + b];
-      const double Eob = shfl_d(Eo, fb);
-      if (l == 0) S.pvac[vb] = p;
-      prev_old = Eob;
-      prev_new = p;
-      D = p - Eob;
-      v = vb + 1;
-      if (!(Eob + D == p && p - D == Eob)) {
-        wave_sync();
-        return v < n ? fold_range<VCAP, RateT>(S, v, n, p) : p;
-      }
+    for (int r = 0; r < G; r++) {
+      const int vc = min(v + 64 * r + l, n - 1);  // v >= 1 here, so vc - 1 >= 0
+      Eo[r] = S.pvac[vc];
+      So[r] = S.pvac[vc - 1];
+      tm[r] = S.tmask[vc];
+    }
+    if (l == 0) So[0] = Sold;
+    const int gend = min(n, v + 64 * G);
+    const double Eo_last = S.pvac[gend - 1];
+    int fail = gend;
+#pragma unroll
+    for (int r = 0; r < G; r++) {
+      const int vi = v + 64 * r + l;
+      const double Sn = So[r] + D, En = Eo[r] + D;
+      const int e = exponent_of(So[r]);
+      const bool ok = (So[r] >= 0x1p-969) & (exponent_of(Eo[r]) == e) & (exponent_of(Sn) == e) &
+                      (exponent_of(En) == e) & tie_free(tm[r], e);
+      const uint64_t m = ballot((vi < n) & !ok);
+      if (fail == gend && m) fail = v + 64 * r + first_set(m);
+    }
+#pragma unroll
+    for (int r = 0; r < G; r++) {
+      const int vi = v + 64 * r + l;
+      if (vi < fail) S.pvac[vi] = Eo[r] + D;
+    }
+    if (fail == gend) {
+      Sold = Eo_last;
+      Pprev = Eo_last + D;
+      v = gend;
+      continue;
+    }
+    // Exact fold of the next 64 vacancies from the offending one: where the
+    // new sums crossed a binade boundary ahead of the old ones, every vacancy
+    // until the old sums cross too would fail the shift check.
+    wave_sync();
+    const int fend = min(n, fail + 64);
+    const double Eo_end = S.pvac[fend - 1];
+    const double pstart = fail == v ? Pprev : S.pvac[fail - 1];
+    const double p = fold_range<VCAP, RateT>(S, fail, fend, pstart);
+    D = p - Eo_end;
+    Sold = Eo_end;
+    Pprev = p;
+    v = fend;
+    if (!(Eo_end + D == p && p - D == Eo_end)) {
+      wave_sync();
+      return v < n ? fold_range<VCAP, RateT>(S, v, n, p) : p;
     }
   }
   wave_sync();
-  if (v >= n) return prev_new;
-  // D == 0: the remaining sums are unchanged.
-  return S.pvac[n - 1];
+  return Pprev;
 }
 
 // ---------------------------------------------------------------- rates
 // Evaluates the rates of the vacancies idx[0..n) (sorted indices; idx ==
-// nullptr means base + i), as the
-// reference's calcRates evaluates a vacancy: 8 direction rates, plus the
-// side effect of deltaE on the vacancy's own species tags (a tag equal to an
-// available neighbour's species is dropped, vacancy_rates_solver.cpp:81-163).
-// Global bytes at the patch keys are replaced by the patch values (the hop
-// that was just written may not be visible yet).
+// nullptr means base + i), as the reference's calcRates evaluates a vacancy:
+// 8 direction rates, plus the side effect of deltaE on the vacancy's own
+// species tags (a tag equal to an available neighbour's species is dropped,
+// vacancy_rates_solver.cpp:81-163).  Global bytes at the patch keys are
+// replaced by the patch values (the hop that was just written may not be
+// visible yet).  pf/pf_vi: this lane's stencil byte (site index = lane) of
+// vacancy pf_vi, gathered earlier by the caller (pf_vi < 0: none).
 template <int VCAP, typename RateT, bool FP32>
 __device__ void eval_list(SectorLDS<VCAP, RateT>& S, const SharedTables& T, const LatticeView& L, const int32_t* idx,
-                          int base, int n, const Patch& pt, bool own_from_lds) {
+                          int base, int n, const Patch& pt, bool own_from_lds, uint8_t pf, int pf_vi) {
   const int l = lane_id();
   const int ns = T.st.n_sites;
   for (int bs = 0; bs < n; bs += EVB) {
     const int bc = min(EVB, n - bs);
-    // Gather: pairs (a, k) = (vacancy in batch, stencil site), 7 per lane.
-    constexpr int PER = (EVB * STENCIL_MAX + 63) / 64;
-    uint8_t f[PER];
+    // Gather: load i is (vacancy i of the batch, stencil site = lane).
+    uint32_t f[EVB];
+    uint32_t qk[EVB];
+    const int lk = min(l, ns - 1);
 #pragma unroll
-    for (int i = 0; i < PER; i++) {
-      const int pidx = l + 64 * i;
-      const int a = pidx / STENCIL_MAX, k = pidx % STENCIL_MAX;
-      f[i] = 0;
-      if (a < bc && k < ns) {
-        const uint32_t w = S.key[idx ? idx[bs + a] : base + bs + a];
-        const int par = (int)(key_x(w) & 1);
-        const uint32_t q = key_add(L.d, w, T.st.off[par][k][0], T.st.off[par][k][1], T.st.off[par][k][2]);
-        f[i] = L.sites[key_index(L.d, q)];
-        if (q == pt.k0) f[i] = pt.f0;
-        if (q == pt.k1) f[i] = pt.f1;
-      }
+    for (int i = 0; i < EVB; i++) {
+      const int vv = idx ? idx[bs + min(i, bc - 1)] : base + bs + min(i, bc - 1);
+      const uint32_t w = S.key[vv];
+      const int par = (int)(key_x(w) & 1);
+      qk[i] = key_add(L.d, w, T.st.off[par][lk][0], T.st.off[par][lk][1], T.st.off[par][lk][2]);
+    }
+    // Wave-uniform conditions only, so the loads issue back to back.
+#pragma unroll
+    for (int i = 0; i < EVB; i++) {
+      const int vv = idx ? idx[bs + min(i, bc - 1)] : base + bs + min(i, bc - 1);
+      f[i] = pf;
+      if (i < bc && vv != pf_vi) f[i] = L.sites[key_index(L.d, qk[i])];
     }
 #pragma unroll
-    for (int i = 0; i < PER; i++) {
-      const int pidx = l + 64 * i;
-      const int a = pidx / STENCIL_MAX, k = pidx % STENCIL_MAX;
-      if (a < bc && k < ns) {
-        const int vv = idx ? idx[bs + a] : base + bs + a;
-        S.sten[a][k] = (uint8_t)readout(f[i]);
-        if (k >= 1 && k <= 8) S.nb1[vv * 8 + (k - 1)] = f[i];
-        if (k == 0) S.cflag[a] = f[i];
+    for (int i = 0; i < EVB; i++) {
+      if (qk[i] == pt.k0) f[i] = pt.f0;
+      if (qk[i] == pt.k1) f[i] = pt.f1;
+    }
+#pragma unroll
+    for (int i = 0; i < EVB; i++) {
+      if (i < bc && l < ns) {
+        const int vv = idx ? idx[bs + i] : base + bs + i;
+        S.sten[i][l] = (uint8_t)readout((uint8_t)f[i]);
+        if (l >= 1 && l <= 8) S.nb1[vv * 8 + (l - 1)] = (uint8_t)f[i];
+        if (l == 0) S.cflag[i] = (uint8_t)f[i];
       }
     }
     wave_sync();
@@ -329,25 +375,27 @@ __device__ void eval_list(SectorLDS<VCAP, RateT>& S, const SharedTables& T, cons
     const bool act = a < bc;
     const int vi = act ? (idx ? idx[bs + a] : base + bs + a) : 0;
     double r = 0.0;
-    uint32_t clr = 0;
+    int X = SP_V;
     if (act) {
       const int par = (int)(key_x(S.key[vi]) & 1);
+      X = S.sten[a][T.st.w1[par][b]];
       if constexpr (FP32) {
-        r = 0.0;  // replaced by the FP32 path of the batched engine
+        r = 0.0;  // the FP32 rate is the batched engine's (akmc_batch.hip)
       } else {
         r = rate_fp64(S.sten[a], par, b, T.st, T.e1, T.e2, T.e0, L.nu, L.kT, T.exp_tab);
       }
-      clr = species_bit(S.sten[a][T.st.w1[par][b]]);
       S.rate[vi * 8 + b] = (RateT)r;
+      S.tieb[a][b] = (int8_t)tie_index((double)(RateT)r);
     }
-    uint64_t tb = tie_bit((double)(RateT)r);
-    clr |= __shfl_xor(clr, 1, 64);
-    clr |= __shfl_xor(clr, 2, 64);
-    clr |= __shfl_xor(clr, 4, 64);
-    tb |= __shfl_xor(tb, 1, 64);
-    tb |= __shfl_xor(tb, 2, 64);
-    tb |= __shfl_xor(tb, 4, 64);
+    // Side effect: species present among the available targets of each vacancy.
+    const uint64_t mcu = ballot(act & (X == SP_CU));
+    const uint64_t mmn = ballot(act & (X == SP_MN));
+    const uint64_t mni = ballot(act & (X == SP_NI));
+    wave_sync();
     if (act && b == 0) {
+      const uint32_t sh = 8u * (uint32_t)a;
+      const uint8_t clr = (uint8_t)((((mcu >> sh) & 0xFF) ? BIT_CU : 0) | (((mmn >> sh) & 0xFF) ? BIT_MN : 0) |
+                                    (((mni >> sh) & 0xFF) ? BIT_NI : 0));
       // The vacancy's own flags: after a hop, the LDS copy (kept current by
       // this engine) or the patch value for the vacancy that just arrived; at
       // a rebuild, the gathered byte.
@@ -355,7 +403,13 @@ __device__ void eval_list(SectorLDS<VCAP, RateT>& S, const SharedTables& T, cons
       const uint8_t nf = (uint8_t)(f0 & ~clr);
       if (nf != f0) L.sites[key_index(L.d, S.key[vi])] = nf;
       S.own[vi] = nf;
-      S.tmask[vi] = tb;
+      uint64_t tm = 0;
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const int t = S.tieb[a][k];
+        if (t >= 0) tm |= 1ull << t;
+      }
+      S.tmask[vi] = tm;
     }
     wave_sync();
   }
