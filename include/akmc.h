@@ -172,6 +172,191 @@ int akmc_copy_bench(int device, int64_t bytes, int iters, double* seconds);
 /* Frees the engine.  Replaces gpu_final (gpu/gpu_simulate.h:56). */
 void akmc_destroy(akmc_engine* e);
 
+/* ------------------------------------------------------------------------
+ * Replica engine: batches of independent lattices.
+ *
+ * The reference runs one lattice per MPI rank and has no batched form; these
+ * entry points stand where a driver would otherwise loop over simulation::
+ * simulate (src/simulation.h:61-85) once per replica.  Each replica follows
+ * the reference's physics (stencil, bond counting, rate, perform rule,
+ * sublattice time loop) with its own xoshiro256** stream (seeded from
+ * seed_time_inc) and FP64 block sums, so trajectories are reproducible per
+ * replica but not the reference's bit for bit; the single-lattice engine
+ * above is the bit-exact path.  A replica's trajectory depends only on its
+ * parameters, initial lattice and seed -- not on the batch it runs in, the
+ * slicing of runs or the kernel tuning.
+ * ---------------------------------------------------------------------- */
+
+/* Initial occupancy of a replica. */
+#define AKMC_INIT_DEVICE 0     /* counter-based draw on the device: each site independently takes a type with
+                                  probability ratio / sum(ratio); then va_count uniform site draws become
+                                  vacancies (Cu tag dropped, as creation.cpp:131-164) */
+#define AKMC_INIT_REFERENCE 1  /* creation::createRandom on the host, bit-exact with akmc_random_sites */
+
+/* Replica status. */
+#define AKMC_REP_RUNNING 0
+#define AKMC_REP_TIME_DONE 1    /* ceil(physics_time / T) time steps finished */
+#define AKMC_REP_TARGET_DONE 2  /* target event count reached */
+
+typedef struct akmc_batch_options {
+  int32_t precision;        /* AKMC_PREC_FP32 (default) or AKMC_PREC_FP64 */
+  int32_t init;             /* AKMC_INIT_* */
+  int32_t error_stats;      /* 1: FP32 rates are also evaluated in FP64 and their error recorded */
+  int32_t use_graph;        /* 1: launches replay a captured hipGraph */
+  int32_t lds_vcap;         /* tuning: replicas with at most this many vacancies keep their sector in
+                               LDS (0 = autotune, -1 = none) */
+  int32_t waves_per_group;  /* tuning: replicas per workgroup, 1, 2 or 4 (0 = autotune) */
+  int32_t order;            /* tuning: 0 replica order, 1 largest first (-1 = autotune) */
+  int32_t reserved;
+  int64_t slice_events;     /* events per replica per launch (0 = default 1024) */
+} akmc_batch_options;
+
+typedef struct akmc_batch_stats {
+  int64_t events;           /* events of all replicas in the last akmc_batch_run */
+  double seconds;           /* its wall time */
+  double kernel_seconds;    /* device time of its launches */
+  int64_t launches;         /* slice launches (graph replays) */
+  int32_t running;          /* replicas still running afterwards */
+  int32_t failed;           /* replicas stopped by an error */
+} akmc_batch_stats;
+
+typedef struct akmc_replica_stats {
+  int64_t events;
+  int64_t step;
+  int32_t status;           /* AKMC_REP_*, or a negative error */
+  int32_t sector_pos;
+  double evolution[8];
+  double sector_time;
+  int64_t nvac;
+  int64_t n_eval;           /* vacancy evaluations */
+  int64_t n_rebuild;        /* sector starts */
+  int64_t n_jump;           /* sector visits with a zero total rate */
+  double err_max;           /* FP32 rate error vs FP64 (error_stats): max relative error */
+  double err_mean;
+  int64_t err_n;            /* rates compared */
+  int64_t err_tiny;         /* FP64 rates below FLT_MIN, excluded */
+} akmc_replica_stats;
+
+typedef struct akmc_batch_tuning {
+  int32_t lds_vcap;
+  int32_t waves_per_group;
+  int32_t order;
+  int32_t reserved;
+  int64_t slice_events;
+  double events_per_second; /* best trial */
+  int32_t trials;
+  int32_t reserved2;
+} akmc_batch_tuning;
+
+typedef struct akmc_batch akmc_batch;
+
+int akmc_batch_options_default(akmc_batch_options* o);
+
+/* Creates n replicas on `device`, one per params entry (box, alloy, va_count,
+ * seeds, temperature, attempt_freq, sector_time, physics_time; the
+ * precision field is ignored in favour of o->precision). */
+int akmc_batch_create(const akmc_params* replicas, int32_t n, const akmc_batch_options* o, int device,
+                      akmc_batch** out);
+
+/* Advances every running replica by up to events_per_replica events (fewer
+ * when its time loop ends). */
+int akmc_batch_run(akmc_batch* b, int64_t events_per_replica, akmc_batch_stats* stats);
+
+/* Searches the tuning space with short slices of real work (the replicas
+ * advance; their trajectories do not depend on the tuning) and keeps the
+ * fastest setting.  trial_events: events per replica per trial. */
+int akmc_batch_tune(akmc_batch* b, int64_t trial_events, akmc_batch_tuning* chosen);
+int akmc_batch_get_tuning(akmc_batch* b, akmc_batch_tuning* t);
+
+int32_t akmc_batch_size(const akmc_batch* b);
+int64_t akmc_batch_num_sites(const akmc_batch* b, int32_t replica);
+int akmc_batch_replica_stats(akmc_batch* b, int32_t replica, akmc_replica_stats* s);
+int akmc_batch_download(akmc_batch* b, int32_t replica, uint8_t* sites, int64_t n);
+/* Replaces a replica's lattice (vacancies <= its va_count) and restarts its
+ * current sector. */
+int akmc_batch_upload(akmc_batch* b, int32_t replica, const uint8_t* sites, int64_t n);
+
+/* Parity hook: the rates of every vacancy of one sector of one replica, as
+ * a sector start computes them (with the stale-tag side effect), in
+ * vacancy-list order, and the FP64 sum of their per-vacancy totals. */
+int akmc_batch_sector_rates(akmc_batch* b, int32_t replica, int sector, int64_t cap, int64_t* sites, double* rates,
+                            double* total, int64_t* n);
+
+void akmc_batch_destroy(akmc_batch* b);
+
+/* The FP32 rate table the replica engine uses at temperature T (K) and
+ * attempt frequency nu: the rate of barrier index m (Ea = m * q / 2 eV) is
+ * tab[256 + ((m - m_min) >> 8)] * tab[(m - m_min) & 255] in FP32.  Writes
+ * 256 + nhi floats (cap must allow it), the energy quantum q and the bond
+ * tables in units of q (e1q, e2q: 25 each, row-major V,Fe,Cu,Ni,Mn; e0q2: 5,
+ * 2*e0/q).  Host only; a verification hook. */
+int akmc_fp32_rate_table(double temperature, double nu, float* tab, int32_t cap, int32_t* nhi, int32_t* m_min,
+                         double* q, float* e1q, float* e2q, float* e0q2);
+
+/* ------------------------------------------------------------------------
+ * Serving: a request loop over the replica engine.  Jobs (one lattice each)
+ * are admitted in arrival order while the HBM budget and the slot count
+ * allow, stay device-resident while they run, and advance together: every
+ * launch carries all live jobs, and jobs that arrive are added at the next
+ * launch (continuous batching).  A job's result does not depend on the jobs
+ * it ran beside.  No counterpart in the reference.
+ * ---------------------------------------------------------------------- */
+
+#define AKMC_JOB_PENDING 0
+#define AKMC_JOB_RUNNING 1
+#define AKMC_JOB_DONE 2
+#define AKMC_JOB_FAILED 3
+
+typedef struct akmc_server_options {
+  int64_t hbm_budget;       /* bytes of job state admitted at once (default 250e9) */
+  int32_t max_live;         /* jobs on the device at once (default 4096) */
+  int32_t precision;        /* AKMC_PREC_FP32 (default) or AKMC_PREC_FP64 */
+  int32_t lds_vcap;         /* as akmc_batch_options (0: 256) */
+  int32_t waves_per_group;  /* (0: 2) */
+  int64_t slice_events;     /* events per job per launch (default 4096) */
+  double slice_seconds;     /* launch time budget (default 2e-3; 0: none) */
+  int32_t keep_results;     /* 1: copy each finished lattice to host memory */
+  int32_t reserved;
+} akmc_server_options;
+
+typedef struct akmc_job_info {
+  int32_t status;           /* AKMC_JOB_* */
+  int32_t rep_status;       /* AKMC_REP_* of the replica */
+  int64_t events;
+  int64_t bytes;            /* HBM bytes the job holds while live */
+  double t_submit;          /* server clock, seconds */
+  double t_admit;
+  double t_done;
+} akmc_job_info;
+
+typedef struct akmc_server_stats {
+  int64_t submitted, admitted, done, failed;
+  int64_t launches;
+  int64_t events;           /* events of all jobs so far */
+  int64_t live;             /* jobs on the device now */
+  int64_t bytes_live;       /* their HBM bytes */
+  int64_t bytes_peak;
+  double busy_seconds;      /* server time spent in launches */
+} akmc_server_stats;
+
+typedef struct akmc_server akmc_server;
+
+int akmc_server_options_default(akmc_server_options* o);
+/* Starts the request loop (a host thread) on `device`. */
+int akmc_server_create(int device, const akmc_server_options* o, akmc_server** out);
+/* Queues a job that stops after target_events events (or at the end of its
+ * time loop).  Fails with AKMC_ERR_BUDGET if the job alone exceeds the
+ * budget.  Thread-safe. */
+int akmc_server_submit(akmc_server* s, const akmc_params* p, int64_t target_events, int64_t* job);
+int akmc_server_job_info(akmc_server* s, int64_t job, akmc_job_info* info);
+/* The finished lattice of a job (keep_results). */
+int akmc_server_result(akmc_server* s, int64_t job, uint8_t* sites, int64_t n);
+/* Waits until no job is pending or running (timeout_s <= 0: no limit). */
+int akmc_server_wait_idle(akmc_server* s, double timeout_s);
+int akmc_server_get_stats(akmc_server* s, akmc_server_stats* st);
+double akmc_server_clock(akmc_server* s);
+void akmc_server_destroy(akmc_server* s);
+
 const char* akmc_last_error(void);
 
 #ifdef __cplusplus

@@ -256,3 +256,254 @@ class Engine:
         _check(lib().akmc_get_event_log(self.h, f.ctypes.data, t.ctypes.data, cap, ctypes.byref(n)))
         m = min(n.value, cap)
         return f[:m], t[:m]
+
+
+# ---------------------------------------------------------------- replica engine
+AKMC_INIT_DEVICE = 0
+AKMC_INIT_REFERENCE = 1
+REP_RUNNING, REP_TIME_DONE, REP_TARGET_DONE = 0, 1, 2
+JOB_PENDING, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
+
+
+class BatchOptions(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int32), ("init", ctypes.c_int32), ("error_stats", ctypes.c_int32),
+                ("use_graph", ctypes.c_int32), ("lds_vcap", ctypes.c_int32), ("waves_per_group", ctypes.c_int32),
+                ("order", ctypes.c_int32), ("reserved", ctypes.c_int32), ("slice_events", ctypes.c_int64)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("events", ctypes.c_int64), ("seconds", ctypes.c_double), ("kernel_seconds", ctypes.c_double),
+                ("launches", ctypes.c_int64), ("running", ctypes.c_int32), ("failed", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ReplicaStats(ctypes.Structure):
+    _fields_ = [("events", ctypes.c_int64), ("step", ctypes.c_int64), ("status", ctypes.c_int32),
+                ("sector_pos", ctypes.c_int32), ("evolution", ctypes.c_double * 8), ("sector_time", ctypes.c_double),
+                ("nvac", ctypes.c_int64), ("n_eval", ctypes.c_int64), ("n_rebuild", ctypes.c_int64),
+                ("n_jump", ctypes.c_int64), ("err_max", ctypes.c_double), ("err_mean", ctypes.c_double),
+                ("err_n", ctypes.c_int64), ("err_tiny", ctypes.c_int64)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["evolution"] = np.array(self.evolution[:])
+        return d
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("lds_vcap", ctypes.c_int32), ("waves_per_group", ctypes.c_int32), ("order", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("slice_events", ctypes.c_int64),
+                ("events_per_second", ctypes.c_double), ("trials", ctypes.c_int32), ("reserved2", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}
+
+
+class ServerOptions(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_int64), ("max_live", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("lds_vcap", ctypes.c_int32), ("waves_per_group", ctypes.c_int32), ("slice_events", ctypes.c_int64),
+                ("slice_seconds", ctypes.c_double), ("keep_results", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class JobInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("rep_status", ctypes.c_int32), ("events", ctypes.c_int64),
+                ("bytes", ctypes.c_int64), ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double),
+                ("t_done", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int64), ("admitted", ctypes.c_int64), ("done", ctypes.c_int64),
+                ("failed", ctypes.c_int64), ("launches", ctypes.c_int64), ("events", ctypes.c_int64),
+                ("live", ctypes.c_int64), ("bytes_live", ctypes.c_int64), ("bytes_peak", ctypes.c_int64),
+                ("busy_seconds", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def _bind_batch(L):
+    vp, i64, i32, dbl = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
+    P = ctypes.POINTER
+    L.akmc_batch_options_default.argtypes = [P(BatchOptions)]
+    L.akmc_batch_create.argtypes = [vp, i32, P(BatchOptions), i32, P(vp)]
+    L.akmc_batch_run.argtypes = [vp, i64, P(BatchStats)]
+    L.akmc_batch_tune.argtypes = [vp, i64, P(BatchTuning)]
+    L.akmc_batch_get_tuning.argtypes = [vp, P(BatchTuning)]
+    L.akmc_batch_size.argtypes = [vp]
+    L.akmc_batch_num_sites.argtypes = [vp, i32]
+    L.akmc_batch_num_sites.restype = i64
+    L.akmc_batch_replica_stats.argtypes = [vp, i32, P(ReplicaStats)]
+    L.akmc_batch_download.argtypes = [vp, i32, vp, i64]
+    L.akmc_batch_upload.argtypes = [vp, i32, vp, i64]
+    L.akmc_batch_sector_rates.argtypes = [vp, i32, i32, i64, vp, vp, P(dbl), P(i64)]
+    L.akmc_fp32_rate_table.argtypes = [dbl, dbl, vp, i32, P(i32), P(i32), P(dbl), vp, vp, vp]
+    L.akmc_batch_destroy.argtypes = [vp]
+    L.akmc_batch_destroy.restype = None
+    L.akmc_server_options_default.argtypes = [P(ServerOptions)]
+    L.akmc_server_create.argtypes = [i32, P(ServerOptions), P(vp)]
+    L.akmc_server_submit.argtypes = [vp, P(Params), i64, P(i64)]
+    L.akmc_server_job_info.argtypes = [vp, i64, P(JobInfo)]
+    L.akmc_server_result.argtypes = [vp, i64, vp, i64]
+    L.akmc_server_wait_idle.argtypes = [vp, dbl]
+    L.akmc_server_get_stats.argtypes = [vp, P(ServerStats)]
+    L.akmc_server_clock.argtypes = [vp]
+    L.akmc_server_clock.restype = dbl
+    L.akmc_server_destroy.argtypes = [vp]
+    L.akmc_server_destroy.restype = None
+
+
+_batch_bound = False
+
+
+def _blib():
+    global _batch_bound
+    L = lib()
+    if not _batch_bound:
+        _bind_batch(L)
+        _batch_bound = True
+    return L
+
+
+def fp32_rate_table(temperature, nu=6e12):
+    """The replica engine's FP32 rate table: (tab, nhi, m_min, q, e1q, e2q, e0q2)."""
+    tab = np.zeros(256 + 4096, dtype=np.float32)
+    e1q = np.zeros(25, dtype=np.float32)
+    e2q = np.zeros(25, dtype=np.float32)
+    e0q2 = np.zeros(5, dtype=np.float32)
+    nhi, m_min, q = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double()
+    _check(_blib().akmc_fp32_rate_table(float(temperature), float(nu), tab.ctypes.data, tab.size, ctypes.byref(nhi),
+                                        ctypes.byref(m_min), ctypes.byref(q), e1q.ctypes.data, e2q.ctypes.data,
+                                        e0q2.ctypes.data))
+    return tab[:256 + nhi.value].copy(), nhi.value, m_min.value, q.value, e1q, e2q, e0q2
+
+
+def batch_options(**kw):
+    o = BatchOptions()
+    _check(_blib().akmc_batch_options_default(ctypes.byref(o)))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Batch:
+    """Independent replicas advanced together by the replica engine."""
+
+    def __init__(self, configs, device=0, **options):
+        L = _blib()
+        params = [c if isinstance(c, Params) else params_from_config(c) for c in configs]
+        arr = (Params * len(params))(*params)
+        self.opts = batch_options(**options)
+        h = ctypes.c_void_p()
+        _check(L.akmc_batch_create(ctypes.cast(arr, ctypes.c_void_p), len(params), ctypes.byref(self.opts),
+                                   int(device), ctypes.byref(h)))
+        self.h = h
+        self.n = len(params)
+
+    def close(self):
+        if getattr(self, "h", None):
+            _blib().akmc_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def run(self, events_per_replica):
+        st = BatchStats()
+        _check(_blib().akmc_batch_run(self.h, int(events_per_replica), ctypes.byref(st)))
+        return st.as_dict()
+
+    def tune(self, trial_events=256):
+        t = BatchTuning()
+        _check(_blib().akmc_batch_tune(self.h, int(trial_events), ctypes.byref(t)))
+        return t.as_dict()
+
+    def tuning(self):
+        t = BatchTuning()
+        _check(_blib().akmc_batch_get_tuning(self.h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def replica_stats(self, i):
+        s = ReplicaStats()
+        _check(_blib().akmc_batch_replica_stats(self.h, int(i), ctypes.byref(s)))
+        return s.as_dict()
+
+    def num_sites(self, i):
+        return _blib().akmc_batch_num_sites(self.h, int(i))
+
+    def download(self, i):
+        out = np.zeros(self.num_sites(i), dtype=np.uint8)
+        _check(_blib().akmc_batch_download(self.h, int(i), out.ctypes.data, out.size))
+        return out
+
+    def upload(self, i, sites):
+        sites = np.ascontiguousarray(sites, dtype=np.uint8)
+        _check(_blib().akmc_batch_upload(self.h, int(i), sites.ctypes.data, sites.size))
+
+    def sector_rates(self, i, sector, cap=1 << 16):
+        sites = np.zeros(cap, dtype=np.int64)
+        rates = np.zeros((cap, 8), dtype=np.float64)
+        tot = ctypes.c_double()
+        n = ctypes.c_int64()
+        _check(_blib().akmc_batch_sector_rates(self.h, int(i), int(sector), cap, sites.ctypes.data,
+                                               rates.ctypes.data, ctypes.byref(tot), ctypes.byref(n)))
+        m = min(n.value, cap)
+        return sites[:m], rates[:m], tot.value
+
+
+def server_options(**kw):
+    o = ServerOptions()
+    _check(_blib().akmc_server_options_default(ctypes.byref(o)))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Server:
+    """The serving loop: jobs are admitted by HBM budget and batched as they arrive."""
+
+    def __init__(self, device=0, **options):
+        L = _blib()
+        self.opts = server_options(**options)
+        h = ctypes.c_void_p()
+        _check(L.akmc_server_create(int(device), ctypes.byref(self.opts), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            _blib().akmc_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, config, target_events):
+        p = config if isinstance(config, Params) else params_from_config(config)
+        job = ctypes.c_int64()
+        _check(_blib().akmc_server_submit(self.h, ctypes.byref(p), int(target_events), ctypes.byref(job)))
+        return job.value
+
+    def info(self, job):
+        i = JobInfo()
+        _check(_blib().akmc_server_job_info(self.h, int(job), ctypes.byref(i)))
+        return i.as_dict()
+
+    def result(self, job, n_sites):
+        out = np.zeros(int(n_sites), dtype=np.uint8)
+        _check(_blib().akmc_server_result(self.h, int(job), out.ctypes.data, out.size))
+        return out
+
+    def wait_idle(self, timeout=0.0):
+        _check(_blib().akmc_server_wait_idle(self.h, float(timeout)))
+
+    def stats(self):
+        s = ServerStats()
+        _check(_blib().akmc_server_get_stats(self.h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def clock(self):
+        return _blib().akmc_server_clock(self.h)

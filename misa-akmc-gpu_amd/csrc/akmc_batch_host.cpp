@@ -1,0 +1,1228 @@
+// libakmc host side of the replica engine (akmc_batch_*) and of the serving
+// loop (akmc_server_*): replica set-up, HBM layout, launch configuration,
+// hipGraph capture, the autotuner and the request loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "../../include/akmc.h"
+#include "akmc_batch.h"
+#include "akmc_host.h"
+#include "akmc_model.h"
+
+using akmc_host::fail;
+
+namespace {
+
+int64_t a256(int64_t x) { return (x + 255) & ~(int64_t)255; }
+
+uint64_t splitmix_next(uint64_t& x) {
+  uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// ---------------------------------------------------------------- FP32 form
+// The bond energies (bonds_counter.cpp:13-33) and e0 (vacancy_rates_solver.cpp:
+// 14-27) are decimal numbers with three digits; on the grid q = 10^-k eV
+// (smallest k that represents every one of them) they are small integers,
+// so the FP32 bond sums are exact.
+struct QTables {
+  bool ok = false;
+  double q = 0;
+  float e1q[25], e2q[25], e0q2[5];
+  int32_t dmax = 0;  // bound on |dE| / q: four counts of 8 + 5 terms
+};
+
+const QTables& qtables() {
+  static const QTables Q = [] {
+    QTables t;
+    const double e1[25] = AKMC_E1_TABLE, e2[25] = AKMC_E2_TABLE, e0[5] = AKMC_E0_TABLE;
+    for (int k = 0; k <= 6 && !t.ok; k++) {
+      const double q = std::pow(10.0, -k);
+      auto fits = [&](double v) {
+        const double m = v / q;
+        return std::fabs(m - std::nearbyint(m)) < 1e-6 && std::fabs(m) < 4096.0;
+      };
+      bool good = true;
+      for (int i = 0; i < 25; i++) good = good && fits(e1[i]) && fits(e2[i]);
+      for (int i = 0; i < 5; i++) good = good && fits(2.0 * e0[i]);
+      if (!good) continue;
+      t.ok = true;
+      t.q = q;
+      int A = 0, B = 0;
+      for (int i = 0; i < 25; i++) {
+        t.e1q[i] = (float)std::nearbyint(e1[i] / q);
+        t.e2q[i] = (float)std::nearbyint(e2[i] / q);
+        A = std::max(A, (int)std::fabs(t.e1q[i]));
+        B = std::max(B, (int)std::fabs(t.e2q[i]));
+      }
+      for (int i = 0; i < 5; i++) t.e0q2[i] = (float)std::nearbyint(2.0 * e0[i] / q);
+      t.dmax = 4 * (8 * A + 5 * B);
+    }
+    return t;
+  }();
+  return Q;
+}
+
+// Rate table of one temperature: rate(m) = nu * exp(-m q / (2 kT)) for the
+// barrier index m = 2 Ea / q, as tab_hi[(m - m_min) >> 8] * tab_lo[(m - m_min) & 255].
+int make_exp_table(double kT, double nu, std::vector<float>* tab, int32_t* nhi, int32_t* m_min) {
+  const QTables& Q = qtables();
+  if (!Q.ok) return fail(AKMC_ERR_CONFIG, "FP32 rates need bond energies on a decimal grid");
+  int lo = INT_MAX, hi = INT_MIN;
+  for (int X = 1; X < 5; X++) {
+    lo = std::min(lo, (int)Q.e0q2[X] - Q.dmax);
+    hi = std::max(hi, (int)Q.e0q2[X] + Q.dmax);
+  }
+  const int span = hi - lo + 1;
+  *nhi = (span + 255) / 256;
+  *m_min = lo;
+  if (*nhi > akmc::BATCH_NHI_MAX) return fail(AKMC_ERR_CONFIG, "FP32 rate table too large (%d)", *nhi);
+  const double c = Q.q / (2.0 * kT);
+  tab->assign(256 + *nhi, 0.0f);
+  for (int i = 0; i < 256; i++) (*tab)[i] = (float)std::exp(-(double)i * c);
+  for (int j = 0; j < *nhi; j++) (*tab)[256 + j] = (float)(nu * std::exp(-(double)(lo + 256 * j) * c));
+  return AKMC_OK;
+}
+
+// ---------------------------------------------------------------- replicas
+struct HostRep {
+  akmc_params p{};
+  akmc::BoxDims d{};
+  int64_t nsites = 0;
+  int32_t cap = 1;
+  int32_t nvac = 0;
+  double kT = 0;
+  int64_t time_steps = 0;
+  int32_t nhi = 0, m_min = 0;
+  std::vector<float> tab;
+  int64_t bytes = 0;
+  uint8_t* mem = nullptr;  // base of this replica's device block
+  uint8_t* sites = nullptr;
+  uint32_t* vac = nullptr;
+  uint8_t* ws = nullptr;
+  float* dtab = nullptr;
+};
+
+int setup_rep(HostRep* r, const akmc_params& p, bool fp32) {
+  for (int d = 0; d < 3; d++)
+    if (p.box[d] < 8) return fail(AKMC_ERR_INVALID, "box edge %lld < 8 cells is not supported", (long long)p.box[d]);
+  if (2 * p.box[0] > 2047 || p.box[1] > 2047 || p.box[2] > 1023)
+    return fail(AKMC_ERR_INVALID, "box exceeds 1023 x 2047 x 1023 cells");
+  if (p.n_alloy <= 0 || p.n_alloy > AKMC_MAX_ALLOY) return fail(AKMC_ERR_INVALID, "bad alloy list");
+  if (p.va_count < 0 || p.va_count > 262144)
+    return fail(AKMC_ERR_CAPACITY, "va_count %lld outside 0..262144", (long long)p.va_count);
+  if (!(p.sector_time > 0) || !(p.temperature > 0) || !(p.attempt_freq > 0))
+    return fail(AKMC_ERR_INVALID, "sector_time, temperature and attempt_freq must be positive");
+  r->p = p;
+  r->d.X = (int32_t)(2 * p.box[0]);
+  r->d.Y = (int32_t)p.box[1];
+  r->d.Z = (int32_t)p.box[2];
+  r->d.hx = (int32_t)(2 * (p.box[0] / 2));
+  r->d.hy = (int32_t)(p.box[1] / 2);
+  r->d.hz = (int32_t)(p.box[2] / 2);
+  r->nsites = (int64_t)r->d.X * r->d.Y * r->d.Z;
+  r->cap = (int32_t)std::max<int64_t>(1, p.va_count);
+  r->kT = AKMC_KB * p.temperature;
+  r->time_steps = (int64_t)std::ceil(p.physics_time / p.sector_time);
+  if (fp32) {
+    const int rc = make_exp_table(r->kT, p.attempt_freq, &r->tab, &r->nhi, &r->m_min);
+    if (rc) return rc;
+  }
+  r->bytes = a256(r->nsites) + a256(4 * (int64_t)r->cap) + a256(akmc::ws_layout(r->cap, fp32 ? 4 : 8).total) +
+             a256(4 * (int64_t)(256 + r->nhi));
+  return AKMC_OK;
+}
+
+void carve(HostRep* r, uint8_t* base, bool fp32) {
+  r->mem = base;
+  r->sites = base;
+  base += a256(r->nsites);
+  r->vac = reinterpret_cast<uint32_t*>(base);
+  base += a256(4 * (int64_t)r->cap);
+  r->ws = base;
+  base += a256(akmc::ws_layout(r->cap, fp32 ? 4 : 8).total);
+  r->dtab = reinterpret_cast<float*>(base);
+}
+
+akmc::ReplicaDesc make_desc(const HostRep& r) {
+  akmc::ReplicaDesc D{};
+  D.sites = r.sites;
+  D.vac_key = r.vac;
+  D.ws = r.ws;
+  D.exp_tab = r.dtab;
+  D.d = r.d;
+  D.nvac = r.nvac;
+  D.cap = r.cap;
+  D.nhi = r.nhi;
+  D.m_min = r.m_min;
+  D.nu = r.p.attempt_freq;
+  D.kT = r.kT;
+  D.sector_T = r.p.sector_time;
+  D.time_steps = r.time_steps;
+  return D;
+}
+
+akmc::ReplicaState initial_state(const HostRep& r) {
+  akmc::ReplicaState st{};
+  st.target = INT64_MAX;
+  uint64_t x = (uint64_t)r.p.seed_time_inc;
+  for (int k = 0; k < 4; k++) st.rng[k] = splitmix_next(x);
+  st.status = AKMC_REP_RUNNING;
+  return st;
+}
+
+uint32_t key_of(const akmc::BoxDims& d, int64_t i) {
+  const uint32_t x = (uint32_t)(i % d.X), y = (uint32_t)((i / d.X) % d.Y), z = (uint32_t)(i / ((int64_t)d.X * d.Y));
+  return x | (y << 11) | (z << 22);
+}
+
+std::vector<uint32_t> vacancy_keys(const akmc::BoxDims& d, const uint8_t* sites, int64_t n) {
+  std::vector<uint32_t> keys;
+  for (int64_t i = 0; i < n; i++)
+    if (sites[i] & akmc::BIT_V) keys.push_back(key_of(d, i));
+  return keys;
+}
+
+// Small device buffers for replica initialisation.
+struct InitScratch {
+  int64_t* idx = nullptr;
+  int64_t idx_cap = 0;
+  uint32_t* cum = nullptr;
+  uint8_t* flag = nullptr;
+  ~InitScratch() {
+    if (idx) (void)hipFree(idx);
+    if (cum) (void)hipFree(cum);
+    if (flag) (void)hipFree(flag);
+  }
+};
+
+// Device initial occupancy (AKMC_INIT_DEVICE), then the vacancy list.
+int init_device(HostRep* r, InitScratch* S, hipStream_t st) {
+  uint32_t cum[AKMC_MAX_ALLOY];
+  uint8_t flag[AKMC_MAX_ALLOY];
+  int nt = 0;
+  uint64_t total = 0;
+  for (int i = 0; i < r->p.n_alloy; i++) {
+    const int t = r->p.alloy_type[i];
+    const int64_t q = r->p.alloy_ratio[i];
+    if (q < 0) return fail(AKMC_ERR_INVALID, "negative alloy ratio");
+    if (q == 0) continue;
+    uint8_t f;
+    switch (t) {
+      case AKMC_TYPE_FE: f = 0; break;
+      case AKMC_TYPE_CU: f = akmc::BIT_CU; break;
+      case AKMC_TYPE_MN: f = akmc::BIT_MN; break;
+      case AKMC_TYPE_NI: f = akmc::BIT_NI; break;
+      default: return fail(AKMC_ERR_INVALID, "alloy type 0x%x is not modelled by the vacancy-hop path", t);
+    }
+    total += (uint64_t)q;
+    if (total > 0xFFFFFFFFull) return fail(AKMC_ERR_INVALID, "alloy ratios too large");
+    cum[nt] = (uint32_t)total;
+    flag[nt] = f;
+    nt++;
+  }
+  if (nt == 0) return fail(AKMC_ERR_INVALID, "empty alloy");
+  if (!S->cum) {
+    HIP_TRY(hipMalloc(&S->cum, sizeof(cum)));
+    HIP_TRY(hipMalloc(&S->flag, sizeof(flag)));
+  }
+  HIP_TRY(hipMemcpyAsync(S->cum, cum, sizeof(uint32_t) * nt, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(S->flag, flag, nt, hipMemcpyHostToDevice, st));
+  if (akmc_batch_init_sites_launch(r->sites, r->nsites, (uint64_t)r->p.seed_create_types, S->cum, S->flag, nt, st))
+    return fail(AKMC_ERR_HIP, "init_sites launch failed");
+  // Vacancy sites: va_count uniform draws (repeats collapse, as in the
+  // reference's hash insertions).
+  std::vector<int64_t> idx((size_t)r->p.va_count);
+  uint64_t x = (uint64_t)r->p.seed_create_vacancy ^ 0x5A5A5A5A5A5A5A5Aull;
+  for (auto& v : idx) v = (int64_t)(((unsigned __int128)splitmix_next(x) * (uint64_t)r->nsites) >> 64);
+  std::sort(idx.begin(), idx.end());
+  idx.erase(std::unique(idx.begin(), idx.end()), idx.end());
+  if ((int64_t)idx.size() > S->idx_cap) {
+    if (S->idx) HIP_TRY(hipFree(S->idx));
+    S->idx_cap = std::max<int64_t>((int64_t)idx.size(), 4096);
+    HIP_TRY(hipMalloc(&S->idx, S->idx_cap * sizeof(int64_t)));
+  }
+  std::vector<uint32_t> keys(idx.size());
+  for (size_t i = 0; i < idx.size(); i++) keys[i] = key_of(r->d, idx[i]);
+  if (!idx.empty()) {
+    HIP_TRY(hipMemcpyAsync(S->idx, idx.data(), idx.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    if (akmc_batch_place_vacancies_launch(r->sites, S->idx, (int64_t)idx.size(), st))
+      return fail(AKMC_ERR_HIP, "place_vacancies launch failed");
+    HIP_TRY(hipMemcpyAsync(r->vac, keys.data(), keys.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+  }
+  r->nvac = (int32_t)keys.size();
+  // The copies above read pageable host vectors that end with this scope.
+  HIP_TRY(hipStreamSynchronize(st));
+  return AKMC_OK;
+}
+
+int upload_rep(HostRep* r, const uint8_t* sites, hipStream_t st) {
+  const std::vector<uint32_t> keys = vacancy_keys(r->d, sites, r->nsites);
+  if ((int64_t)keys.size() > r->cap)
+    return fail(AKMC_ERR_CAPACITY, "lattice holds %zu vacancies, replica capacity %d", keys.size(), r->cap);
+  HIP_TRY(hipMemcpyAsync(r->sites, sites, r->nsites, hipMemcpyHostToDevice, st));
+  if (!keys.empty())
+    HIP_TRY(hipMemcpyAsync(r->vac, keys.data(), keys.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+  r->nvac = (int32_t)keys.size();
+  HIP_TRY(hipStreamSynchronize(st));
+  return AKMC_OK;
+}
+
+// ---------------------------------------------------------------- launches
+struct Tuning {
+  int32_t vcap = 256;   // LDS class capacity (-1: no LDS class)
+  int32_t waves = 2;
+  int32_t order = 1;
+  int64_t slice = 1024;
+};
+
+// Launch configuration and work lists of a set of replica slots.
+struct Runner {
+  int device = 0;
+  bool fp32 = true;
+  int flags = 0;
+  uint64_t budget_ticks = 0;
+  hipStream_t s0 = nullptr, s1 = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr, e0 = nullptr, e1 = nullptr;
+  akmc::ReplicaDesc* d_desc = nullptr;
+  akmc::ReplicaState* d_state = nullptr;
+  int32_t slots = 0;
+  int32_t* d_items = nullptr;
+  uint64_t* d_queue = nullptr;
+  int32_t n_lds = 0, n_glob = 0;
+  int grid_lds = 0, grid_glob = 0;
+  int region_lds = 0, region_glob = 0;
+  int n_cu = 0;
+  Tuning tun;
+  bool use_graph = false;
+  hipGraphExec_t exec = nullptr;
+  bool graph_valid = false;
+
+  int init(int dev, bool f32, int32_t nslots) {
+    device = dev;
+    fp32 = f32;
+    slots = nslots;
+    HIP_TRY(hipSetDevice(dev));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    n_cu = prop.multiProcessorCount;
+    HIP_TRY(hipStreamCreateWithFlags(&s0, hipStreamNonBlocking));
+    HIP_TRY(hipStreamCreateWithFlags(&s1, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipMalloc(&d_desc, sizeof(akmc::ReplicaDesc) * std::max(1, nslots)));
+    HIP_TRY(hipMalloc(&d_state, sizeof(akmc::ReplicaState) * std::max(1, nslots)));
+    HIP_TRY(hipMalloc(&d_items, sizeof(int32_t) * std::max(1, nslots)));
+    HIP_TRY(hipMalloc(&d_queue, 4 * sizeof(uint64_t)));
+    HIP_TRY(hipMemset(d_queue, 0, 4 * sizeof(uint64_t)));
+    static akmc::StencilTables tables;
+    static std::once_flag once;
+    std::call_once(once, [] { akmc_host::build_stencil(&tables); });
+    if (akmc_batch_set_stencil(&tables)) return fail(AKMC_ERR_HIP, "stencil upload failed");
+    return AKMC_OK;
+  }
+
+  void destroy() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (d_desc) (void)hipFree(d_desc);
+    if (d_state) (void)hipFree(d_state);
+    if (d_items) (void)hipFree(d_items);
+    if (d_queue) (void)hipFree(d_queue);
+    for (hipEvent_t e : {fork, join, e0, e1})
+      if (e) (void)hipEventDestroy(e);
+    if (s0) (void)hipStreamDestroy(s0);
+    if (s1) (void)hipStreamDestroy(s1);
+  }
+
+  // Splits the live slots into the LDS-resident and global kinds and orders
+  // each work list.
+  int set_items(const std::vector<int32_t>& live, const std::vector<int32_t>& cap_of_slot) {
+    std::vector<int32_t> a, b;
+    for (int32_t s : live) (tun.vcap > 0 && cap_of_slot[s] <= tun.vcap ? a : b).push_back(s);
+    if (tun.order == 1) {
+      auto big_first = [&](int32_t x, int32_t y) {
+        return cap_of_slot[x] != cap_of_slot[y] ? cap_of_slot[x] > cap_of_slot[y] : x < y;
+      };
+      std::stable_sort(a.begin(), a.end(), big_first);
+      std::stable_sort(b.begin(), b.end(), big_first);
+    }
+    n_lds = (int32_t)a.size();
+    n_glob = (int32_t)b.size();
+    std::vector<int32_t> all(a);
+    all.insert(all.end(), b.begin(), b.end());
+    if (!all.empty())
+      HIP_TRY(hipMemcpy(d_items, all.data(), all.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    region_lds = tun.vcap > 0 ? akmc_batch_region_bytes(fp32, tun.vcap) : 0;
+    region_glob = akmc_batch_region_bytes(fp32, 0);
+    grid_lds = grid_glob = 0;
+    if (n_lds) {
+      int per_cu = 0;
+      if (akmc_batch_occupancy(fp32, 1, tun.waves, region_lds, &per_cu) || per_cu <= 0)
+        return fail(AKMC_ERR_CAPACITY, "LDS class does not fit (vcap %d, %d waves)", tun.vcap, tun.waves);
+      grid_lds = std::min(per_cu * n_cu, (n_lds + tun.waves - 1) / tun.waves);
+    }
+    if (n_glob) {
+      int per_cu = 0;
+      if (akmc_batch_occupancy(fp32, 0, tun.waves, region_glob, &per_cu) || per_cu <= 0)
+        return fail(AKMC_ERR_CAPACITY, "global class does not fit");
+      grid_glob = std::min(per_cu * n_cu, (n_glob + tun.waves - 1) / tun.waves);
+    }
+    graph_valid = false;
+    return AKMC_OK;
+  }
+
+  akmc::BatchArgs args(bool lds) const {
+    const auto& Q = qtables();
+    akmc::BatchArgs a{};
+    a.desc = d_desc;
+    a.state = d_state;
+    a.items = lds ? d_items : d_items + n_lds;
+    a.n_items = lds ? n_lds : n_glob;
+    a.flags = flags;
+    a.queue = lds ? d_queue : d_queue + 2;
+    a.slice_events = tun.slice;
+    a.budget_ticks = budget_ticks;
+    a.region_bytes = lds ? region_lds : region_glob;
+    a.vcap = lds ? tun.vcap : 0;
+    a.rates_sector = -1;
+    for (int i = 0; i < 25; i++) { a.e1q[i] = Q.e1q[i]; a.e2q[i] = Q.e2q[i]; }
+    for (int i = 0; i < 5; i++) a.e0q2[i] = Q.e0q2[i];
+    return a;
+  }
+
+  // One slice: both kinds, the global kind on a forked stream.
+  int enqueue() {
+    const bool both = n_lds && n_glob;
+    if (both) {
+      HIP_TRY(hipEventRecord(fork, s0));
+      HIP_TRY(hipStreamWaitEvent(s1, fork, 0));
+    }
+    if (n_glob) {
+      hipStream_t s = both ? s1 : s0;
+      const akmc::BatchArgs a = args(false);
+      if (akmc_batch_reset_launch(d_queue + 2, s)) return fail(AKMC_ERR_HIP, "reset launch failed");
+      if (akmc_batch_launch(&a, fp32, 0, tun.waves, grid_glob, s)) return fail(AKMC_ERR_HIP, "launch failed");
+    }
+    if (n_lds) {
+      const akmc::BatchArgs a = args(true);
+      if (akmc_batch_reset_launch(d_queue, s0)) return fail(AKMC_ERR_HIP, "reset launch failed");
+      if (akmc_batch_launch(&a, fp32, 1, tun.waves, grid_lds, s0)) return fail(AKMC_ERR_HIP, "launch failed");
+    }
+    if (both) {
+      HIP_TRY(hipEventRecord(join, s1));
+      HIP_TRY(hipStreamWaitEvent(s0, join, 0));
+    }
+    return AKMC_OK;
+  }
+
+  int launch() {
+    if (!n_lds && !n_glob) return AKMC_OK;
+    if (!use_graph) return enqueue();
+    if (!graph_valid) {
+      if (exec) {
+        HIP_TRY(hipGraphExecDestroy(exec));
+        exec = nullptr;
+      }
+      hipGraph_t g = nullptr;
+      HIP_TRY(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+      const int rc = enqueue();
+      const hipError_t ce = hipStreamEndCapture(s0, &g);
+      if (rc) return rc;
+      if (ce != hipSuccess) return fail(AKMC_ERR_HIP, "graph capture: %s", hipGetErrorString(ce));
+      const hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) return fail(AKMC_ERR_HIP, "graph instantiate: %s", hipGetErrorString(ie));
+      graph_valid = true;
+    }
+    HIP_TRY(hipGraphLaunch(exec, s0));
+    return AKMC_OK;
+  }
+};
+
+void fill_rep_stats(const HostRep& r, const akmc::ReplicaState& st, akmc_replica_stats* s) {
+  memset(s, 0, sizeof(*s));
+  s->events = st.events;
+  s->step = st.step;
+  s->status = st.status;
+  s->sector_pos = st.sect;
+  for (int i = 0; i < 8; i++) s->evolution[i] = st.evo[i];
+  s->sector_time = st.t;
+  s->nvac = r.nvac;
+  s->n_eval = st.n_eval;
+  s->n_rebuild = st.n_rebuild;
+  s->n_jump = st.n_jump;
+  s->err_max = st.err_max;
+  s->err_mean = st.err_n ? st.err_sum / (double)st.err_n : 0.0;
+  s->err_n = st.err_n;
+  s->err_tiny = st.err_tiny;
+}
+
+}  // namespace
+
+// ================================================================ batch
+struct akmc_batch {
+  akmc_batch_options opt{};
+  Runner R;
+  std::vector<HostRep> reps;
+  std::vector<int32_t> cap_of_slot;
+  std::vector<int32_t> live;
+  uint8_t* pool = nullptr;
+  InitScratch scratch;
+  std::vector<akmc::ReplicaState> h_state;
+  akmc_batch_tuning last_tune{};
+};
+
+namespace {
+
+int read_states(akmc_batch* b) {
+  HIP_TRY(hipStreamSynchronize(b->R.s0));
+  HIP_TRY(hipMemcpy(b->h_state.data(), b->R.d_state, b->h_state.size() * sizeof(akmc::ReplicaState),
+                    hipMemcpyDeviceToHost));
+  return AKMC_OK;
+}
+int write_states(akmc_batch* b) {
+  HIP_TRY(hipMemcpy(b->R.d_state, b->h_state.data(), b->h_state.size() * sizeof(akmc::ReplicaState),
+                    hipMemcpyHostToDevice));
+  return AKMC_OK;
+}
+
+// Runs `launches` slices; returns the events of all replicas and device seconds.
+int run_slices(akmc_batch* b, int64_t launches, int64_t* events, double* ksec) {
+  int64_t before = 0, after = 0;
+  for (const auto& s : b->h_state) before += s.events;
+  HIP_TRY(hipEventRecord(b->R.e0, b->R.s0));
+  for (int64_t k = 0; k < launches; k++) {
+    const int rc = b->R.launch();
+    if (rc) return rc;
+  }
+  HIP_TRY(hipEventRecord(b->R.e1, b->R.s0));
+  HIP_TRY(hipEventSynchronize(b->R.e1));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, b->R.e0, b->R.e1));
+  const int rc = read_states(b);
+  if (rc) return rc;
+  for (const auto& s : b->h_state) after += s.events;
+  *events = after - before;
+  *ksec = ms * 1e-3;
+  return AKMC_OK;
+}
+
+int set_targets(akmc_batch* b, int64_t events) {
+  for (auto& s : b->h_state)
+    if (s.status == AKMC_REP_RUNNING) s.target = s.events + events;
+  return write_states(b);
+}
+
+}  // namespace
+
+extern "C" {
+
+int akmc_batch_options_default(akmc_batch_options* o) {
+  if (!o) return fail(AKMC_ERR_INVALID, "null options");
+  memset(o, 0, sizeof(*o));
+  o->precision = AKMC_PREC_FP32;
+  o->init = AKMC_INIT_DEVICE;
+  o->use_graph = 1;
+  o->lds_vcap = 0;
+  o->waves_per_group = 0;
+  o->order = -1;
+  o->slice_events = 0;
+  return AKMC_OK;
+}
+
+int akmc_batch_create(const akmc_params* reps, int32_t n, const akmc_batch_options* o, int device,
+                      akmc_batch** out) {
+  if (!reps || !out || n <= 0) return fail(AKMC_ERR_INVALID, "bad arguments");
+  *out = nullptr;
+  akmc_batch_options opt;
+  if (o) opt = *o; else akmc_batch_options_default(&opt);
+  if (opt.precision != AKMC_PREC_FP32 && opt.precision != AKMC_PREC_FP64)
+    return fail(AKMC_ERR_INVALID, "bad precision");
+  const bool fp32 = opt.precision == AKMC_PREC_FP32;
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(AKMC_ERR_INVALID, "device %d out of range", device);
+  HIP_TRY(hipSetDevice(device));
+  std::unique_ptr<akmc_batch> b(new akmc_batch());
+  b->opt = opt;
+  b->reps.resize(n);
+  int64_t total = 0;
+  for (int i = 0; i < n; i++) {
+    const int rc = setup_rep(&b->reps[i], reps[i], fp32);
+    if (rc) return fail(rc, "replica %d: %s", i, akmc_host::g_err.c_str());
+    total += b->reps[i].bytes;
+  }
+  int rc = b->R.init(device, fp32, n);
+  if (rc) return rc;
+  HIP_TRY(hipMalloc(&b->pool, (size_t)total));
+  int64_t off = 0;
+  for (auto& r : b->reps) {
+    carve(&r, b->pool + off, fp32);
+    off += r.bytes;
+  }
+  // Initial lattices.
+  if (opt.init == AKMC_INIT_REFERENCE) {
+    // creation::createRandom per replica on the host, in parallel.
+    const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::vector<uint8_t>> host(n);
+    std::vector<int> rcs(n, 0);
+    std::atomic<int> next{0};
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nth; t++)
+      th.emplace_back([&] {
+        for (int i = next++; i < n; i = next++) {
+          host[i].resize(b->reps[i].nsites);
+          rcs[i] = akmc_random_sites(&b->reps[i].p, host[i].data(), b->reps[i].nsites);
+        }
+      });
+    for (auto& t : th) t.join();
+    for (int i = 0; i < n; i++) {
+      if (rcs[i]) return fail(rcs[i], "replica %d: creation failed", i);
+      rc = upload_rep(&b->reps[i], host[i].data(), b->R.s0);
+      if (rc) return rc;
+      host[i].clear();
+      host[i].shrink_to_fit();
+    }
+  } else if (opt.init == AKMC_INIT_DEVICE) {
+    for (int i = 0; i < n; i++) {
+      rc = init_device(&b->reps[i], &b->scratch, b->R.s0);
+      if (rc) return fail(rc, "replica %d: %s", i, akmc_host::g_err.c_str());
+    }
+  } else {
+    return fail(AKMC_ERR_INVALID, "bad init mode");
+  }
+  std::vector<akmc::ReplicaDesc> desc(n);
+  b->h_state.resize(n);
+  b->cap_of_slot.resize(n);
+  for (int i = 0; i < n; i++) {
+    if (fp32)
+      HIP_TRY(hipMemcpy(b->reps[i].dtab, b->reps[i].tab.data(), b->reps[i].tab.size() * sizeof(float),
+                        hipMemcpyHostToDevice));
+    desc[i] = make_desc(b->reps[i]);
+    b->h_state[i] = initial_state(b->reps[i]);
+    b->cap_of_slot[i] = b->reps[i].cap;
+    b->live.push_back(i);
+  }
+  HIP_TRY(hipMemcpy(b->R.d_desc, desc.data(), n * sizeof(akmc::ReplicaDesc), hipMemcpyHostToDevice));
+  rc = write_states(b.get());
+  if (rc) return rc;
+  b->R.use_graph = opt.use_graph != 0;
+  b->R.flags = opt.error_stats ? AKMC_BF_ERROR_STATS : 0;
+  b->R.tun.vcap = opt.lds_vcap > 0 ? opt.lds_vcap : (opt.lds_vcap < 0 ? -1 : 256);
+  b->R.tun.waves = opt.waves_per_group > 0 ? opt.waves_per_group : 2;
+  b->R.tun.order = opt.order >= 0 ? opt.order : 1;
+  b->R.tun.slice = opt.slice_events > 0 ? opt.slice_events : 1024;
+  if (b->R.tun.waves != 1 && b->R.tun.waves != 2 && b->R.tun.waves != 4)
+    return fail(AKMC_ERR_INVALID, "waves_per_group must be 1, 2 or 4");
+  rc = b->R.set_items(b->live, b->cap_of_slot);
+  if (rc) return rc;
+  HIP_TRY(hipDeviceSynchronize());
+  *out = b.release();
+  return AKMC_OK;
+}
+
+int akmc_batch_run(akmc_batch* b, int64_t events, akmc_batch_stats* stats) {
+  if (!b || events < 0) return fail(AKMC_ERR_INVALID, "bad arguments");
+  HIP_TRY(hipSetDevice(b->R.device));
+  const auto t0 = std::chrono::steady_clock::now();
+  int rc = read_states(b);
+  if (rc) return rc;
+  rc = set_targets(b, events);
+  if (rc) return rc;
+  int64_t total = 0, launches = 0;
+  double ksec = 0;
+  int64_t need = (events + b->R.tun.slice - 1) / b->R.tun.slice;
+  while (need > 0) {
+    int64_t ev = 0;
+    double ks = 0;
+    rc = run_slices(b, need, &ev, &ks);
+    if (rc) return rc;
+    total += ev;
+    ksec += ks;
+    launches += need;
+    need = 0;
+    for (const auto& s : b->h_state)
+      if (s.status == AKMC_REP_RUNNING && s.events < s.target) need = 1;
+  }
+  if (stats) {
+    memset(stats, 0, sizeof(*stats));
+    stats->events = total;
+    stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    stats->kernel_seconds = ksec;
+    stats->launches = launches;
+    for (const auto& s : b->h_state) {
+      if (s.status == AKMC_REP_RUNNING || s.status == AKMC_REP_TARGET_DONE) stats->running++;
+      if (s.status < 0) stats->failed++;
+    }
+  }
+  // Replicas that reached this run's target continue in the next run.
+  for (auto& s : b->h_state)
+    if (s.status == AKMC_REP_TARGET_DONE) s.status = AKMC_REP_RUNNING;
+  return write_states(b);
+}
+
+int akmc_batch_get_tuning(akmc_batch* b, akmc_batch_tuning* t) {
+  if (!b || !t) return fail(AKMC_ERR_INVALID, "bad arguments");
+  memset(t, 0, sizeof(*t));
+  t->lds_vcap = b->R.tun.vcap;
+  t->waves_per_group = b->R.tun.waves;
+  t->order = b->R.tun.order;
+  t->slice_events = b->R.tun.slice;
+  t->events_per_second = b->last_tune.events_per_second;
+  t->trials = b->last_tune.trials;
+  return AKMC_OK;
+}
+
+int akmc_batch_tune(akmc_batch* b, int64_t trial_events, akmc_batch_tuning* chosen) {
+  if (!b || trial_events <= 0) return fail(AKMC_ERR_INVALID, "bad arguments");
+  HIP_TRY(hipSetDevice(b->R.device));
+  const Tuning keep = b->R.tun;
+  int32_t maxcap = 1;
+  for (int32_t c : b->cap_of_slot) maxcap = std::max(maxcap, c);
+  std::vector<int32_t> vcaps = {-1};
+  for (int v : {32, 64, 128, 256, 512, 1024}) {
+    vcaps.push_back(v);
+    if (v >= maxcap) break;  // larger capacities only add LDS
+  }
+  Tuning best = keep;
+  double best_rate = -1;
+  int trials = 0;
+  for (int32_t vc : vcaps)
+    for (int32_t w : {1, 2, 4})
+      for (int32_t ord : {0, 1}) {
+        Tuning t = keep;
+        t.vcap = vc;
+        t.waves = w;
+        t.order = ord;
+        t.slice = trial_events;
+        if (vc > 0 && (int64_t)w * akmc_batch_region_bytes(b->R.fp32, vc) > 160 * 1024) continue;
+        b->R.tun = t;
+        if (b->R.set_items(b->live, b->cap_of_slot)) continue;  // does not fit: skip
+        int rc = read_states(b);
+        if (rc) return rc;
+        rc = set_targets(b, trial_events);
+        if (rc) return rc;
+        int64_t ev = 0;
+        double ks = 0;
+        rc = run_slices(b, 1, &ev, &ks);
+        if (rc) return rc;
+        trials++;
+        const double rate = ks > 0 ? ev / ks : 0;
+        if (rate > best_rate) {
+          best_rate = rate;
+          best = t;
+        }
+      }
+  for (auto& s : b->h_state)
+    if (s.status == AKMC_REP_TARGET_DONE) s.status = AKMC_REP_RUNNING;
+  int rc = write_states(b);
+  if (rc) return rc;
+  best.slice = keep.slice;
+  b->R.tun = best;
+  rc = b->R.set_items(b->live, b->cap_of_slot);
+  if (rc) return rc;
+  b->last_tune.events_per_second = best_rate;
+  b->last_tune.trials = trials;
+  return chosen ? akmc_batch_get_tuning(b, chosen) : AKMC_OK;
+}
+
+int32_t akmc_batch_size(const akmc_batch* b) { return b ? (int32_t)b->reps.size() : 0; }
+
+int64_t akmc_batch_num_sites(const akmc_batch* b, int32_t i) {
+  if (!b || i < 0 || i >= (int32_t)b->reps.size()) return -1;
+  return b->reps[i].nsites;
+}
+
+int akmc_batch_replica_stats(akmc_batch* b, int32_t i, akmc_replica_stats* s) {
+  if (!b || !s || i < 0 || i >= (int32_t)b->reps.size()) return fail(AKMC_ERR_INVALID, "bad arguments");
+  HIP_TRY(hipSetDevice(b->R.device));
+  HIP_TRY(hipStreamSynchronize(b->R.s0));
+  akmc::ReplicaState st;
+  HIP_TRY(hipMemcpy(&st, b->R.d_state + i, sizeof(st), hipMemcpyDeviceToHost));
+  fill_rep_stats(b->reps[i], st, s);
+  return AKMC_OK;
+}
+
+int akmc_batch_download(akmc_batch* b, int32_t i, uint8_t* sites, int64_t n) {
+  if (!b || !sites || i < 0 || i >= (int32_t)b->reps.size()) return fail(AKMC_ERR_INVALID, "bad arguments");
+  if (n != b->reps[i].nsites) return fail(AKMC_ERR_INVALID, "site count %lld != %lld", (long long)n,
+                                          (long long)b->reps[i].nsites);
+  HIP_TRY(hipSetDevice(b->R.device));
+  HIP_TRY(hipStreamSynchronize(b->R.s0));
+  HIP_TRY(hipMemcpy(sites, b->reps[i].sites, n, hipMemcpyDeviceToHost));
+  return AKMC_OK;
+}
+
+int akmc_batch_upload(akmc_batch* b, int32_t i, const uint8_t* sites, int64_t n) {
+  if (!b || !sites || i < 0 || i >= (int32_t)b->reps.size()) return fail(AKMC_ERR_INVALID, "bad arguments");
+  if (n != b->reps[i].nsites) return fail(AKMC_ERR_INVALID, "site count mismatch");
+  HIP_TRY(hipSetDevice(b->R.device));
+  int rc = read_states(b);
+  if (rc) return rc;
+  rc = upload_rep(&b->reps[i], sites, b->R.s0);
+  if (rc) return rc;
+  const akmc::ReplicaDesc D = make_desc(b->reps[i]);
+  HIP_TRY(hipMemcpy(b->R.d_desc + i, &D, sizeof(D), hipMemcpyHostToDevice));
+  b->h_state[i].in_sector = 0;
+  return write_states(b);
+}
+
+int akmc_batch_sector_rates(akmc_batch* b, int32_t i, int sector, int64_t cap, int64_t* sites, double* rates,
+                            double* total, int64_t* n) {
+  if (!b || i < 0 || i >= (int32_t)b->reps.size() || sector < 0 || sector > 7 || !n)
+    return fail(AKMC_ERR_INVALID, "bad arguments");
+  HIP_TRY(hipSetDevice(b->R.device));
+  HIP_TRY(hipStreamSynchronize(b->R.s0));
+  const HostRep& r = b->reps[i];
+  const bool fp32 = b->R.fp32;
+  const akmc::WsLayout L = akmc::ws_layout(r.cap, fp32 ? 4 : 8);
+  uint8_t* ws = nullptr;
+  akmc::ReplicaDesc* dd = nullptr;
+  akmc::ReplicaState* ds = nullptr;
+  int32_t* di = nullptr;
+  uint64_t* dq = nullptr;
+  int rc = AKMC_OK;
+  auto cleanup = [&] {
+    if (ws) (void)hipFree(ws);
+    if (dd) (void)hipFree(dd);
+    if (ds) (void)hipFree(ds);
+    if (di) (void)hipFree(di);
+    if (dq) (void)hipFree(dq);
+  };
+  do {
+    if (hipMalloc(&ws, L.total) || hipMalloc(&dd, sizeof(*dd)) || hipMalloc(&ds, sizeof(*ds)) ||
+        hipMalloc(&di, sizeof(*di)) || hipMalloc(&dq, 4 * sizeof(uint64_t))) {
+      rc = fail(AKMC_ERR_HIP, "scratch allocation failed");
+      break;
+    }
+    akmc::ReplicaDesc D = make_desc(r);
+    D.ws = ws;
+    akmc::ReplicaState st = b->h_state[i];
+    const int32_t zero = 0;
+    if (hipMemcpy(dd, &D, sizeof(D), hipMemcpyHostToDevice) || hipMemcpy(ds, &st, sizeof(st), hipMemcpyHostToDevice) ||
+        hipMemcpy(di, &zero, sizeof(zero), hipMemcpyHostToDevice) || hipMemset(dq, 0, 4 * sizeof(uint64_t))) {
+      rc = fail(AKMC_ERR_HIP, "scratch upload failed");
+      break;
+    }
+    akmc::BatchArgs a = b->R.args(false);
+    a.desc = dd;
+    a.state = ds;
+    a.items = di;
+    a.n_items = 1;
+    a.queue = dq;
+    a.flags = AKMC_BF_RATES_ONLY;
+    a.rates_sector = sector;
+    if (akmc_batch_launch(&a, fp32, 0, 1, 1, b->R.s0)) {
+      rc = fail(AKMC_ERR_HIP, "rates launch failed");
+      break;
+    }
+    if (hipStreamSynchronize(b->R.s0) || hipMemcpy(&st, ds, sizeof(st), hipMemcpyDeviceToHost)) {
+      rc = fail(AKMC_ERR_HIP, "rates launch did not complete");
+      break;
+    }
+    if (st.nv < 0) {
+      rc = fail(AKMC_ERR_CONFIG, "an FP32 rate left the table range");
+      break;
+    }
+    *n = st.nv;
+    const int64_t m = std::min<int64_t>(cap, st.nv);
+    std::vector<uint8_t> host(L.total);
+    if (hipMemcpy(host.data(), ws, L.total, hipMemcpyDeviceToHost)) {
+      rc = fail(AKMC_ERR_HIP, "workspace download failed");
+      break;
+    }
+    const uint32_t* keys = reinterpret_cast<const uint32_t*>(host.data() + L.key);
+    const double* vt = reinterpret_cast<const double*>(host.data() + L.vt);
+    double sum = 0;
+    for (int64_t v = 0; v < st.nv; v++) sum += vt[v];
+    if (total) *total = sum;
+    for (int64_t v = 0; v < m; v++) {
+      const uint32_t k = keys[v];
+      if (sites)
+        sites[v] = (int64_t)(k & 0x7FF) + (int64_t)r.d.X * ((int64_t)((k >> 11) & 0x7FF) + (int64_t)r.d.Y * (k >> 22));
+      if (rates)
+        for (int bb = 0; bb < 8; bb++)
+          rates[8 * v + bb] = fp32 ? (double)reinterpret_cast<const float*>(host.data() + L.rate)[8 * v + bb]
+                                   : reinterpret_cast<const double*>(host.data() + L.rate)[8 * v + bb];
+    }
+  } while (0);
+  cleanup();
+  return rc;
+}
+
+int akmc_fp32_rate_table(double temperature, double nu, float* tab, int32_t cap, int32_t* nhi, int32_t* m_min,
+                         double* q, float* e1q, float* e2q, float* e0q2) {
+  if (!tab || !nhi || !m_min || !(temperature > 0) || !(nu > 0)) return fail(AKMC_ERR_INVALID, "bad arguments");
+  std::vector<float> t;
+  const int rc = make_exp_table(AKMC_KB * temperature, nu, &t, nhi, m_min);
+  if (rc) return rc;
+  if ((int64_t)t.size() > cap) return fail(AKMC_ERR_INVALID, "table needs %zu floats", t.size());
+  memcpy(tab, t.data(), t.size() * sizeof(float));
+  const QTables& Q = qtables();
+  if (q) *q = Q.q;
+  if (e1q) memcpy(e1q, Q.e1q, sizeof(Q.e1q));
+  if (e2q) memcpy(e2q, Q.e2q, sizeof(Q.e2q));
+  if (e0q2) memcpy(e0q2, Q.e0q2, sizeof(Q.e0q2));
+  return AKMC_OK;
+}
+
+void akmc_batch_destroy(akmc_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->R.device);
+  (void)hipDeviceSynchronize();
+  b->R.destroy();
+  if (b->pool) (void)hipFree(b->pool);
+  delete b;
+}
+
+}  // extern "C"
+
+// ================================================================ server
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Job {
+  int64_t id = 0;
+  akmc_params p{};
+  int64_t target = 0;
+  HostRep rep;
+  int32_t status = AKMC_JOB_PENDING;
+  int32_t rep_status = AKMC_REP_RUNNING;
+  int32_t slot = -1;
+  int64_t events = 0;
+  double t_submit = 0, t_admit = 0, t_done = 0;
+  std::vector<uint8_t> result;
+};
+
+}  // namespace
+
+struct akmc_server {
+  akmc_server_options opt{};
+  Runner R;
+  Clock::time_point t0;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<Job*> pending;
+  std::map<int64_t, std::unique_ptr<Job>> jobs;
+  std::vector<Job*> slot_job;
+  std::vector<int32_t> free_slots;
+  std::vector<int32_t> cap_of_slot;
+  std::vector<akmc::ReplicaState> h_state;
+  InitScratch scratch;
+  int64_t next_id = 1;
+  int64_t bytes_live = 0;
+  akmc_server_stats st{};
+  std::atomic<bool> stop{false};
+  std::thread worker;
+  int error = AKMC_OK;
+  std::string error_msg;
+
+  double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+};
+
+namespace {
+
+// Admits pending jobs in arrival order while the budget and slots allow.
+int admit(akmc_server* s, bool* changed) {
+  for (;;) {
+    Job* j = nullptr;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (s->pending.empty() || s->free_slots.empty()) return AKMC_OK;
+      Job* h = s->pending.front();
+      if (s->bytes_live + h->rep.bytes > s->opt.hbm_budget) return AKMC_OK;
+      s->pending.pop_front();
+      j = h;
+      j->slot = s->free_slots.back();
+      s->free_slots.pop_back();
+      s->bytes_live += j->rep.bytes;
+      s->st.bytes_peak = std::max(s->st.bytes_peak, s->bytes_live);
+    }
+    const bool fp32 = s->R.fp32;
+    uint8_t* mem = nullptr;
+    HIP_TRY(hipMalloc(&mem, (size_t)j->rep.bytes));
+    carve(&j->rep, mem, fp32);
+    int rc = init_device(&j->rep, &s->scratch, s->R.s0);
+    if (rc) return rc;
+    if (fp32)
+      HIP_TRY(hipMemcpy(j->rep.dtab, j->rep.tab.data(), j->rep.tab.size() * sizeof(float), hipMemcpyHostToDevice));
+    const akmc::ReplicaDesc D = make_desc(j->rep);
+    akmc::ReplicaState st = initial_state(j->rep);
+    st.target = j->target;
+    HIP_TRY(hipMemcpy(s->R.d_desc + j->slot, &D, sizeof(D), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->R.d_state + j->slot, &st, sizeof(st), hipMemcpyHostToDevice));
+    s->cap_of_slot[j->slot] = j->rep.cap;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->slot_job[j->slot] = j;
+      j->status = AKMC_JOB_RUNNING;
+      j->t_admit = s->now();
+      s->st.admitted++;
+      s->st.live++;
+      s->st.bytes_live = s->bytes_live;
+    }
+    *changed = true;
+  }
+}
+
+int retire(akmc_server* s, Job* j, int32_t rep_status) {
+  if (s->opt.keep_results) {
+    j->result.resize(j->rep.nsites);
+    HIP_TRY(hipMemcpy(j->result.data(), j->rep.sites, j->rep.nsites, hipMemcpyDeviceToHost));
+  }
+  HIP_TRY(hipFree(j->rep.mem));
+  j->rep.mem = nullptr;
+  std::lock_guard<std::mutex> lk(s->mu);
+  j->rep_status = rep_status;
+  j->status = rep_status < 0 ? AKMC_JOB_FAILED : AKMC_JOB_DONE;
+  j->t_done = s->now();
+  s->slot_job[j->slot] = nullptr;
+  s->free_slots.push_back(j->slot);
+  j->slot = -1;
+  s->bytes_live -= j->rep.bytes;
+  s->st.bytes_live = s->bytes_live;
+  s->st.live--;
+  if (rep_status < 0) s->st.failed++; else s->st.done++;
+  s->cv.notify_all();
+  return AKMC_OK;
+}
+
+int serve_step(akmc_server* s) {
+  bool changed = false;
+  int rc = admit(s, &changed);
+  if (rc) return rc;
+  std::vector<int32_t> live;
+  int32_t hi = 0;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (int32_t k = 0; k < (int32_t)s->slot_job.size(); k++)
+      if (s->slot_job[k]) { live.push_back(k); hi = k + 1; }
+  }
+  if (live.empty()) return AKMC_OK;
+  if (changed) {
+    rc = s->R.set_items(live, s->cap_of_slot);
+    if (rc) return rc;
+  }
+  const double t_a = s->now();
+  rc = s->R.launch();
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(s->R.s0));
+  HIP_TRY(hipMemcpy(s->h_state.data(), s->R.d_state, hi * sizeof(akmc::ReplicaState), hipMemcpyDeviceToHost));
+  const double t_b = s->now();
+  bool any_retired = false;
+  int64_t ev = 0;
+  for (int32_t k : live) {
+    Job* j = s->slot_job[k];
+    const akmc::ReplicaState& st = s->h_state[k];
+    ev += st.events - j->events;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      j->events = st.events;
+    }
+    if (st.status != AKMC_REP_RUNNING) {
+      rc = retire(s, j, st.status);
+      if (rc) return rc;
+      any_retired = true;
+    }
+  }
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->st.launches++;
+    s->st.events += ev;
+    s->st.busy_seconds += t_b - t_a;
+  }
+  if (any_retired) {
+    // Rebuild the work lists without the finished jobs on the next step.
+    std::vector<int32_t> rest;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      for (int32_t k = 0; k < (int32_t)s->slot_job.size(); k++)
+        if (s->slot_job[k]) rest.push_back(k);
+    }
+    rc = s->R.set_items(rest, s->cap_of_slot);
+    if (rc) return rc;
+  }
+  return AKMC_OK;
+}
+
+void serve_loop(akmc_server* s) {
+  (void)hipSetDevice(s->R.device);
+  while (!s->stop.load()) {
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      if (s->pending.empty() && s->st.live == 0) {
+        s->cv.wait_for(lk, std::chrono::milliseconds(20));
+        continue;
+      }
+    }
+    const int rc = serve_step(s);
+    if (rc) {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->error = rc;
+      s->error_msg = akmc_host::g_err;
+      s->cv.notify_all();
+      return;
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int akmc_server_options_default(akmc_server_options* o) {
+  if (!o) return fail(AKMC_ERR_INVALID, "null options");
+  memset(o, 0, sizeof(*o));
+  o->hbm_budget = (int64_t)250e9;
+  o->max_live = 4096;
+  o->precision = AKMC_PREC_FP32;
+  o->lds_vcap = 256;
+  o->waves_per_group = 2;
+  o->slice_events = 4096;
+  o->slice_seconds = 2e-3;
+  o->keep_results = 0;
+  return AKMC_OK;
+}
+
+int akmc_server_create(int device, const akmc_server_options* o, akmc_server** out) {
+  if (!out) return fail(AKMC_ERR_INVALID, "null argument");
+  *out = nullptr;
+  akmc_server_options opt;
+  if (o) opt = *o; else akmc_server_options_default(&opt);
+  if (opt.max_live <= 0 || opt.hbm_budget <= 0) return fail(AKMC_ERR_INVALID, "bad server options");
+  if (opt.precision != AKMC_PREC_FP32 && opt.precision != AKMC_PREC_FP64)
+    return fail(AKMC_ERR_INVALID, "bad precision");
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(AKMC_ERR_INVALID, "device %d out of range", device);
+  HIP_TRY(hipSetDevice(device));
+  std::unique_ptr<akmc_server> s(new akmc_server());
+  s->opt = opt;
+  s->t0 = Clock::now();
+  int rc = s->R.init(device, opt.precision == AKMC_PREC_FP32, opt.max_live);
+  if (rc) return rc;
+  s->R.use_graph = false;  // the live set changes between launches
+  s->R.tun.vcap = opt.lds_vcap > 0 ? opt.lds_vcap : (opt.lds_vcap < 0 ? -1 : 256);
+  s->R.tun.waves = opt.waves_per_group > 0 ? opt.waves_per_group : 2;
+  s->R.tun.order = 1;
+  s->R.tun.slice = opt.slice_events > 0 ? opt.slice_events : 4096;
+  s->R.budget_ticks = opt.slice_seconds > 0 ? (uint64_t)(opt.slice_seconds * 100e6) : 0;  // 100 MHz clock
+  s->slot_job.assign(opt.max_live, nullptr);
+  s->cap_of_slot.assign(opt.max_live, 1);
+  s->h_state.resize(opt.max_live);
+  for (int32_t k = opt.max_live - 1; k >= 0; k--) s->free_slots.push_back(k);
+  // Unused slots hold a stopped state.
+  std::vector<akmc::ReplicaState> idle(opt.max_live);
+  for (auto& x : idle) x.status = AKMC_REP_TARGET_DONE;
+  HIP_TRY(hipMemcpy(s->R.d_state, idle.data(), idle.size() * sizeof(akmc::ReplicaState), hipMemcpyHostToDevice));
+  akmc_server* raw = s.release();
+  raw->worker = std::thread(serve_loop, raw);
+  *out = raw;
+  return AKMC_OK;
+}
+
+int akmc_server_submit(akmc_server* s, const akmc_params* p, int64_t target_events, int64_t* job) {
+  if (!s || !p || target_events < 0) return fail(AKMC_ERR_INVALID, "bad arguments");
+  std::unique_ptr<Job> j(new Job());
+  j->p = *p;
+  j->target = target_events;
+  int rc = setup_rep(&j->rep, *p, s->R.fp32);
+  if (rc) return rc;
+  if (j->rep.bytes > s->opt.hbm_budget)
+    return fail(AKMC_ERR_BUDGET, "job needs %lld bytes, budget %lld", (long long)j->rep.bytes,
+                (long long)s->opt.hbm_budget);
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->error) return fail(s->error, "server stopped: %s", s->error_msg.c_str());
+  j->id = s->next_id++;
+  j->t_submit = s->now();
+  if (job) *job = j->id;
+  s->pending.push_back(j.get());
+  s->jobs[j->id] = std::move(j);
+  s->st.submitted++;
+  s->cv.notify_all();
+  return AKMC_OK;
+}
+
+int akmc_server_job_info(akmc_server* s, int64_t id, akmc_job_info* info) {
+  if (!s || !info) return fail(AKMC_ERR_INVALID, "bad arguments");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->jobs.find(id);
+  if (it == s->jobs.end()) return fail(AKMC_ERR_INVALID, "no job %lld", (long long)id);
+  const Job& j = *it->second;
+  memset(info, 0, sizeof(*info));
+  info->status = j.status;
+  info->rep_status = j.rep_status;
+  info->events = j.events;
+  info->bytes = j.rep.bytes;
+  info->t_submit = j.t_submit;
+  info->t_admit = j.t_admit;
+  info->t_done = j.t_done;
+  return AKMC_OK;
+}
+
+int akmc_server_result(akmc_server* s, int64_t id, uint8_t* sites, int64_t n) {
+  if (!s || !sites) return fail(AKMC_ERR_INVALID, "bad arguments");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->jobs.find(id);
+  if (it == s->jobs.end()) return fail(AKMC_ERR_INVALID, "no job %lld", (long long)id);
+  const Job& j = *it->second;
+  if (j.status != AKMC_JOB_DONE) return fail(AKMC_ERR_INVALID, "job %lld has not finished", (long long)id);
+  if ((int64_t)j.result.size() != n) return fail(AKMC_ERR_INVALID, "no kept result of %lld sites", (long long)n);
+  memcpy(sites, j.result.data(), n);
+  return AKMC_OK;
+}
+
+int akmc_server_wait_idle(akmc_server* s, double timeout_s) {
+  if (!s) return fail(AKMC_ERR_INVALID, "null server");
+  const auto until = Clock::now() + std::chrono::duration<double>(timeout_s > 0 ? timeout_s : 1e9);
+  std::unique_lock<std::mutex> lk(s->mu);
+  for (;;) {
+    if (s->error) return fail(s->error, "server stopped: %s", s->error_msg.c_str());
+    if (s->pending.empty() && s->st.live == 0) return AKMC_OK;
+    if (Clock::now() >= until) return fail(AKMC_ERR_INVALID, "timeout");
+    s->cv.wait_for(lk, std::chrono::milliseconds(5));
+  }
+}
+
+int akmc_server_get_stats(akmc_server* s, akmc_server_stats* st) {
+  if (!s || !st) return fail(AKMC_ERR_INVALID, "bad arguments");
+  std::lock_guard<std::mutex> lk(s->mu);
+  *st = s->st;
+  return s->error ? fail(s->error, "server stopped: %s", s->error_msg.c_str()) : AKMC_OK;
+}
+
+double akmc_server_clock(akmc_server* s) { return s ? s->now() : 0.0; }
+
+void akmc_server_destroy(akmc_server* s) {
+  if (!s) return;
+  s->stop.store(true);
+  s->cv.notify_all();
+  if (s->worker.joinable()) s->worker.join();
+  (void)hipSetDevice(s->R.device);
+  (void)hipDeviceSynchronize();
+  for (auto& kv : s->jobs)
+    if (kv.second->rep.mem) (void)hipFree(kv.second->rep.mem);
+  s->R.destroy();
+  delete s;
+}
+
+}  // extern "C"

@@ -26,11 +26,12 @@
 #include "../../include/akmc.h"
 #include "akmc_exp.h"
 #include "akmc_exp_table.h"
+#include "akmc_host.h"
 #include "akmc_model.h"
 #include "akmc_plain.h"
 #include "akmc_yaml.h"
 
-namespace {
+namespace akmc_host {
 
 thread_local std::string g_err;
 
@@ -43,14 +44,6 @@ int fail(int code, const char* fmt, ...) {
   g_err = buf;
   return code;
 }
-
-#define HIP_TRY(expr)                                                                         \
-  do {                                                                                        \
-    hipError_t _e = (expr);                                                                   \
-    if (_e != hipSuccess) return fail(AKMC_ERR_HIP, "%s: %s", #expr, hipGetErrorString(_e)); \
-  } while (0)
-
-constexpr int64_t kChunkMax = int64_t(1) << 20;  // events per random-number chunk (default and maximum)
 
 // Builds the 55-site rate stencil tables (see akmc_model.h).
 void build_stencil(akmc::StencilTables* S) {
@@ -99,6 +92,16 @@ void build_stencil(akmc::StencilTables* S) {
     }
   }
 }
+
+}  // namespace akmc_host
+
+namespace {
+
+using akmc_host::build_stencil;
+using akmc_host::fail;
+using akmc_host::g_err;
+
+constexpr int64_t kChunkMax = int64_t(1) << 20;  // events per random-number chunk (default and maximum)
 
 int lattype_from_name(const std::string& s) {
   static const std::map<std::string, int> m = {

@@ -36,7 +36,7 @@ __constant__ double c_e1[25] = AKMC_E1_TABLE;
 __constant__ double c_e2[25] = AKMC_E2_TABLE;
 __constant__ double c_e0[5] = AKMC_E0_TABLE;
 
-static constexpr int PLAIN_VCAP = 1024;
+static constexpr int PLAIN_VCAP = 1536;
 static const int kRing[8] = {0, 7, 2, 5, 3, 4, 1, 6};  // SectorMeta ring, sublattice.h:21-30
 
 using PlainSector = SectorLDS<PLAIN_VCAP, double>;
@@ -67,20 +67,22 @@ __device__ int collect_sector(PlainSector& S, const LatticeView& L, int s) {
     n += __popcll(m);
   }
   if (n > PLAIN_VCAP) return -1;
+  wave_sync();
+  // Bitonic network in its all-ascending form: each stage first compares i
+  // with its mirror i ^ (k - 1), then with i ^ d for d = k/4 .. 1.  Every
+  // comparator puts the smaller key at the lower index, so positions >= n act
+  // as +infinity and are never read or written: no padding beyond n.
   int n2 = 1;
   while (n2 < n) n2 <<= 1;
-  for (int i = n + l; i < n2; i += 64) { S.key[i] = 0xFFFFFFFFu; S.slot[i] = -1; }
-  wave_sync();
   for (int k = 2; k <= n2; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = l; i < n2; i += 64) {
-        const int ixj = i ^ j;
-        if (ixj > i) {
-          const bool up = (i & k) == 0;
-          const uint32_t a = S.key[i], b = S.key[ixj];
-          if ((a > b) == up) {
-            S.key[i] = b; S.key[ixj] = a;
-            const int32_t t = S.slot[i]; S.slot[i] = S.slot[ixj]; S.slot[ixj] = t;
+    for (int d = k - 1; d > 0; d = (d == k - 1) ? (k >> 2) : (d >> 1)) {
+      for (int i = l; i < n; i += 64) {
+        const int p = i ^ d;
+        if (p > i && p < n) {
+          const uint32_t a = S.key[i], b = S.key[p];
+          if (a > b) {
+            S.key[i] = b; S.key[p] = a;
+            const int32_t t = S.slot[i]; S.slot[i] = S.slot[p]; S.slot[p] = t;
           }
         }
       }

@@ -98,6 +98,9 @@ TRAJ_CASES = [
                          seeds=dict(create_types=21, create_vacancy=22, time_inc=23)), 10000),
     ("hot_long_T", make_cfg(n=20, va=30, T=700.0, sector_T=2e-6), 20000),
     ("cluster_heavy", make_cfg(n=16, va=400, T=560.0), 10000),
+    # 1024 < vacancies per sector <= capacity: a sector list that is not a
+    # power of two above 1024 (the sort must not touch entries past the count).
+    ("crowded_sector", make_cfg(n=32, va=9400, T=573.0), 300),
 ]
 
 
@@ -158,7 +161,7 @@ def test_time_loop_end_and_empty_box():
 
 def test_sector_capacity_error():
     say("test_sector_capacity_error")
-    cfg = make_cfg(n=32, va=12000)  # ~1500 vacancies per sector > 1024
+    cfg = make_cfg(n=32, va=16000)  # ~2000 vacancies per sector > 1536
     eng = akmc.Engine(cfg)
     with pytest.raises(akmc.AkmcError) as e:
         eng.run(10)
