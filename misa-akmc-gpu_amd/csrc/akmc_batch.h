@@ -109,7 +109,7 @@ struct BatchArgs {
   const int32_t* items;     // replica index of each work item
   int32_t n_items;
   int32_t flags;            // AKMC_BF_*
-  uint64_t* queue;          // [0] work-queue head, [1] launch start tick (set by the reset kernel)
+  uint64_t* queue;          // [1]: launch start tick (set by the reset kernel)
   int64_t slice_events;     // events per replica per launch
   uint64_t budget_ticks;    // s_memrealtime ticks (100 MHz) after the launch start
                             // at which waves stop at the next event (0: none)

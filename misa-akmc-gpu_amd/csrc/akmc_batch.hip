@@ -372,11 +372,12 @@ __device__ void run_item(const BatchArgs& a, const RTabs& T, RStage& G, uint8_t*
     if (l == 0) a.state[r].nv = bad ? -1 : nv;
     return;
   }
-  if (LDS && st.in_sector) {
-    copy_sector(S, Sg, st.nv);
-  }
+  if (LDS && st.in_sector && st.nv >= 0) copy_sector(S, Sg, st.nv);
   const uint64_t t_start = a.queue[1];
-  int64_t done = 0;
+  // Work budget of this item: events plus sector visits, so that a launch
+  // ends even when a replica performs no events.
+  int64_t budget = a.slice_events;
+  int zero_run = 0;  // consecutive sector visits with a zero total and no event
   int32_t status = AKMC_REP_RUNNING;
   const Patch nopatch{0xFFFFFFFFu, 0xFFFFFFFFu, 0, 0, -1};
   (void)nopatch;
@@ -387,22 +388,31 @@ __device__ void run_item(const BatchArgs& a, const RTabs& T, RStage& G, uint8_t*
       // step_threshold_time (sublattice.inl:72)
       st.thr = static_cast<double>(st.step + 1) * D.sector_T - G.evo[s0];
       st.t = 0.0;
-      st.nv = rebuild<RateT, FP32>(S, G, T, R, s0, a.flags, E, bad);
-      st.n_rebuild++;
-      st.n_eval += st.nv;
       st.in_sector = 1;
-      if (bad) { status = AKMC_REP_ERR_RANGE; break; }
+      st.nv = -1;  // the sector's rates are evaluated at its first event
     }
     const int s = ring_sector(st.sect);
     if (!(st.t < st.thr)) {
       if (l == 0) G.evo[s] += st.t;  // sublattice.inl:140
       wave_sync();
+      // A sector that ended before its rates were evaluated (its threshold
+      // was already passed) says nothing about its total: restart the count
+      // of empty sectors.
+      if (st.nv < 0) zero_run = 0;
       st.in_sector = 0;
       if (++st.sect == 8) { st.sect = 0; st.step++; }
+      budget--;
       continue;
     }
     if (st.events >= st.target) { status = AKMC_REP_TARGET_DONE; break; }
-    if (done >= a.slice_events) break;
+    if (budget <= 0) break;
+    if (st.nv < 0) {
+      // calcRates of the sector's first event (kmc.cpp:68-115).
+      st.nv = rebuild<RateT, FP32>(S, G, T, R, s, a.flags, E, bad);
+      st.n_rebuild++;
+      st.n_eval += st.nv;
+      if (bad) { status = AKMC_REP_ERR_RANGE; break; }
+    }
     if (a.budget_ticks && (uint64_t)wall_clock64() - t_start > a.budget_ticks) break;
 
     // ---- total and selection (three scan levels).
@@ -422,6 +432,19 @@ __device__ void run_item(const BatchArgs& a, const RTabs& T, RStage& G, uint8_t*
     if (sum == 0.0 || fabs(sum) < 2.220446049250313e-16) {
       st.t = st.thr;
       st.n_jump++;
+      if (++zero_run >= 8) {
+        // Eight consecutive sectors (the whole ring) evaluated with a zero
+        // total and no event between them: the lattice can no longer
+        // change, so every later sector is empty too.  Jump to the end of
+        // the time loop.
+        if (l < 8) G.evo[l] = (double)D.time_steps * D.sector_T;
+        wave_sync();
+        st.step = D.time_steps;
+        st.sect = 0;
+        st.in_sector = 0;
+        status = AKMC_REP_TIME_DONE;
+        break;
+      }
       continue;
     }
     const double u = (double)(xo_next(st.rng) >> 11) * 0x1p-53;  // [0, 1)
@@ -497,7 +520,8 @@ __device__ void run_item(const BatchArgs& a, const RTabs& T, RStage& G, uint8_t*
     // ---- residence time (sublattice.inl:118-120).
     st.t += -log(rr) / sum;
     st.events++;
-    done++;
+    budget--;
+    zero_run = 0;
     if (!(st.t < st.thr)) continue;  // the sector ends before any further evaluation
 
     // ---- re-evaluate the vacancies whose stencil holds `from` or `to`.
@@ -544,7 +568,7 @@ __device__ void run_item(const BatchArgs& a, const RTabs& T, RStage& G, uint8_t*
     }
     if (bad) { status = AKMC_REP_ERR_RANGE; break; }
   }
-  if (LDS && st.in_sector) copy_sector(Sg, S, st.nv);
+  if (LDS && st.in_sector && st.nv >= 0) copy_sector(Sg, S, st.nv);
   // Error statistics of the FP32 rates, reduced over the wave.
   if (FP32 && (a.flags & AKMC_BF_ERROR_STATS)) {
     const double emax = wave_max_d(E.emax);
@@ -576,19 +600,20 @@ __global__ __launch_bounds__(256) void replica_kernel(BatchArgs a) {
     for (int i = threadIdx.x; i < 256; i += blockDim.x) T.exp_tab[i] = r_exp_tab[i];
   }
   __syncthreads();
-  const int wave = threadIdx.x >> 6;
+  // Work items are dealt to the waves of the grid round-robin.  The item
+  // index is computed from uniform values only (never from a value that one
+  // lane produced), so every lane of a wave runs the same item sequence.
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int waves = (int)(blockDim.x >> 6);
   uint8_t* region = dyn + (size_t)wave * a.region_bytes;
   RStage& G = *reinterpret_cast<RStage*>(region);
   uint8_t* lds_sec = region + RSTAGE_BYTES;
-  for (;;) {
-    uint64_t item = 0;
-    if (lane_id() == 0) item = atomicAdd((unsigned long long*)&a.queue[0], 1ull);
-    item = (uint64_t)__builtin_amdgcn_readfirstlane((int)item);
-    if (item >= (uint64_t)a.n_items) break;
+  const int stride = (int)gridDim.x * waves;
+  for (int item = (int)blockIdx.x * waves + wave; item < a.n_items; item += stride)
     run_item<RateT, FP32, LDS>(a, T, G, lds_sec, a.items[item]);
-  }
 }
 
+// Records the launch start tick (queue[1]) for the time budget.
 __global__ void reset_kernel(uint64_t* q) {
   if (threadIdx.x == 0) {
     q[0] = 0;

@@ -531,6 +531,36 @@ int set_targets(akmc_batch* b, int64_t events) {
   return write_states(b);
 }
 
+// Advances every running replica by exactly `events` events (fewer only when
+// its time loop ends or it fails): launches ceil(events / slice) slices, then
+// more while a replica is short (a slice's budget also counts sector visits).
+int advance(akmc_batch* b, int64_t events, int64_t* total, double* ksec, int64_t* launches) {
+  int rc = read_states(b);
+  if (rc) return rc;
+  rc = set_targets(b, events);
+  if (rc) return rc;
+  *total = 0;
+  *ksec = 0;
+  *launches = 0;
+  int64_t need = (events + b->R.tun.slice - 1) / b->R.tun.slice;
+  while (need > 0) {
+    int64_t ev = 0;
+    double ks = 0;
+    rc = run_slices(b, need, &ev, &ks);
+    if (rc) return rc;
+    *total += ev;
+    *ksec += ks;
+    *launches += need;
+    need = 0;
+    for (const auto& s : b->h_state)
+      if (s.status == AKMC_REP_RUNNING && s.events < s.target) need = 1;
+  }
+  // Replicas that reached this target continue in the next call.
+  for (auto& s : b->h_state)
+    if (s.status == AKMC_REP_TARGET_DONE) s.status = AKMC_REP_RUNNING;
+  return write_states(b);
+}
+
 }  // namespace
 
 extern "C" {
@@ -643,25 +673,10 @@ int akmc_batch_run(akmc_batch* b, int64_t events, akmc_batch_stats* stats) {
   if (!b || events < 0) return fail(AKMC_ERR_INVALID, "bad arguments");
   HIP_TRY(hipSetDevice(b->R.device));
   const auto t0 = std::chrono::steady_clock::now();
-  int rc = read_states(b);
-  if (rc) return rc;
-  rc = set_targets(b, events);
-  if (rc) return rc;
   int64_t total = 0, launches = 0;
   double ksec = 0;
-  int64_t need = (events + b->R.tun.slice - 1) / b->R.tun.slice;
-  while (need > 0) {
-    int64_t ev = 0;
-    double ks = 0;
-    rc = run_slices(b, need, &ev, &ks);
-    if (rc) return rc;
-    total += ev;
-    ksec += ks;
-    launches += need;
-    need = 0;
-    for (const auto& s : b->h_state)
-      if (s.status == AKMC_REP_RUNNING && s.events < s.target) need = 1;
-  }
+  const int rc = advance(b, events, &total, &ksec, &launches);
+  if (rc) return rc;
   if (stats) {
     memset(stats, 0, sizeof(*stats));
     stats->events = total;
@@ -669,14 +684,11 @@ int akmc_batch_run(akmc_batch* b, int64_t events, akmc_batch_stats* stats) {
     stats->kernel_seconds = ksec;
     stats->launches = launches;
     for (const auto& s : b->h_state) {
-      if (s.status == AKMC_REP_RUNNING || s.status == AKMC_REP_TARGET_DONE) stats->running++;
+      if (s.status == AKMC_REP_RUNNING) stats->running++;
       if (s.status < 0) stats->failed++;
     }
   }
-  // Replicas that reached this run's target continue in the next run.
-  for (auto& s : b->h_state)
-    if (s.status == AKMC_REP_TARGET_DONE) s.status = AKMC_REP_RUNNING;
-  return write_states(b);
+  return AKMC_OK;
 }
 
 int akmc_batch_get_tuning(akmc_batch* b, akmc_batch_tuning* t) {
@@ -716,13 +728,9 @@ int akmc_batch_tune(akmc_batch* b, int64_t trial_events, akmc_batch_tuning* chos
         if (vc > 0 && (int64_t)w * akmc_batch_region_bytes(b->R.fp32, vc) > 160 * 1024) continue;
         b->R.tun = t;
         if (b->R.set_items(b->live, b->cap_of_slot)) continue;  // does not fit: skip
-        int rc = read_states(b);
-        if (rc) return rc;
-        rc = set_targets(b, trial_events);
-        if (rc) return rc;
-        int64_t ev = 0;
+        int64_t ev = 0, nl = 0;
         double ks = 0;
-        rc = run_slices(b, 1, &ev, &ks);
+        const int rc = advance(b, trial_events, &ev, &ks, &nl);
         if (rc) return rc;
         trials++;
         const double rate = ks > 0 ? ev / ks : 0;
@@ -731,13 +739,9 @@ int akmc_batch_tune(akmc_batch* b, int64_t trial_events, akmc_batch_tuning* chos
           best = t;
         }
       }
-  for (auto& s : b->h_state)
-    if (s.status == AKMC_REP_TARGET_DONE) s.status = AKMC_REP_RUNNING;
-  int rc = write_states(b);
-  if (rc) return rc;
   best.slice = keep.slice;
   b->R.tun = best;
-  rc = b->R.set_items(b->live, b->cap_of_slot);
+  const int rc = b->R.set_items(b->live, b->cap_of_slot);
   if (rc) return rc;
   b->last_tune.events_per_second = best_rate;
   b->last_tune.trials = trials;
