@@ -9,6 +9,17 @@ akmc_run of --events-per-step events; the initial occupancy (host) and the
 upload are outside the timed region, so the lattice is resident in HBM when
 timing starts.  value = events performed in the timed steps / seconds.
 
+Beside the plain line, the same JSON line carries two more legs of the same
+metric (disable with --no-batched / --no-served):
+  batched  the replica engine on 1,024 independent replicas (edge U{16..96},
+           random Fe-Cu-Ni-Mn compositions, T U[500,700] K, 2e-4 vacancies),
+           FP32 rates with FP64 sums, autotuned, hipGraph replays; a "step"
+           advances every replica by --batch-events events.
+  served   the serving loop on a Poisson stream of --jobs jobs (edge U{32..320},
+           vacancies U[1e-4,5e-4], targets log-uniform in [1e5,1e7] events scaled
+           by --serve-scale), admitted by a 250 GB HBM budget; reports
+           throughput and p50/p99 completion latency.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--events-per-step E]
 """
 import argparse
@@ -59,6 +70,88 @@ def cpu_baseline(events, seconds_budget):
                       "1 thread, initial occupancy excluded, %.1f s" % (done, dt)}
 
 
+def batched_leg(args, plain_value):
+    """Replica engine throughput (events of all replicas per second)."""
+    import akmc
+    from akmc import workloads
+    cfgs = workloads.replicas(args.replicas)
+    t0 = time.perf_counter()
+    b = akmc.Batch(cfgs, device=0, precision=akmc.AKMC_PREC_FP32, init=akmc.AKMC_INIT_DEVICE,
+                   slice_events=args.batch_slice)
+    t_create = time.perf_counter() - t0
+    print("bench: %d replicas created in %.1f s" % (len(cfgs), t_create), file=sys.stderr, flush=True)
+    tun = b.tune(trial_events=args.tune_events)
+    print("bench: tuning %s" % tun, file=sys.stderr, flush=True)
+    b.run(args.batch_events)  # warmup
+    akmc.sync(0)
+    ev, ksec, launches = 0, 0.0, 0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        st = b.run(args.batch_events)
+        ev += st["events"]
+        ksec += st["kernel_seconds"]
+        launches += st["launches"]
+    akmc.sync(0)
+    dt = time.perf_counter() - t0
+    n_eval = sum(b.replica_stats(i)["n_eval"] for i in range(b.n))
+    nvac = sum(b.replica_stats(i)["nvac"] for i in range(b.n))
+    sites = sum(b.num_sites(i) for i in range(b.n))
+    b.close()
+    # FP32 rate error against FP64 on a subset (every rate of every evaluation).
+    sub = akmc.Batch(cfgs[:64], device=0, precision=akmc.AKMC_PREC_FP32, init=akmc.AKMC_INIT_DEVICE, error_stats=1)
+    sub.run(2000)
+    errs = [sub.replica_stats(i) for i in range(sub.n)]
+    sub.close()
+    value = ev / dt
+    return {"value": value, "unit": "events/s", "replicas": len(cfgs), "sites": sites, "vacancies": nvac,
+            "events_per_step": args.batch_events * len(cfgs), "events_timed": ev, "ms_per_step": dt / args.steps * 1e3,
+            "kernel_seconds": ksec, "launches": launches, "dtype": "f32 rates, f64 sums/search/dt",
+            "tuning": tun, "vs_plain": value / plain_value if plain_value else None,
+            "evaluations_per_event": n_eval / max(1, sum(1 for _ in cfgs)) / max(1, ev) * len(cfgs),
+            "rate_error": {"max_rel": max(e["err_max"] for e in errs),
+                           "mean_rel": sum(e["err_mean"] * e["err_n"] for e in errs) / max(1, sum(e["err_n"] for e in errs)),
+                           "rates_compared": sum(e["err_n"] for e in errs),
+                           "sample": "64 replicas x 2000 events, every rate of every evaluation"}}
+
+
+def served_leg(args, batched_value):
+    """Serving loop on a Poisson job stream: throughput and completion latency."""
+    import random
+    import akmc
+    from akmc import workloads
+    jobs = workloads.jobs(args.jobs, ev_lo=1e5 * args.serve_scale, ev_hi=1e7 * args.serve_scale)
+    mean_events = sum(t for _, t in jobs) / len(jobs)
+    # Offered rate: about 70% of the replica engine's throughput, in jobs/s.
+    rate = args.serve_rate or max(1.0, 0.7 * batched_value / mean_events)
+    srv = akmc.Server(device=0, hbm_budget=int(250e9))
+    rng = random.Random(12345)
+    t_start = srv.clock()
+    t_next = t_start
+    ids = []
+    for cfg, target in jobs:
+        t_next += rng.expovariate(rate)
+        while True:
+            now = srv.clock()
+            if now >= t_next:
+                break
+            time.sleep(min(0.002, t_next - now))
+        ids.append((srv.submit(cfg, target), target))
+    srv.wait_idle(0)
+    st = srv.stats()
+    infos = [srv.info(j) for j, _ in ids]
+    srv.close()
+    lat = sorted(i["t_done"] - i["t_submit"] for i in infos)
+    q = lambda p: lat[min(len(lat) - 1, int(p * len(lat)))]
+    span = max(i["t_done"] for i in infos) - t_start
+    ev = sum(i["events"] for i in infos)
+    return {"value": ev / span, "unit": "events/s", "jobs": len(ids), "offered_jobs_per_s": rate,
+            "mean_target_events": mean_events, "target_scale": args.serve_scale,
+            "p50_latency_s": q(0.50), "p99_latency_s": q(0.99), "max_latency_s": lat[-1],
+            "makespan_s": span, "gpu_busy_frac": st["busy_seconds"] / span, "launches": st["launches"],
+            "hbm_peak_bytes": st["bytes_peak"], "failed": st["failed"],
+            "data": "synthetic jobs; vacancy-only (self-interstitial dumbbells are not modelled)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +163,17 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run the streaming copy kernel (profiler counter calibration)")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--replicas", type=int, default=1024)
+    ap.add_argument("--batch-events", type=int, default=20000, help="events per replica per step")
+    ap.add_argument("--batch-slice", type=int, default=4096)
+    ap.add_argument("--tune-events", type=int, default=256)
+    ap.add_argument("--jobs", type=int, default=4000)
+    ap.add_argument("--serve-scale", type=float, default=0.01,
+                    help="scale of the job targets (1 = the full 1e5..1e7 events)")
+    ap.add_argument("--serve-rate", type=float, default=0.0, help="offered jobs/s (0: 70%% of batched capacity)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (one lattice per GPU; replica and job "
@@ -78,49 +182,58 @@ def main():
         return 2
 
     import akmc
-    print("bench: creating the 256^3 lattice", file=sys.stderr, flush=True)
-    eng = akmc.Engine(WORKLOAD, device=0)
-    E = args.events_per_step
-    for _ in range(args.warmup):
-        eng.run(E)
-    akmc.sync(0)
-    s0 = eng.stats()
-    t0 = time.perf_counter()
-    ksec = 0.0
-    launches = 0
-    for i in range(args.steps):
-        st = eng.run(E)
-        ksec += st["kernel_seconds"]
-        launches += st["launches"]
-        print("bench: step %d/%d events %d" % (i + 1, args.steps, st["events"]), file=sys.stderr, flush=True)
-    akmc.sync(0)
-    dt = time.perf_counter() - t0
-    s1 = eng.stats()
-    d_events = s1["events"] - s0["events"]
-    value = d_events / dt
-    nbytes = algorithmic_bytes(d_events, s1["n_eval"] - s0["n_eval"], s1["n_fold"] - s0["n_fold"],
-                               s1["n_shift"] - s0["n_shift"])
-    per_launch = nbytes / max(1, launches)
-    avg_launch_s = ksec / max(1, launches)
-    achieved = per_launch / avg_launch_s / 1e9
+    out = {"metric": METRIC, "value": None, "unit": "events/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic"}
+    value = None
+    if not args.no_plain:
+        print("bench: creating the 256^3 lattice", file=sys.stderr, flush=True)
+        eng = akmc.Engine(WORKLOAD, device=0)
+        E = args.events_per_step
+        for _ in range(args.warmup):
+            eng.run(E)
+        akmc.sync(0)
+        s0 = eng.stats()
+        t0 = time.perf_counter()
+        ksec = 0.0
+        launches = 0
+        for i in range(args.steps):
+            st = eng.run(E)
+            ksec += st["kernel_seconds"]
+            launches += st["launches"]
+            print("bench: step %d/%d events %d" % (i + 1, args.steps, st["events"]), file=sys.stderr, flush=True)
+        akmc.sync(0)
+        dt = time.perf_counter() - t0
+        s1 = eng.stats()
+        eng.close()
+        d_events = s1["events"] - s0["events"]
+        value = d_events / dt
+        nbytes = algorithmic_bytes(d_events, s1["n_eval"] - s0["n_eval"], s1["n_fold"] - s0["n_fold"],
+                                   s1["n_shift"] - s0["n_shift"])
+        per_launch = nbytes / max(1, launches)
+        avg_launch_s = ksec / max(1, launches)
+        achieved = per_launch / avg_launch_s / 1e9
+        out.update({
+            "value": value, "ms_per_step": dt / args.steps * 1e3,
+            "config": {"workload": "plain FP64 AKMC: BCC 256^3 cells (33,554,432 sites), Fe-1.34Cu-1.30Mn-0.70Ni "
+                                   "at%, 6711 vacancies, T=573 K, nu=6e12/s, 8-sector sublattice, T_sector=1e-6 s",
+                       "events_per_step": E, "events_timed": d_events, "launches": launches},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "algorithmic_bytes_per_event": nbytes / max(1, d_events),
+                         "kernel_us_per_event": ksec / max(1, d_events) * 1e6}})
     if args.copy_calibration:
         akmc.copy_bench(1 << 30, iters=5)
-    cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_events, args.cpu_seconds)
-    out = {
-        "metric": METRIC, "value": value, "unit": "events/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "plain FP64 AKMC: BCC 256^3 cells (33,554,432 sites), Fe-1.34Cu-1.30Mn-0.70Ni at%, "
-                               "6711 vacancies, T=573 K, nu=6e12/s, 8-sector sublattice, T_sector=1e-6 s",
-                   "events_per_step": E, "events_timed": d_events, "launches": launches},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "algorithmic_bytes_per_event": nbytes / max(1, d_events),
-                     "kernel_us_per_event": ksec / max(1, d_events) * 1e6},
-        "cpu_baseline": cpu,
-    }
+    batched = None
+    if not args.no_batched:
+        batched = batched_leg(args, value)
+        out["batched"] = batched
+        print("bench: batched %.3g events/s" % batched["value"], file=sys.stderr, flush=True)
+    if not args.no_served:
+        out["served"] = served_leg(args, batched["value"] if batched else 5e7)
+        print("bench: served %.3g events/s" % out["served"]["value"], file=sys.stderr, flush=True)
+    out["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(args.cpu_events, args.cpu_seconds)
     print(json.dumps(out))
-    eng.close()
     return 0
 
 

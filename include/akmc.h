@@ -235,6 +235,7 @@ typedef struct akmc_replica_stats {
   double err_mean;
   int64_t err_n;            /* rates compared */
   int64_t err_tiny;         /* FP64 rates below FLT_MIN, excluded */
+  int64_t n_skip;           /* time steps skipped: every occupied sector already past its threshold */
 } akmc_replica_stats;
 
 typedef struct akmc_batch_tuning {

@@ -101,6 +101,8 @@ struct ReplicaState {
   double err_sum;
   int64_t err_n;
   int64_t err_tiny;     // rates below FLT_MIN (FP64 value), not in the error
+  int32_t sec_count[8]; // vacancies in each sector (kept by the kernel)
+  int64_t n_skip;       // time steps skipped because no sector could have an event
 };
 
 struct BatchArgs {

@@ -46,6 +46,7 @@ struct RTabs {
 struct RStage {
   float tab[256 + BATCH_NHI_MAX];  // FP32 exp table of the replica: lo[256], hi[nhi]
   double evo[8];
+  int32_t cnt[8];                  // vacancies per sector
   uint8_t sten[BATCH_EVB][64];
   int32_t list[64];
   uint8_t cflag[BATCH_EVB];
@@ -359,7 +360,10 @@ __device__ void run_item(const BatchArgs& a, const RTabs& T, RStage& G, uint8_t*
   const RView R{D.sites, D.vac_key, D.nvac, D.d, D.nu, D.kT, D.nhi, D.m_min};
   if (FP32)
     for (int i = l; i < 256 + D.nhi; i += 64) G.tab[i] = D.exp_tab[i];
-  if (l < 8) G.evo[l] = a.state[r].evo[l];
+  if (l < 8) {
+    G.evo[l] = a.state[r].evo[l];
+    G.cnt[l] = a.state[r].sec_count[l];
+  }
   wave_sync();
   const Sec<RateT> Sg = sec_at<RateT>(D.ws, D.cap);
   const Sec<RateT> S = LDS ? sec_at<RateT>(lds_sec, a.vcap) : Sg;
@@ -384,6 +388,32 @@ __device__ void run_item(const BatchArgs& a, const RTabs& T, RStage& G, uint8_t*
   for (;;) {
     if (!st.in_sector) {
       if (st.step >= D.time_steps) { status = AKMC_REP_TIME_DONE; break; }
+      if (st.sect == 0) {
+        // Time steps in which every sector holding a vacancy is already past
+        // its threshold have no event: those sectors end at once and the
+        // empty ones only advance their clocks to (step + 1) T.  Skip them:
+        // jump to the first step at which an occupied sector can run.  The
+        // jump depends on the state alone, so it is taken wherever a run
+        // reaches this ring start.
+        int64_t K = INT64_MAX;
+        for (int q = 0; q < 8; q++) {
+          if (G.cnt[q] == 0) continue;
+          const double e = G.evo[q];
+          int64_t j = max(st.step, (int64_t)ceil(e / D.sector_T) - 1);
+          while (!(static_cast<double>(j + 1) * D.sector_T - e > 0.0)) j++;
+          while (j > st.step && static_cast<double>(j) * D.sector_T - e > 0.0) j--;
+          K = min(K, j);
+        }
+        if (K != INT64_MAX && K > st.step) {
+          K = min(K, D.time_steps);
+          const double tk = static_cast<double>(K) * D.sector_T;
+          if (l < 8 && G.cnt[l] == 0) G.evo[l] = fmax(G.evo[l], tk);
+          wave_sync();
+          st.n_skip += K - st.step;
+          st.step = K;
+          if (st.step >= D.time_steps) { status = AKMC_REP_TIME_DONE; break; }
+        }
+      }
       const int s0 = ring_sector(st.sect);
       // step_threshold_time (sublattice.inl:72)
       st.thr = static_cast<double>(st.step + 1) * D.sector_T - G.evo[s0];
@@ -406,6 +436,22 @@ __device__ void run_item(const BatchArgs& a, const RTabs& T, RStage& G, uint8_t*
     }
     if (st.events >= st.target) { status = AKMC_REP_TARGET_DONE; break; }
     if (budget <= 0) break;
+    if (st.nv < 0 && G.cnt[s] == 0) {
+      // An empty sector: its total is zero without evaluating anything.
+      st.nv = 0;
+      st.t = st.thr;
+      st.n_jump++;
+      if (++zero_run >= 8) {
+        if (l < 8) G.evo[l] = (double)D.time_steps * D.sector_T;
+        wave_sync();
+        st.step = D.time_steps;
+        st.sect = 0;
+        st.in_sector = 0;
+        status = AKMC_REP_TIME_DONE;
+        break;
+      }
+      continue;
+    }
     if (st.nv < 0) {
       // calcRates of the sector's first event (kmc.cpp:68-115).
       st.nv = rebuild<RateT, FP32>(S, G, T, R, s, a.flags, E, bad);
@@ -515,6 +561,8 @@ __device__ void run_item(const BatchArgs& a, const RTabs& T, RStage& G, uint8_t*
         S.slot[v] = -1;
         S.vt[v] = 0.0;
         S.own[v] = 0;
+        G.cnt[s] -= 1;
+        G.cnt[key_sector(R.d, to)] += 1;
       }
     }
     // ---- residence time (sublattice.inl:118-120).
@@ -580,7 +628,10 @@ __device__ void run_item(const BatchArgs& a, const RTabs& T, RStage& G, uint8_t*
     st.err_tiny += (int64_t)et;
   }
   st.status = status;
-  for (int i = 0; i < 8; i++) st.evo[i] = G.evo[i];
+  for (int i = 0; i < 8; i++) {
+    st.evo[i] = G.evo[i];
+    st.sec_count[i] = G.cnt[i];
+  }
   wave_sync();
   if (l == 0) a.state[r] = st;
 }

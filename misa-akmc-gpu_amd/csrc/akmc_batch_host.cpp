@@ -110,6 +110,7 @@ struct HostRep {
   double kT = 0;
   int64_t time_steps = 0;
   int32_t nhi = 0, m_min = 0;
+  int32_t sec_count[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // vacancies per sector
   std::vector<float> tab;
   int64_t bytes = 0;
   uint8_t* mem = nullptr;  // base of this replica's device block
@@ -184,12 +185,24 @@ akmc::ReplicaState initial_state(const HostRep& r) {
   uint64_t x = (uint64_t)r.p.seed_time_inc;
   for (int k = 0; k < 4; k++) st.rng[k] = splitmix_next(x);
   st.status = AKMC_REP_RUNNING;
+  for (int q = 0; q < 8; q++) st.sec_count[q] = r.sec_count[q];
   return st;
 }
 
 uint32_t key_of(const akmc::BoxDims& d, int64_t i) {
   const uint32_t x = (uint32_t)(i % d.X), y = (uint32_t)((i / d.X) % d.Y), z = (uint32_t)(i / ((int64_t)d.X * d.Y));
   return x | (y << 11) | (z << 22);
+}
+
+// Sector id of a key: bit0 x high, bit1 y high, bit2 z high.
+int sector_of(const akmc::BoxDims& d, uint32_t k) {
+  return ((int)(k & 0x7FF) >= d.hx ? 1 : 0) | ((int)((k >> 11) & 0x7FF) >= d.hy ? 2 : 0) |
+         ((int)(k >> 22) >= d.hz ? 4 : 0);
+}
+
+void count_sectors(HostRep* r, const std::vector<uint32_t>& keys) {
+  for (int q = 0; q < 8; q++) r->sec_count[q] = 0;
+  for (uint32_t k : keys) r->sec_count[sector_of(r->d, k)]++;
 }
 
 std::vector<uint32_t> vacancy_keys(const akmc::BoxDims& d, const uint8_t* sites, int64_t n) {
@@ -267,6 +280,7 @@ int init_device(HostRep* r, InitScratch* S, hipStream_t st) {
     HIP_TRY(hipMemcpyAsync(r->vac, keys.data(), keys.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
   }
   r->nvac = (int32_t)keys.size();
+  count_sectors(r, keys);
   // The copies above read pageable host vectors that end with this scope.
   HIP_TRY(hipStreamSynchronize(st));
   return AKMC_OK;
@@ -280,6 +294,7 @@ int upload_rep(HostRep* r, const uint8_t* sites, hipStream_t st) {
   if (!keys.empty())
     HIP_TRY(hipMemcpyAsync(r->vac, keys.data(), keys.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
   r->nvac = (int32_t)keys.size();
+  count_sectors(r, keys);
   HIP_TRY(hipStreamSynchronize(st));
   return AKMC_OK;
 }
@@ -473,6 +488,7 @@ void fill_rep_stats(const HostRep& r, const akmc::ReplicaState& st, akmc_replica
   s->err_mean = st.err_n ? st.err_sum / (double)st.err_n : 0.0;
   s->err_n = st.err_n;
   s->err_tiny = st.err_tiny;
+  s->n_skip = st.n_skip;
 }
 
 }  // namespace
@@ -786,6 +802,7 @@ int akmc_batch_upload(akmc_batch* b, int32_t i, const uint8_t* sites, int64_t n)
   const akmc::ReplicaDesc D = make_desc(b->reps[i]);
   HIP_TRY(hipMemcpy(b->R.d_desc + i, &D, sizeof(D), hipMemcpyHostToDevice));
   b->h_state[i].in_sector = 0;
+  for (int q = 0; q < 8; q++) b->h_state[i].sec_count[q] = b->reps[i].sec_count[q];
   return write_states(b);
 }
 
@@ -937,6 +954,8 @@ struct akmc_server {
   akmc_server_stats st{};
   std::atomic<bool> stop{false};
   std::thread worker;
+  hipStream_t init_stream = nullptr;  // job set-up, overlapping the running slice
+  bool items_dirty = false;
   int error = AKMC_OK;
   std::string error_msg;
 
@@ -965,7 +984,7 @@ int admit(akmc_server* s, bool* changed) {
     uint8_t* mem = nullptr;
     HIP_TRY(hipMalloc(&mem, (size_t)j->rep.bytes));
     carve(&j->rep, mem, fp32);
-    int rc = init_device(&j->rep, &s->scratch, s->R.s0);
+    int rc = init_device(&j->rep, &s->scratch, s->init_stream);
     if (rc) return rc;
     if (fp32)
       HIP_TRY(hipMemcpy(j->rep.dtab, j->rep.tab.data(), j->rep.tab.size() * sizeof(float), hipMemcpyHostToDevice));
@@ -1010,10 +1029,10 @@ int retire(akmc_server* s, Job* j, int32_t rep_status) {
   return AKMC_OK;
 }
 
+// One step of the request loop: launch a slice over the live jobs, admit
+// pending jobs while it runs (their set-up overlaps the device work), then
+// collect the slice and retire the jobs that finished.
 int serve_step(akmc_server* s) {
-  bool changed = false;
-  int rc = admit(s, &changed);
-  if (rc) return rc;
   std::vector<int32_t> live;
   int32_t hi = 0;
   {
@@ -1021,18 +1040,26 @@ int serve_step(akmc_server* s) {
     for (int32_t k = 0; k < (int32_t)s->slot_job.size(); k++)
       if (s->slot_job[k]) { live.push_back(k); hi = k + 1; }
   }
-  if (live.empty()) return AKMC_OK;
-  if (changed) {
+  int rc;
+  if (s->items_dirty) {
     rc = s->R.set_items(live, s->cap_of_slot);
     if (rc) return rc;
+    s->items_dirty = false;
   }
   const double t_a = s->now();
-  rc = s->R.launch();
+  const bool launched = !live.empty();
+  if (launched) {
+    rc = s->R.launch();
+    if (rc) return rc;
+  }
+  bool changed = false;
+  rc = admit(s, &changed);
   if (rc) return rc;
+  if (changed) s->items_dirty = true;
+  if (!launched) return AKMC_OK;
   HIP_TRY(hipStreamSynchronize(s->R.s0));
-  HIP_TRY(hipMemcpy(s->h_state.data(), s->R.d_state, hi * sizeof(akmc::ReplicaState), hipMemcpyDeviceToHost));
   const double t_b = s->now();
-  bool any_retired = false;
+  HIP_TRY(hipMemcpy(s->h_state.data(), s->R.d_state, hi * sizeof(akmc::ReplicaState), hipMemcpyDeviceToHost));
   int64_t ev = 0;
   for (int32_t k : live) {
     Job* j = s->slot_job[k];
@@ -1045,26 +1072,13 @@ int serve_step(akmc_server* s) {
     if (st.status != AKMC_REP_RUNNING) {
       rc = retire(s, j, st.status);
       if (rc) return rc;
-      any_retired = true;
+      s->items_dirty = true;
     }
   }
-  {
-    std::lock_guard<std::mutex> lk(s->mu);
-    s->st.launches++;
-    s->st.events += ev;
-    s->st.busy_seconds += t_b - t_a;
-  }
-  if (any_retired) {
-    // Rebuild the work lists without the finished jobs on the next step.
-    std::vector<int32_t> rest;
-    {
-      std::lock_guard<std::mutex> lk(s->mu);
-      for (int32_t k = 0; k < (int32_t)s->slot_job.size(); k++)
-        if (s->slot_job[k]) rest.push_back(k);
-    }
-    rc = s->R.set_items(rest, s->cap_of_slot);
-    if (rc) return rc;
-  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->st.launches++;
+  s->st.events += ev;
+  s->st.busy_seconds += t_b - t_a;
   return AKMC_OK;
 }
 
@@ -1125,6 +1139,7 @@ int akmc_server_create(int device, const akmc_server_options* o, akmc_server** o
   int rc = s->R.init(device, opt.precision == AKMC_PREC_FP32, opt.max_live);
   if (rc) return rc;
   s->R.use_graph = false;  // the live set changes between launches
+  HIP_TRY(hipStreamCreateWithFlags(&s->init_stream, hipStreamNonBlocking));
   s->R.tun.vcap = opt.lds_vcap > 0 ? opt.lds_vcap : (opt.lds_vcap < 0 ? -1 : 256);
   s->R.tun.waves = opt.waves_per_group > 0 ? opt.waves_per_group : 2;
   s->R.tun.order = 1;
@@ -1225,6 +1240,7 @@ void akmc_server_destroy(akmc_server* s) {
   (void)hipDeviceSynchronize();
   for (auto& kv : s->jobs)
     if (kv.second->rep.mem) (void)hipFree(kv.second->rep.mem);
+  if (s->init_stream) (void)hipStreamDestroy(s->init_stream);
   s->R.destroy();
   delete s;
 }

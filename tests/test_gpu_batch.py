@@ -42,6 +42,7 @@ RATE_CASES = [
     dict(n=12, va=60, T=650.0, alloy=[("Fe", 6000), ("Mn", 2000), ("Ni", 1500), ("Cu", 500)]),
     dict(box=(17, 13, 11), va=30, T=520.0),
     dict(n=32, va=700),   # several 64-vacancy blocks per sector
+    dict(n=48, va=40000),  # > 4096 vacancies per sector: the superblock level
 ]
 
 
@@ -116,6 +117,22 @@ def test_invariants_and_event_counts():
         assert r["nvac"] == int((s1 & 1).sum())
     assert st["events"] == 3000 * b.n
     b.close()
+
+
+def test_superblock_replica_invariants():
+    """A replica with more than 4096 vacancies per sector (three scan levels),
+    FP32 and FP64, both kinds of launch."""
+    say("test_superblock_replica_invariants")
+    cfg = make_cfg(n=48, va=40000, T=600.0, physics=1e3)
+    for prec in (akmc.AKMC_PREC_FP32, akmc.AKMC_PREC_FP64):
+        b = akmc.Batch([cfg, make_cfg(n=12, va=3)], precision=prec, init=akmc.AKMC_INIT_REFERENCE)
+        s0 = b.download(0)
+        st = b.run(3000)
+        assert st["failed"] == 0
+        s1 = b.download(0)
+        assert _counts(s1) == _counts(s0)
+        assert b.replica_stats(0)["events"] == 3000
+        b.close()
 
 
 def _final(cfgs, idx, events, runs=1, **opt):
