@@ -236,6 +236,7 @@ typedef struct akmc_replica_stats {
   int64_t err_n;            /* rates compared */
   int64_t err_tiny;         /* FP64 rates below FLT_MIN, excluded */
   int64_t n_skip;           /* time steps skipped: every occupied sector already past its threshold */
+  double busy_seconds;      /* device time of the work items that advanced this replica */
 } akmc_replica_stats;
 
 typedef struct akmc_batch_tuning {

@@ -284,7 +284,8 @@ class ReplicaStats(ctypes.Structure):
                 ("sector_pos", ctypes.c_int32), ("evolution", ctypes.c_double * 8), ("sector_time", ctypes.c_double),
                 ("nvac", ctypes.c_int64), ("n_eval", ctypes.c_int64), ("n_rebuild", ctypes.c_int64),
                 ("n_jump", ctypes.c_int64), ("err_max", ctypes.c_double), ("err_mean", ctypes.c_double),
-                ("err_n", ctypes.c_int64), ("err_tiny", ctypes.c_int64), ("n_skip", ctypes.c_int64)]
+                ("err_n", ctypes.c_int64), ("err_tiny", ctypes.c_int64), ("n_skip", ctypes.c_int64),
+                ("busy_seconds", ctypes.c_double)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}

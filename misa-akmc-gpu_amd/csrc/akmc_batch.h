@@ -103,6 +103,7 @@ struct ReplicaState {
   int64_t err_tiny;     // rates below FLT_MIN (FP64 value), not in the error
   int32_t sec_count[8]; // vacancies in each sector (kept by the kernel)
   int64_t n_skip;       // time steps skipped because no sector could have an event
+  int64_t busy_ticks;   // s_memrealtime ticks (100 MHz) spent on this replica's work items
 };
 
 struct BatchArgs {

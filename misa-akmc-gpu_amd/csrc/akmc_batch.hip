@@ -257,6 +257,107 @@ __device__ void eval_rep(const Sec<RateT>& S, RStage& G, const RTabs& T, const R
   }
 }
 
+// Stencil indices that lane (b = lane >> 3, p = lane & 7) reads in the
+// single-vacancy FP32 evaluation, for both parities: bytes w1[p], t1[b][p],
+// w2[min(p,4)], t2[b][min(p,4)] in idx[par], and w1[b] in xidx (byte par).
+struct LaneIdx {
+  uint32_t idx[2];
+  uint32_t xidx;
+};
+__device__ __forceinline__ LaneIdx lane_indices(const StencilTables& st) {
+  const int l = lane_id(), b = l >> 3, p = l & 7, q = min(p, 4);
+  LaneIdx L;
+  for (int par = 0; par < 2; par++)
+    L.idx[par] = (uint32_t)st.w1[par][p] | ((uint32_t)st.t1[par][b][p] << 8) | ((uint32_t)st.w2[par][q] << 16) |
+                 ((uint32_t)st.t2[par][b][q] << 24);
+  L.xidx = (uint32_t)st.w1[0][b] | ((uint32_t)st.w1[1][b] << 8);
+  return L;
+}
+
+__device__ __forceinline__ int bperm_i(int src_lane, int v) { return __builtin_amdgcn_ds_bpermute(src_lane << 2, v); }
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// FP32 evaluation of one vacancy (list index vi) whose stencil bytes are
+// already in registers: f = flag byte of stencil site `lane` (lanes < the
+// stencil size).  All 64 lanes work: lane (b, p) takes the p-th 1nn and 2nn
+// terms of the four bond counts of direction b, and the eight lanes of b sum
+// their parts.  The terms are integer multiples of q held exactly in FP32,
+// so the sum is the same in any order -- the same dE as the term-by-term
+// evaluation (rate_fp32q).  Writes the 8 rates, the total, the 1nn flag
+// bytes and the stale-tag side effect like eval_rep.
+template <typename RateT>
+__device__ void eval_one(const Sec<RateT>& S, RStage& G, const RTabs& T, const RView& R, const LaneIdx& LI, int vi,
+                         uint32_t f, const Patch& pt, bool own_from_sec, int flags, ErrAcc& E, bool& bad) {
+  const int l = lane_id(), b = l >> 3, p = l & 7;
+  const uint32_t w = S.key[vi];
+  const int par = (int)(key_x(w) & 1);
+  const int spl = readout((uint8_t)f);
+  const uint32_t P = LI.idx[par];
+  const int s_w1 = bperm_i((int)(P & 255u), spl);
+  const int s_t1 = bperm_i((int)((P >> 8) & 255u), spl);
+  const int s_w2 = bperm_i((int)((P >> 16) & 255u), spl);
+  const int s_t2 = bperm_i((int)(P >> 24), spl);
+  const int X = bperm_i((int)((LI.xidx >> (8 * par)) & 255u), spl);
+  const int Xc = X == SP_V ? SP_FE : X;  // a vacancy target has rate 0; any row keeps the lookups in range
+  // src_a - src_b + des_a - des_b, 1nn term k = p.
+  const int a_src = p == b ? (int)SP_V : s_w1;
+  const int a_des = p == 7 - b ? Xc : s_t1;
+  float d = (T.e1q[Xc * 5 + a_src] + T.e1q[SP_V * 5 + a_des]) - (T.e1q[SP_V * 5 + s_w1] + T.e1q[Xc * 5 + s_t1]);
+  // 2nn term k = p (five of them).
+  const float d2 = (T.e2q[Xc * 5 + s_w2] + T.e2q[SP_V * 5 + s_t2]) - (T.e2q[SP_V * 5 + s_w2] + T.e2q[Xc * 5 + s_t2]);
+  d += p < 5 ? d2 : 0.0f;
+  d += dpp_f<0xB1>(d);   // quad_perm [1,0,3,2]
+  d += dpp_f<0x4E>(d);   // quad_perm [2,3,0,1]
+  d += dpp_f<0x141>(d);  // row_half_mirror: the other quad of the eight lanes
+  const int m = (int)(T.e0q2[Xc] + d) - R.m_min;
+  const bool ok = (m >= 0) & ((m >> 8) < R.nhi);
+  const int mc = ok ? m : 0;
+  const float rf32 = G.tab[256 + (mc >> 8)] * G.tab[mc & 255];
+  const bool vac = X == SP_V;
+  bad = bad | (!vac & (!ok | !(rf32 <= 3.40282347e38f)));
+  const float rf = vac ? 0.0f : rf32;
+  double r = (double)rf;
+  if (flags & AKMC_BF_ERROR_STATS) {
+    if (l < T.st.n_sites) G.sten[0][l] = (uint8_t)spl;
+    wave_sync();
+    if (p == 0 && !vac) {
+      const double r64 = rate_fp64(G.sten[0], par, b, T.st, T.e1, T.e2, T.e0, R.nu, R.kT, T.exp_tab);
+      if (r64 >= 1.1754943508222875e-38) {
+        const double rel = fabs(r - r64) / r64;
+        E.emax = fmax(E.emax, rel);
+        E.esum += rel;
+        E.n += 1;
+      } else if (r64 > 0.0) {
+        E.tiny += 1;
+      }
+    }
+    wave_sync();
+  }
+  if (p == 0) S.rate[vi * 8 + b] = (RateT)rf;
+  if (l >= 1 && l <= 8) S.nb1[vi * 8 + (l - 1)] = (uint8_t)f;
+  double rr[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) rr[k] = bcast_d(r, 8 * k);
+  const double vt = ((rr[0] + rr[1]) + (rr[2] + rr[3])) + ((rr[4] + rr[5]) + (rr[6] + rr[7]));
+  const uint64_t sel = 0x0101010101010101ull;  // lane p == 0 of each direction
+  const bool cu = (ballot((p == 0) & (X == SP_CU)) & sel) != 0;
+  const bool mn = (ballot((p == 0) & (X == SP_MN)) & sel) != 0;
+  const bool ni = (ballot((p == 0) & (X == SP_NI)) & sel) != 0;
+  const uint32_t f_own = (uint32_t)__builtin_amdgcn_readlane((int)f, 0);  // stencil site 0: the vacancy
+  if (l == 0) {
+    const uint8_t clr = (uint8_t)((cu ? BIT_CU : 0) | (mn ? BIT_MN : 0) | (ni ? BIT_NI : 0));
+    const uint8_t f0 = vi == pt.moved ? pt.f1 : (own_from_sec ? S.own[vi] : (uint8_t)f_own);
+    const uint8_t nf = (uint8_t)(f0 & ~clr);
+    if (nf != f0) R.sites[key_index(R.d, w)] = nf;
+    S.own[vi] = nf;
+    S.vt[vi] = vt;
+  }
+  wave_sync();
+}
+
 // Block j's sum (and, at a rebuild, its z range) from the per-vacancy totals.
 template <typename RateT>
 __device__ void block_sum(const Sec<RateT>& S, int j, int nv, bool with_z) {
@@ -357,6 +458,7 @@ __device__ void run_item(const BatchArgs& a, const RTabs& T, RStage& G, uint8_t*
   ReplicaState st = a.state[r];
   const bool rates_only = (a.flags & AKMC_BF_RATES_ONLY) != 0;
   if (!rates_only && st.status != AKMC_REP_RUNNING) return;
+  const uint64_t t_item = (uint64_t)wall_clock64();
   const RView R{D.sites, D.vac_key, D.nvac, D.d, D.nu, D.kT, D.nhi, D.m_min};
   if (FP32)
     for (int i = l; i < 256 + D.nhi; i += 64) G.tab[i] = D.exp_tab[i];
@@ -378,9 +480,13 @@ __device__ void run_item(const BatchArgs& a, const RTabs& T, RStage& G, uint8_t*
   }
   if (LDS && st.in_sector && st.nv >= 0) copy_sector(S, Sg, st.nv);
   const uint64_t t_start = a.queue[1];
-  // Work budget of this item: events plus sector visits, so that a launch
-  // ends even when a replica performs no events.
+  // Work budget of this item: slice_events events, and at most 64 times as
+  // many sector visits, so that a launch ends even when a replica performs
+  // no events.
   int64_t budget = a.slice_events;
+  int64_t visits = 64 * a.slice_events;
+  const double inv_T = 1.0 / D.sector_T;
+  const LaneIdx LI = lane_indices(T.st);
   int zero_run = 0;  // consecutive sector visits with a zero total and no event
   int32_t status = AKMC_REP_RUNNING;
   const Patch nopatch{0xFFFFFFFFu, 0xFFFFFFFFu, 0, 0, -1};
@@ -399,7 +505,7 @@ __device__ void run_item(const BatchArgs& a, const RTabs& T, RStage& G, uint8_t*
         for (int q = 0; q < 8; q++) {
           if (G.cnt[q] == 0) continue;
           const double e = G.evo[q];
-          int64_t j = max(st.step, (int64_t)ceil(e / D.sector_T) - 1);
+          int64_t j = max(st.step, (int64_t)ceil(e * inv_T) - 1);
           while (!(static_cast<double>(j + 1) * D.sector_T - e > 0.0)) j++;
           while (j > st.step && static_cast<double>(j) * D.sector_T - e > 0.0) j--;
           K = min(K, j);
@@ -431,11 +537,11 @@ __device__ void run_item(const BatchArgs& a, const RTabs& T, RStage& G, uint8_t*
       if (st.nv < 0) zero_run = 0;
       st.in_sector = 0;
       if (++st.sect == 8) { st.sect = 0; st.step++; }
-      budget--;
+      visits--;
       continue;
     }
     if (st.events >= st.target) { status = AKMC_REP_TARGET_DONE; break; }
-    if (budget <= 0) break;
+    if (budget <= 0 || visits <= 0) break;
     if (st.nv < 0 && G.cnt[s] == 0) {
       // An empty sector: its total is zero without evaluating anything.
       st.nv = 0;
@@ -466,14 +572,14 @@ __device__ void run_item(const BatchArgs& a, const RTabs& T, RStage& G, uint8_t*
     const int nb = (nv + 63) >> 6, nsup = (nb + 63) >> 6;
     double sum, target, base = 0.0;
     int j0 = 0, nbk = nb;
-    double xt;
-    if (nsup > 1) {
-      xt = l < nsup ? S.ssum[l] : 0.0;
+    double xt = 0.0, inc_top = 0.0;
+    if (nb == 1) {
+      sum = S.bsum[0];  // one block: its sum is the total
     } else {
-      xt = l < nb ? S.bsum[l] : 0.0;
+      xt = nsup > 1 ? (l < nsup ? S.ssum[l] : 0.0) : (l < nb ? S.bsum[l] : 0.0);
+      inc_top = wave_incl_scan(xt);
+      sum = bcast_d(inc_top, 63);
     }
-    const double inc_top = wave_incl_scan(xt);
-    sum = bcast_d(inc_top, 63);
     // sum_rates == 0 or below epsilon: jump to the threshold (sublattice.inl:108-112).
     if (sum == 0.0 || fabs(sum) < 2.220446049250313e-16) {
       st.t = st.thr;
@@ -503,10 +609,12 @@ __device__ void run_item(const BatchArgs& a, const RTabs& T, RStage& G, uint8_t*
       j0 = 64 * max(k, 0);
       nbk = min(64, nb - j0);
       k = scan_pick(l < nbk ? S.bsum[j0 + l] : 0.0, nbk, target - base, base);
-    } else {
+    } else if (nb > 1) {
       k = first_set(ballot((l < nb) & (xt > 0.0) & (inc_top > target)));
       if (k == 64) k = last_set(ballot((l < nb) & (xt > 0.0)));
       if (k > 0) base += bcast_d(inc_top, k - 1);
+    } else {
+      k = 0;
     }
     const int blk = j0 + max(k, 0);
     const int i0 = 64 * blk;
@@ -514,20 +622,8 @@ __device__ void run_item(const BatchArgs& a, const RTabs& T, RStage& G, uint8_t*
     const int kv = scan_pick(l < nvk ? S.vt[i0 + l] : 0.0, nvk, target - base, base);
     if (kv < 0) { status = AKMC_REP_ERR_NO_EVENT; break; }
     const int v = i0 + kv;
-    int bsel = -1, blast = -1;
-    {
-      double r8[8];
-#pragma unroll
-      for (int b = 0; b < 8; b++) r8[b] = (double)S.rate[v * 8 + b];
-      double c = base;
-#pragma unroll
-      for (int b = 0; b < 8; b++) {
-        c += r8[b];
-        if (bsel < 0 && r8[b] > 0.0 && c > target) bsel = b;
-        if (r8[b] > 0.0) blast = b;
-      }
-    }
-    if (bsel < 0) bsel = blast;
+    // Direction: an inclusive scan of the vacancy's 8 rates in lanes 0..7.
+    const int bsel = scan_pick(l < 8 ? (double)S.rate[v * 8 + min(l, 7)] : 0.0, 8, target - base, base);
     if (bsel < 0) { status = AKMC_REP_ERR_NO_EVENT; break; }
     const double rr = (double)((xo_next(st.rng) >> 11) + 1) * 0x1p-53;  // (0, 1]
 
@@ -538,9 +634,11 @@ __device__ void run_item(const BatchArgs& a, const RTabs& T, RStage& G, uint8_t*
     nn1_offset(par, bsel, &ox, &oy, &oz);
     const uint32_t to = key_add(R.d, from, ox, oy, oz);
     uint8_t pf = 0;
+    uint32_t pf_key = 0xFFFFFFFFu;
     if (l < T.st.n_sites) {
       const int pp = par ^ 1;
-      pf = R.sites[key_index(R.d, key_add(R.d, to, T.st.off[pp][l][0], T.st.off[pp][l][1], T.st.off[pp][l][2]))];
+      pf_key = key_add(R.d, to, T.st.off[pp][l][0], T.st.off[pp][l][1], T.st.off[pp][l][2]);
+      pf = R.sites[key_index(R.d, pf_key)];
     }
     const uint8_t ff = S.own[v];
     const uint8_t gt = S.nb1[v * 8 + bsel];
@@ -592,7 +690,23 @@ __device__ void run_item(const BatchArgs& a, const RTabs& T, RStage& G, uint8_t*
         const bool live = (i < nv) & (S.slot[ic] >= 0);
         const bool hit = (i == v) ? stays : (live & (in_stencil(R.d, T.st, w, from) | in_stencil(R.d, T.st, w, to)));
         const uint64_t m = ballot(hit);
-        if (m) {
+        if (FP32 && __popcll(m) == 1) {
+          // One vacancy: the all-lane evaluation, from the prefetched stencil
+          // when it is the one that hopped.
+          const int ih = 64 * jb + first_set(m);
+          uint32_t f = pf, fk = pf_key;
+          if (ih != v) {
+            const uint32_t wh = S.key[ih];
+            const int ph = (int)(key_x(wh) & 1);
+            const int lk = min(l, T.st.n_sites - 1);
+            fk = key_add(R.d, wh, T.st.off[ph][lk][0], T.st.off[ph][lk][1], T.st.off[ph][lk][2]);
+            f = R.sites[key_index(R.d, fk)];
+          }
+          if (fk == from) f = nf_from;
+          if (fk == to) f = nf_to;
+          eval_one<RateT>(S, G, T, R, LI, ih, f, pt, true, a.flags, E, bad);
+          st.n_eval += 1;
+        } else if (m) {
           const int rk = __popcll(m & ((1ull << l) - 1));
           if (hit) G.list[rk] = i;
           wave_sync();
@@ -628,6 +742,7 @@ __device__ void run_item(const BatchArgs& a, const RTabs& T, RStage& G, uint8_t*
     st.err_tiny += (int64_t)et;
   }
   st.status = status;
+  st.busy_ticks += (int64_t)((uint64_t)wall_clock64() - t_item);
   for (int i = 0; i < 8; i++) {
     st.evo[i] = G.evo[i];
     st.sec_count[i] = G.cnt[i];

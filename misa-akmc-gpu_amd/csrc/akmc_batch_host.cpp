@@ -489,6 +489,7 @@ void fill_rep_stats(const HostRep& r, const akmc::ReplicaState& st, akmc_replica
   s->err_n = st.err_n;
   s->err_tiny = st.err_tiny;
   s->n_skip = st.n_skip;
+  s->busy_seconds = st.busy_ticks * 1e-8;  // 100 MHz
 }
 
 }  // namespace
@@ -740,7 +741,7 @@ int akmc_batch_tune(akmc_batch* b, int64_t trial_events, akmc_batch_tuning* chos
         t.vcap = vc;
         t.waves = w;
         t.order = ord;
-        t.slice = trial_events;
+        t.slice = std::min<int64_t>(keep.slice, trial_events);
         if (vc > 0 && (int64_t)w * akmc_batch_region_bytes(b->R.fp32, vc) > 160 * 1024) continue;
         b->R.tun = t;
         if (b->R.set_items(b->live, b->cap_of_slot)) continue;  // does not fit: skip
@@ -979,6 +980,7 @@ int admit(akmc_server* s, bool* changed) {
       s->free_slots.pop_back();
       s->bytes_live += j->rep.bytes;
       s->st.bytes_peak = std::max(s->st.bytes_peak, s->bytes_live);
+      s->st.live++;  // counted from here, so that the server never looks idle while it sets the job up
     }
     const bool fp32 = s->R.fp32;
     uint8_t* mem = nullptr;
@@ -1000,7 +1002,6 @@ int admit(akmc_server* s, bool* changed) {
       j->status = AKMC_JOB_RUNNING;
       j->t_admit = s->now();
       s->st.admitted++;
-      s->st.live++;
       s->st.bytes_live = s->bytes_live;
     }
     *changed = true;
