@@ -80,19 +80,27 @@ def batched_leg(args, plain_value):
                    slice_events=args.batch_slice)
     t_create = time.perf_counter() - t0
     print("bench: %d replicas created in %.1f s" % (len(cfgs), t_create), file=sys.stderr, flush=True)
-    tun = b.tune(trial_events=args.tune_events)
+    tun = b.tune(trial_seconds=args.tune_seconds)
     print("bench: tuning %s" % tun, file=sys.stderr, flush=True)
     b.run(args.batch_events)  # warmup
     akmc.sync(0)
-    ev, ksec, launches = 0, 0.0, 0
+    # Throughput: each step is one launch in which every replica runs until a
+    # deadline (no replica waits for another).
+    ev, ksec = 0, 0.0
     t0 = time.perf_counter()
     for i in range(args.steps):
-        st = b.run(args.batch_events)
+        st = b.run_for(args.batch_seconds)
         ev += st["events"]
         ksec += st["kernel_seconds"]
-        launches += st["launches"]
     akmc.sync(0)
     dt = time.perf_counter() - t0
+    launches = args.steps
+    # Time to advance every replica by the same number of events (bounded by
+    # the slowest replica).
+    t1 = time.perf_counter()
+    st_fixed = b.run(args.batch_events)
+    akmc.sync(0)
+    dt_fixed = time.perf_counter() - t1
     n_eval = sum(b.replica_stats(i)["n_eval"] for i in range(b.n))
     nvac = sum(b.replica_stats(i)["nvac"] for i in range(b.n))
     sites = sum(b.num_sites(i) for i in range(b.n))
@@ -104,7 +112,10 @@ def batched_leg(args, plain_value):
     sub.close()
     value = ev / dt
     return {"value": value, "unit": "events/s", "replicas": len(cfgs), "sites": sites, "vacancies": nvac,
-            "events_per_step": args.batch_events * len(cfgs), "events_timed": ev, "ms_per_step": dt / args.steps * 1e3,
+            "step": "one launch, every replica runs until a %.0f ms deadline" % (args.batch_seconds * 1e3),
+            "events_timed": ev, "ms_per_step": dt / args.steps * 1e3,
+            "fixed_events": {"events_per_replica": args.batch_events, "seconds": dt_fixed,
+                             "events_per_s": st_fixed["events"] / dt_fixed},
             "kernel_seconds": ksec, "launches": launches, "dtype": "f32 rates, f64 sums/search/dt",
             "tuning": tun, "vs_plain": value / plain_value if plain_value else None,
             "evaluations_per_event": n_eval / max(1, sum(1 for _ in cfgs)) / max(1, ev) * len(cfgs),
@@ -121,8 +132,21 @@ def served_leg(args, batched_value):
     from akmc import workloads
     jobs = workloads.jobs(args.jobs, ev_lo=1e5 * args.serve_scale, ev_hi=1e7 * args.serve_scale)
     mean_events = sum(t for _, t in jobs) / len(jobs)
-    # Offered rate: about 70% of the replica engine's throughput, in jobs/s.
-    rate = args.serve_rate or max(1.0, 0.7 * batched_value / mean_events)
+    capacity = None
+    if not args.serve_rate:
+        # Capacity: a separate stream of jobs of the same law, all submitted at
+        # once; the offered rate is then 70% of the jobs/s it sustains.
+        probe = workloads.jobs(max(200, args.jobs // 5), base=0x5EED0003, ev_lo=1e5 * args.serve_scale,
+                               ev_hi=1e7 * args.serve_scale)
+        srv = akmc.Server(device=0, hbm_budget=int(250e9))
+        t0 = srv.clock()
+        for cfg, target in probe:
+            srv.submit(cfg, target)
+        srv.wait_idle(0)
+        capacity = len(probe) / (srv.clock() - t0)
+        srv.close()
+        print("bench: serving capacity %.0f jobs/s" % capacity, file=sys.stderr, flush=True)
+    rate = args.serve_rate or 0.7 * capacity
     srv = akmc.Server(device=0, hbm_budget=int(250e9))
     rng = random.Random(12345)
     t_start = srv.clock()
@@ -145,6 +169,7 @@ def served_leg(args, batched_value):
     span = max(i["t_done"] for i in infos) - t_start
     ev = sum(i["events"] for i in infos)
     return {"value": ev / span, "unit": "events/s", "jobs": len(ids), "offered_jobs_per_s": rate,
+            "capacity_jobs_per_s": capacity,
             "mean_target_events": mean_events, "target_scale": args.serve_scale,
             "p50_latency_s": q(0.50), "p99_latency_s": q(0.99), "max_latency_s": lat[-1],
             "makespan_s": span, "gpu_busy_frac": st["busy_seconds"] / span, "launches": st["launches"],
@@ -167,9 +192,10 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--replicas", type=int, default=1024)
-    ap.add_argument("--batch-events", type=int, default=20000, help="events per replica per step")
+    ap.add_argument("--batch-events", type=int, default=20000, help="events per replica of the fixed-event run")
+    ap.add_argument("--batch-seconds", type=float, default=0.1, help="deadline of one batched step")
     ap.add_argument("--batch-slice", type=int, default=4096)
-    ap.add_argument("--tune-events", type=int, default=256)
+    ap.add_argument("--tune-seconds", type=float, default=0.01)
     ap.add_argument("--jobs", type=int, default=4000)
     ap.add_argument("--serve-scale", type=float, default=0.01,
                     help="scale of the job targets (1 = the full 1e5..1e7 events)")

@@ -264,10 +264,17 @@ int akmc_batch_create(const akmc_params* replicas, int32_t n, const akmc_batch_o
  * when its time loop ends). */
 int akmc_batch_run(akmc_batch* b, int64_t events_per_replica, akmc_batch_stats* stats);
 
-/* Searches the tuning space with short slices of real work (the replicas
- * advance; their trajectories do not depend on the tuning) and keeps the
- * fastest setting.  trial_events: events per replica per trial. */
-int akmc_batch_tune(akmc_batch* b, int64_t trial_events, akmc_batch_tuning* chosen);
+/* One launch in which every running replica advances until a deadline of
+ * `seconds` after the launch starts (at an event boundary; the events per
+ * replica then differ).  Trajectories are those of akmc_batch_run. */
+int akmc_batch_run_for(akmc_batch* b, double seconds, akmc_batch_stats* stats);
+
+/* Searches the tuning space (LDS capacity, replicas per workgroup, work
+ * order) with one deadline launch of real work per setting (the replicas
+ * advance; their trajectories do not depend on the tuning), keeps the
+ * setting with the most events per device second, then brings every running
+ * replica to the same event count. */
+int akmc_batch_tune(akmc_batch* b, double trial_seconds, akmc_batch_tuning* chosen);
 int akmc_batch_get_tuning(akmc_batch* b, akmc_batch_tuning* t);
 
 int32_t akmc_batch_size(const akmc_batch* b);

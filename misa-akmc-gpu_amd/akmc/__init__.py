@@ -333,7 +333,8 @@ def _bind_batch(L):
     L.akmc_batch_options_default.argtypes = [P(BatchOptions)]
     L.akmc_batch_create.argtypes = [vp, i32, P(BatchOptions), i32, P(vp)]
     L.akmc_batch_run.argtypes = [vp, i64, P(BatchStats)]
-    L.akmc_batch_tune.argtypes = [vp, i64, P(BatchTuning)]
+    L.akmc_batch_run_for.argtypes = [vp, dbl, P(BatchStats)]
+    L.akmc_batch_tune.argtypes = [vp, dbl, P(BatchTuning)]
     L.akmc_batch_get_tuning.argtypes = [vp, P(BatchTuning)]
     L.akmc_batch_size.argtypes = [vp]
     L.akmc_batch_num_sites.argtypes = [vp, i32]
@@ -418,9 +419,14 @@ class Batch:
         _check(_blib().akmc_batch_run(self.h, int(events_per_replica), ctypes.byref(st)))
         return st.as_dict()
 
-    def tune(self, trial_events=256):
+    def run_for(self, seconds):
+        st = BatchStats()
+        _check(_blib().akmc_batch_run_for(self.h, float(seconds), ctypes.byref(st)))
+        return st.as_dict()
+
+    def tune(self, trial_seconds=0.01):
         t = BatchTuning()
-        _check(_blib().akmc_batch_tune(self.h, int(trial_events), ctypes.byref(t)))
+        _check(_blib().akmc_batch_tune(self.h, float(trial_seconds), ctypes.byref(t)))
         return t.as_dict()
 
     def tuning(self):

@@ -565,7 +565,7 @@ __device__ void run_item(const BatchArgs& a, const RTabs& T, RStage& G, uint8_t*
       st.n_eval += st.nv;
       if (bad) { status = AKMC_REP_ERR_RANGE; break; }
     }
-    if (a.budget_ticks && (uint64_t)wall_clock64() - t_start > a.budget_ticks) break;
+    if (a.budget_ticks && (st.events & 7) == 0 && (uint64_t)wall_clock64() - t_start > a.budget_ticks) break;
 
     // ---- total and selection (three scan levels).
     const int nv = st.nv;

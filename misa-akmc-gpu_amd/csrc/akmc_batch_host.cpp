@@ -112,12 +112,24 @@ struct HostRep {
   int32_t nhi = 0, m_min = 0;
   int32_t sec_count[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // vacancies per sector
   std::vector<float> tab;
-  int64_t bytes = 0;
+  int64_t bytes = 0;  // device bytes of the replica's block
   uint8_t* mem = nullptr;  // base of this replica's device block
   uint8_t* sites = nullptr;
   uint32_t* vac = nullptr;
   uint8_t* ws = nullptr;
   float* dtab = nullptr;
+  int64_t* didx = nullptr;   // vacancy site draws (set-up only)
+  uint32_t* dcum = nullptr;  // composition table (set-up only)
+  uint8_t* dflag = nullptr;
+  // Host data of an asynchronous set-up, alive until its stream is synchronized.
+  std::vector<int64_t> h_idx;
+  std::vector<uint32_t> h_keys;
+  uint32_t h_cum[AKMC_MAX_ALLOY] = {};
+  uint8_t h_flag[AKMC_MAX_ALLOY] = {};
+  void release_setup() {
+    std::vector<int64_t>().swap(h_idx);
+    std::vector<uint32_t>().swap(h_keys);
+  }
 };
 
 int setup_rep(HostRep* r, const akmc_params& p, bool fp32) {
@@ -146,7 +158,8 @@ int setup_rep(HostRep* r, const akmc_params& p, bool fp32) {
     if (rc) return rc;
   }
   r->bytes = a256(r->nsites) + a256(4 * (int64_t)r->cap) + a256(akmc::ws_layout(r->cap, fp32 ? 4 : 8).total) +
-             a256(4 * (int64_t)(256 + r->nhi));
+             a256(4 * (int64_t)(256 + r->nhi)) + a256(8 * (int64_t)r->cap) + a256(4 * AKMC_MAX_ALLOY) +
+             a256(AKMC_MAX_ALLOY);
   return AKMC_OK;
 }
 
@@ -159,6 +172,12 @@ void carve(HostRep* r, uint8_t* base, bool fp32) {
   r->ws = base;
   base += a256(akmc::ws_layout(r->cap, fp32 ? 4 : 8).total);
   r->dtab = reinterpret_cast<float*>(base);
+  base += a256(4 * (int64_t)(256 + r->nhi));
+  r->didx = reinterpret_cast<int64_t*>(base);
+  base += a256(8 * (int64_t)r->cap);
+  r->dcum = reinterpret_cast<uint32_t*>(base);
+  base += a256(4 * AKMC_MAX_ALLOY);
+  r->dflag = base;
 }
 
 akmc::ReplicaDesc make_desc(const HostRep& r) {
@@ -212,23 +231,11 @@ std::vector<uint32_t> vacancy_keys(const akmc::BoxDims& d, const uint8_t* sites,
   return keys;
 }
 
-// Small device buffers for replica initialisation.
-struct InitScratch {
-  int64_t* idx = nullptr;
-  int64_t idx_cap = 0;
-  uint32_t* cum = nullptr;
-  uint8_t* flag = nullptr;
-  ~InitScratch() {
-    if (idx) (void)hipFree(idx);
-    if (cum) (void)hipFree(cum);
-    if (flag) (void)hipFree(flag);
-  }
-};
-
-// Device initial occupancy (AKMC_INIT_DEVICE), then the vacancy list.
-int init_device(HostRep* r, InitScratch* S, hipStream_t st) {
-  uint32_t cum[AKMC_MAX_ALLOY];
-  uint8_t flag[AKMC_MAX_ALLOY];
+// Device initial occupancy (AKMC_INIT_DEVICE), then the vacancy list,
+// enqueued on `st` without waiting: the host data it copies stays in the
+// replica (h_idx, h_keys, h_cum, h_flag) until the caller synchronizes `st`
+// and calls release_setup().  Also uploads the FP32 rate table.
+int init_device_async(HostRep* r, hipStream_t st) {
   int nt = 0;
   uint64_t total = 0;
   for (int i = 0; i < r->p.n_alloy; i++) {
@@ -246,43 +253,36 @@ int init_device(HostRep* r, InitScratch* S, hipStream_t st) {
     }
     total += (uint64_t)q;
     if (total > 0xFFFFFFFFull) return fail(AKMC_ERR_INVALID, "alloy ratios too large");
-    cum[nt] = (uint32_t)total;
-    flag[nt] = f;
+    r->h_cum[nt] = (uint32_t)total;
+    r->h_flag[nt] = f;
     nt++;
   }
   if (nt == 0) return fail(AKMC_ERR_INVALID, "empty alloy");
-  if (!S->cum) {
-    HIP_TRY(hipMalloc(&S->cum, sizeof(cum)));
-    HIP_TRY(hipMalloc(&S->flag, sizeof(flag)));
-  }
-  HIP_TRY(hipMemcpyAsync(S->cum, cum, sizeof(uint32_t) * nt, hipMemcpyHostToDevice, st));
-  HIP_TRY(hipMemcpyAsync(S->flag, flag, nt, hipMemcpyHostToDevice, st));
-  if (akmc_batch_init_sites_launch(r->sites, r->nsites, (uint64_t)r->p.seed_create_types, S->cum, S->flag, nt, st))
+  HIP_TRY(hipMemcpyAsync(r->dcum, r->h_cum, sizeof(uint32_t) * nt, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(r->dflag, r->h_flag, nt, hipMemcpyHostToDevice, st));
+  if (akmc_batch_init_sites_launch(r->sites, r->nsites, (uint64_t)r->p.seed_create_types, r->dcum, r->dflag, nt, st))
     return fail(AKMC_ERR_HIP, "init_sites launch failed");
   // Vacancy sites: va_count uniform draws (repeats collapse, as in the
-  // reference's hash insertions).
-  std::vector<int64_t> idx((size_t)r->p.va_count);
+  // reference's hash insertions), in site order.
+  std::vector<int64_t>& idx = r->h_idx;
+  idx.resize((size_t)r->p.va_count);
   uint64_t x = (uint64_t)r->p.seed_create_vacancy ^ 0x5A5A5A5A5A5A5A5Aull;
   for (auto& v : idx) v = (int64_t)(((unsigned __int128)splitmix_next(x) * (uint64_t)r->nsites) >> 64);
   std::sort(idx.begin(), idx.end());
   idx.erase(std::unique(idx.begin(), idx.end()), idx.end());
-  if ((int64_t)idx.size() > S->idx_cap) {
-    if (S->idx) HIP_TRY(hipFree(S->idx));
-    S->idx_cap = std::max<int64_t>((int64_t)idx.size(), 4096);
-    HIP_TRY(hipMalloc(&S->idx, S->idx_cap * sizeof(int64_t)));
-  }
-  std::vector<uint32_t> keys(idx.size());
+  std::vector<uint32_t>& keys = r->h_keys;
+  keys.resize(idx.size());
   for (size_t i = 0; i < idx.size(); i++) keys[i] = key_of(r->d, idx[i]);
   if (!idx.empty()) {
-    HIP_TRY(hipMemcpyAsync(S->idx, idx.data(), idx.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
-    if (akmc_batch_place_vacancies_launch(r->sites, S->idx, (int64_t)idx.size(), st))
+    HIP_TRY(hipMemcpyAsync(r->didx, idx.data(), idx.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    if (akmc_batch_place_vacancies_launch(r->sites, r->didx, (int64_t)idx.size(), st))
       return fail(AKMC_ERR_HIP, "place_vacancies launch failed");
     HIP_TRY(hipMemcpyAsync(r->vac, keys.data(), keys.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
   }
+  if (!r->tab.empty())
+    HIP_TRY(hipMemcpyAsync(r->dtab, r->tab.data(), r->tab.size() * sizeof(float), hipMemcpyHostToDevice, st));
   r->nvac = (int32_t)keys.size();
   count_sectors(r, keys);
-  // The copies above read pageable host vectors that end with this scope.
-  HIP_TRY(hipStreamSynchronize(st));
   return AKMC_OK;
 }
 
@@ -502,7 +502,6 @@ struct akmc_batch {
   std::vector<int32_t> cap_of_slot;
   std::vector<int32_t> live;
   uint8_t* pool = nullptr;
-  InitScratch scratch;
   std::vector<akmc::ReplicaState> h_state;
   akmc_batch_tuning last_tune{};
 };
@@ -650,9 +649,11 @@ int akmc_batch_create(const akmc_params* reps, int32_t n, const akmc_batch_optio
     }
   } else if (opt.init == AKMC_INIT_DEVICE) {
     for (int i = 0; i < n; i++) {
-      rc = init_device(&b->reps[i], &b->scratch, b->R.s0);
+      rc = init_device_async(&b->reps[i], b->R.s0);
       if (rc) return fail(rc, "replica %d: %s", i, akmc_host::g_err.c_str());
     }
+    HIP_TRY(hipStreamSynchronize(b->R.s0));
+    for (auto& r : b->reps) r.release_setup();
   } else {
     return fail(AKMC_ERR_INVALID, "bad init mode");
   }
@@ -708,6 +709,43 @@ int akmc_batch_run(akmc_batch* b, int64_t events, akmc_batch_stats* stats) {
   return AKMC_OK;
 }
 
+int akmc_batch_run_for(akmc_batch* b, double seconds, akmc_batch_stats* stats) {
+  if (!b || !(seconds > 0)) return fail(AKMC_ERR_INVALID, "bad arguments");
+  HIP_TRY(hipSetDevice(b->R.device));
+  const auto t0 = std::chrono::steady_clock::now();
+  int rc = read_states(b);
+  if (rc) return rc;
+  for (auto& s : b->h_state)
+    if (s.status == AKMC_REP_RUNNING) s.target = INT64_MAX;
+  rc = write_states(b);
+  if (rc) return rc;
+  // One direct launch with a deadline instead of an event budget.
+  const Tuning keep = b->R.tun;
+  const bool graph = b->R.use_graph;
+  b->R.tun.slice = (int64_t)1 << 40;
+  b->R.budget_ticks = (uint64_t)(seconds * 100e6);  // s_memrealtime runs at 100 MHz
+  b->R.use_graph = false;
+  int64_t ev = 0;
+  double ks = 0;
+  rc = run_slices(b, 1, &ev, &ks);
+  b->R.tun = keep;
+  b->R.budget_ticks = 0;
+  b->R.use_graph = graph;
+  if (rc) return rc;
+  if (stats) {
+    memset(stats, 0, sizeof(*stats));
+    stats->events = ev;
+    stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    stats->kernel_seconds = ks;
+    stats->launches = 1;
+    for (const auto& s : b->h_state) {
+      if (s.status == AKMC_REP_RUNNING) stats->running++;
+      if (s.status < 0) stats->failed++;
+    }
+  }
+  return AKMC_OK;
+}
+
 int akmc_batch_get_tuning(akmc_batch* b, akmc_batch_tuning* t) {
   if (!b || !t) return fail(AKMC_ERR_INVALID, "bad arguments");
   memset(t, 0, sizeof(*t));
@@ -720,8 +758,8 @@ int akmc_batch_get_tuning(akmc_batch* b, akmc_batch_tuning* t) {
   return AKMC_OK;
 }
 
-int akmc_batch_tune(akmc_batch* b, int64_t trial_events, akmc_batch_tuning* chosen) {
-  if (!b || trial_events <= 0) return fail(AKMC_ERR_INVALID, "bad arguments");
+int akmc_batch_tune(akmc_batch* b, double trial_seconds, akmc_batch_tuning* chosen) {
+  if (!b || !(trial_seconds > 0)) return fail(AKMC_ERR_INVALID, "bad arguments");
   HIP_TRY(hipSetDevice(b->R.device));
   const Tuning keep = b->R.tun;
   int32_t maxcap = 1;
@@ -741,24 +779,46 @@ int akmc_batch_tune(akmc_batch* b, int64_t trial_events, akmc_batch_tuning* chos
         t.vcap = vc;
         t.waves = w;
         t.order = ord;
-        t.slice = std::min<int64_t>(keep.slice, trial_events);
         if (vc > 0 && (int64_t)w * akmc_batch_region_bytes(b->R.fp32, vc) > 160 * 1024) continue;
         b->R.tun = t;
         if (b->R.set_items(b->live, b->cap_of_slot)) continue;  // does not fit: skip
-        int64_t ev = 0, nl = 0;
-        double ks = 0;
-        const int rc = advance(b, trial_events, &ev, &ks, &nl);
+        akmc_batch_stats st;
+        const int rc = akmc_batch_run_for(b, trial_seconds, &st);
         if (rc) return rc;
         trials++;
-        const double rate = ks > 0 ? ev / ks : 0;
+        const double rate = st.kernel_seconds > 0 ? st.events / st.kernel_seconds : 0;
         if (rate > best_rate) {
           best_rate = rate;
           best = t;
         }
       }
-  best.slice = keep.slice;
   b->R.tun = best;
-  const int rc = b->R.set_items(b->live, b->cap_of_slot);
+  int rc = b->R.set_items(b->live, b->cap_of_slot);
+  if (rc) return rc;
+  // The deadline trials leave the replicas at different event counts: bring
+  // every running replica up to the largest count.
+  rc = read_states(b);
+  if (rc) return rc;
+  int64_t top = 0;
+  for (const auto& s : b->h_state)
+    if (s.status == AKMC_REP_RUNNING) top = std::max(top, s.events);
+  for (auto& s : b->h_state)
+    if (s.status == AKMC_REP_RUNNING) s.target = top;
+  rc = write_states(b);
+  if (rc) return rc;
+  for (;;) {
+    bool need = false;
+    for (const auto& s : b->h_state)
+      if (s.status == AKMC_REP_RUNNING && s.events < s.target) need = true;
+    if (!need) break;
+    int64_t ev = 0;
+    double ks = 0;
+    rc = run_slices(b, 1, &ev, &ks);
+    if (rc) return rc;
+  }
+  for (auto& s : b->h_state)
+    if (s.status == AKMC_REP_TARGET_DONE) s.status = AKMC_REP_RUNNING;
+  rc = write_states(b);
   if (rc) return rc;
   b->last_tune.events_per_second = best_rate;
   b->last_tune.trials = trials;
@@ -922,6 +982,65 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
+// Device memory for jobs: large chunks from hipMalloc, carved first-fit in
+// 256-byte granules and coalesced on release, so that admitting or retiring
+// a job calls neither hipMalloc nor hipFree once the chunks exist.
+struct Arena {
+  struct Chunk {
+    uint8_t* base = nullptr;
+    int64_t size = 0;
+    std::map<int64_t, int64_t> free;  // offset -> bytes
+  };
+  std::vector<Chunk> chunks;
+  int64_t chunk_bytes = (int64_t)4 << 30;
+
+  int alloc(int64_t n, uint8_t** out) {
+    n = a256(n);
+    for (auto& c : chunks)
+      for (auto it = c.free.begin(); it != c.free.end(); ++it)
+        if (it->second >= n) {
+          const int64_t off = it->first, sz = it->second;
+          c.free.erase(it);
+          if (sz > n) c.free[off + n] = sz - n;
+          *out = c.base + off;
+          return AKMC_OK;
+        }
+    Chunk c;
+    c.size = std::max(chunk_bytes, n);
+    HIP_TRY(hipMalloc(&c.base, (size_t)c.size));
+    if (c.size > n) c.free[n] = c.size - n;
+    *out = c.base;
+    chunks.push_back(std::move(c));
+    return AKMC_OK;
+  }
+
+  void release(uint8_t* p, int64_t n) {
+    n = a256(n);
+    for (auto& c : chunks) {
+      if (p < c.base || p >= c.base + c.size) continue;
+      auto it = c.free.emplace(p - c.base, n).first;
+      auto nx = std::next(it);
+      if (nx != c.free.end() && it->first + it->second == nx->first) {
+        it->second += nx->second;
+        c.free.erase(nx);
+      }
+      if (it != c.free.begin()) {
+        auto pv = std::prev(it);
+        if (pv->first + pv->second == it->first) {
+          pv->second += it->second;
+          c.free.erase(it);
+        }
+      }
+      return;
+    }
+  }
+
+  void destroy() {
+    for (auto& c : chunks) (void)hipFree(c.base);
+    chunks.clear();
+  }
+};
+
 struct Job {
   int64_t id = 0;
   akmc_params p{};
@@ -933,6 +1052,8 @@ struct Job {
   int64_t events = 0;
   double t_submit = 0, t_admit = 0, t_done = 0;
   std::vector<uint8_t> result;
+  akmc::ReplicaDesc h_desc{};   // sources of the admission copies
+  akmc::ReplicaState h_state{};
 };
 
 }  // namespace
@@ -949,7 +1070,7 @@ struct akmc_server {
   std::vector<int32_t> free_slots;
   std::vector<int32_t> cap_of_slot;
   std::vector<akmc::ReplicaState> h_state;
-  InitScratch scratch;
+  Arena arena;
   int64_t next_id = 1;
   int64_t bytes_live = 0;
   akmc_server_stats st{};
@@ -965,15 +1086,18 @@ struct akmc_server {
 
 namespace {
 
-// Admits pending jobs in arrival order while the budget and slots allow.
+// Admits pending jobs in arrival order while the budget and slots allow (at
+// most 256 per step).  Their set-up is enqueued on the set-up stream and
+// waited for once.
 int admit(akmc_server* s, bool* changed) {
-  for (;;) {
+  std::vector<Job*> batch;
+  while (batch.size() < 256) {
     Job* j = nullptr;
     {
       std::lock_guard<std::mutex> lk(s->mu);
-      if (s->pending.empty() || s->free_slots.empty()) return AKMC_OK;
+      if (s->pending.empty() || s->free_slots.empty()) break;
       Job* h = s->pending.front();
-      if (s->bytes_live + h->rep.bytes > s->opt.hbm_budget) return AKMC_OK;
+      if (s->bytes_live + h->rep.bytes > s->opt.hbm_budget) break;
       s->pending.pop_front();
       j = h;
       j->slot = s->free_slots.back();
@@ -982,30 +1106,36 @@ int admit(akmc_server* s, bool* changed) {
       s->st.bytes_peak = std::max(s->st.bytes_peak, s->bytes_live);
       s->st.live++;  // counted from here, so that the server never looks idle while it sets the job up
     }
-    const bool fp32 = s->R.fp32;
     uint8_t* mem = nullptr;
-    HIP_TRY(hipMalloc(&mem, (size_t)j->rep.bytes));
-    carve(&j->rep, mem, fp32);
-    int rc = init_device(&j->rep, &s->scratch, s->init_stream);
+    int rc = s->arena.alloc(j->rep.bytes, &mem);
     if (rc) return rc;
-    if (fp32)
-      HIP_TRY(hipMemcpy(j->rep.dtab, j->rep.tab.data(), j->rep.tab.size() * sizeof(float), hipMemcpyHostToDevice));
-    const akmc::ReplicaDesc D = make_desc(j->rep);
-    akmc::ReplicaState st = initial_state(j->rep);
-    st.target = j->target;
-    HIP_TRY(hipMemcpy(s->R.d_desc + j->slot, &D, sizeof(D), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(s->R.d_state + j->slot, &st, sizeof(st), hipMemcpyHostToDevice));
-    s->cap_of_slot[j->slot] = j->rep.cap;
-    {
-      std::lock_guard<std::mutex> lk(s->mu);
-      s->slot_job[j->slot] = j;
-      j->status = AKMC_JOB_RUNNING;
-      j->t_admit = s->now();
-      s->st.admitted++;
-      s->st.bytes_live = s->bytes_live;
-    }
-    *changed = true;
+    carve(&j->rep, mem, s->R.fp32);
+    rc = init_device_async(&j->rep, s->init_stream);
+    if (rc) return rc;
+    j->h_desc = make_desc(j->rep);
+    j->h_state = initial_state(j->rep);
+    j->h_state.target = j->target;
+    HIP_TRY(hipMemcpyAsync(s->R.d_desc + j->slot, &j->h_desc, sizeof(j->h_desc), hipMemcpyHostToDevice,
+                           s->init_stream));
+    HIP_TRY(hipMemcpyAsync(s->R.d_state + j->slot, &j->h_state, sizeof(j->h_state), hipMemcpyHostToDevice,
+                           s->init_stream));
+    batch.push_back(j);
   }
+  if (batch.empty()) return AKMC_OK;
+  HIP_TRY(hipStreamSynchronize(s->init_stream));
+  const double t = s->now();
+  std::lock_guard<std::mutex> lk(s->mu);
+  for (Job* j : batch) {
+    j->rep.release_setup();
+    s->cap_of_slot[j->slot] = j->rep.cap;
+    s->slot_job[j->slot] = j;
+    j->status = AKMC_JOB_RUNNING;
+    j->t_admit = t;
+    s->st.admitted++;
+  }
+  s->st.bytes_live = s->bytes_live;
+  *changed = true;
+  return AKMC_OK;
 }
 
 int retire(akmc_server* s, Job* j, int32_t rep_status) {
@@ -1013,7 +1143,7 @@ int retire(akmc_server* s, Job* j, int32_t rep_status) {
     j->result.resize(j->rep.nsites);
     HIP_TRY(hipMemcpy(j->result.data(), j->rep.sites, j->rep.nsites, hipMemcpyDeviceToHost));
   }
-  HIP_TRY(hipFree(j->rep.mem));
+  s->arena.release(j->rep.mem, j->rep.bytes);
   j->rep.mem = nullptr;
   std::lock_guard<std::mutex> lk(s->mu);
   j->rep_status = rep_status;
@@ -1239,8 +1369,7 @@ void akmc_server_destroy(akmc_server* s) {
   if (s->worker.joinable()) s->worker.join();
   (void)hipSetDevice(s->R.device);
   (void)hipDeviceSynchronize();
-  for (auto& kv : s->jobs)
-    if (kv.second->rep.mem) (void)hipFree(kv.second->rep.mem);
+  s->arena.destroy();
   if (s->init_stream) (void)hipStreamDestroy(s->init_stream);
   s->R.destroy();
   delete s;
