@@ -33,6 +33,12 @@ sys.path.insert(0, os.path.join(ROOT, "misa-akmc-gpu_amd"))
 
 METRIC = "vacancy-hop events/s, plain FP64 vs batched FP32-rate; p50/p99 job latency"
 HBM_PEAK_GBS = 8000.0
+# HBM traffic measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on the
+# same workloads (FETCH_SIZE scaled by 2.0: it reads 524,306 KiB for the
+# 1 GiB read of the library's copy kernel; WRITE_SIZE reads it exactly):
+# profiles/plain_fp64_256cube_hbm_counters.md, profiles/replica_fp32_1024_hbm_counters.md.
+PLAIN_TRAFFIC_BYTES_PER_EVENT = 75.0
+REPLICA_TRAFFIC_BYTES_PER_EVENT = 249.2
 
 WORKLOAD = dict(box=(256, 256, 256), alloy=[("Fe", 9666), ("Cu", 134), ("Mn", 130), ("Ni", 70)], va_count=6711,
                 seeds=dict(create_types=100565, create_vacancy=16650, event_selection=563264, time_inc=594689),
@@ -76,14 +82,19 @@ def batched_leg(args, plain_value):
     from akmc import workloads
     cfgs = workloads.replicas(args.replicas)
     t0 = time.perf_counter()
+    tuning = {}
+    if args.batch_vcap:
+        tuning = dict(lds_vcap=args.batch_vcap, waves_per_group=args.batch_waves, order=args.batch_order)
     b = akmc.Batch(cfgs, device=0, precision=akmc.AKMC_PREC_FP32, init=akmc.AKMC_INIT_DEVICE,
-                   slice_events=args.batch_slice)
+                   slice_events=args.batch_slice, **tuning)
     t_create = time.perf_counter() - t0
     print("bench: %d replicas created in %.1f s" % (len(cfgs), t_create), file=sys.stderr, flush=True)
-    tun = b.tune(trial_seconds=args.tune_seconds)
+    tun = b.tune(trial_seconds=args.tune_seconds) if not (args.no_tune or tuning) else b.tuning()
     print("bench: tuning %s" % tun, file=sys.stderr, flush=True)
-    b.run(args.batch_events)  # warmup
+    for _ in range(args.batch_warmup):
+        b.run(args.batch_events)
     akmc.sync(0)
+    before = [b.replica_stats(i) for i in range(b.n)]
     # Throughput: each step is one launch in which every replica runs until a
     # deadline (no replica waits for another).
     ev, ksec = 0, 0.0
@@ -95,34 +106,50 @@ def batched_leg(args, plain_value):
     akmc.sync(0)
     dt = time.perf_counter() - t0
     launches = args.steps
-    # Time to advance every replica by the same number of events (bounded by
-    # the slowest replica).
-    t1 = time.perf_counter()
-    st_fixed = b.run(args.batch_events)
-    akmc.sync(0)
-    dt_fixed = time.perf_counter() - t1
-    n_eval = sum(b.replica_stats(i)["n_eval"] for i in range(b.n))
+    after = [b.replica_stats(i) for i in range(b.n)]
+    d_eval = sum(a["n_eval"] - x["n_eval"] for a, x in zip(after, before))
+    # Algorithmic HBM bytes: 55 stencil bytes per evaluation, the 2-byte swap
+    # and the 4-byte vacancy key per event (the sector state stays in LDS).
+    nbytes = d_eval * 55 + ev * 6
+    achieved = nbytes / max(1, launches) / (ksec / max(1, launches)) / 1e9
+    fixed = None
+    if not args.no_fixed:
+        # Time to advance every replica by the same number of events (bounded
+        # by the slowest replica).
+        t1 = time.perf_counter()
+        st_fixed = b.run(args.batch_events)
+        akmc.sync(0)
+        dt_fixed = time.perf_counter() - t1
+        fixed = {"events_per_replica": args.batch_events, "seconds": dt_fixed,
+                 "events_per_s": st_fixed["events"] / dt_fixed}
     nvac = sum(b.replica_stats(i)["nvac"] for i in range(b.n))
     sites = sum(b.num_sites(i) for i in range(b.n))
     b.close()
-    # FP32 rate error against FP64 on a subset (every rate of every evaluation).
-    sub = akmc.Batch(cfgs[:64], device=0, precision=akmc.AKMC_PREC_FP32, init=akmc.AKMC_INIT_DEVICE, error_stats=1)
-    sub.run(2000)
-    errs = [sub.replica_stats(i) for i in range(sub.n)]
-    sub.close()
+    rate_error = None
+    if not args.no_error_stats:
+        # FP32 rate error against FP64 on a subset (every rate of every evaluation).
+        sub = akmc.Batch(cfgs[:64], device=0, precision=akmc.AKMC_PREC_FP32, init=akmc.AKMC_INIT_DEVICE,
+                         error_stats=1)
+        sub.run(2000)
+        errs = [sub.replica_stats(i) for i in range(sub.n)]
+        sub.close()
+        rate_error = {"max_rel": max(e["err_max"] for e in errs),
+                      "mean_rel": sum(e["err_mean"] * e["err_n"] for e in errs) / max(1, sum(e["err_n"] for e in errs)),
+                      "rates_compared": sum(e["err_n"] for e in errs),
+                      "sample": "64 replicas x 2000 events, every rate of every evaluation"}
     value = ev / dt
     return {"value": value, "unit": "events/s", "replicas": len(cfgs), "sites": sites, "vacancies": nvac,
             "step": "one launch, every replica runs until a %.0f ms deadline" % (args.batch_seconds * 1e3),
             "events_timed": ev, "ms_per_step": dt / args.steps * 1e3,
-            "fixed_events": {"events_per_replica": args.batch_events, "seconds": dt_fixed,
-                             "events_per_s": st_fixed["events"] / dt_fixed},
+            "fixed_events": fixed,
             "kernel_seconds": ksec, "launches": launches, "dtype": "f32 rates, f64 sums/search/dt",
             "tuning": tun, "vs_plain": value / plain_value if plain_value else None,
-            "evaluations_per_event": n_eval / max(1, sum(1 for _ in cfgs)) / max(1, ev) * len(cfgs),
-            "rate_error": {"max_rel": max(e["err_max"] for e in errs),
-                           "mean_rel": sum(e["err_mean"] * e["err_n"] for e in errs) / max(1, sum(e["err_n"] for e in errs)),
-                           "rates_compared": sum(e["err_n"] for e in errs),
-                           "sample": "64 replicas x 2000 events, every rate of every evaluation"}}
+            "evaluations_per_event": d_eval / max(1, ev),
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS,
+                         "traffic": REPLICA_TRAFFIC_BYTES_PER_EVENT * ev / max(1, launches),
+                         "algorithmic_bytes_per_event": nbytes / max(1, ev)},
+            "rate_error": rate_error}
 
 
 def served_leg(args, batched_value):
@@ -196,6 +223,13 @@ def main():
     ap.add_argument("--batch-seconds", type=float, default=0.1, help="deadline of one batched step")
     ap.add_argument("--batch-slice", type=int, default=4096)
     ap.add_argument("--tune-seconds", type=float, default=0.01)
+    ap.add_argument("--no-tune", action="store_true", help="batched leg: keep the default tuning")
+    ap.add_argument("--batch-vcap", type=int, default=0, help="batched leg: fixed tuning (LDS capacity) instead of the tuner")
+    ap.add_argument("--batch-waves", type=int, default=1)
+    ap.add_argument("--batch-order", type=int, default=0)
+    ap.add_argument("--batch-warmup", type=int, default=1)
+    ap.add_argument("--no-fixed", action="store_true", help="batched leg: skip the fixed-event run")
+    ap.add_argument("--no-error-stats", action="store_true", help="batched leg: skip the FP32 error sample")
     ap.add_argument("--jobs", type=int, default=4000)
     ap.add_argument("--serve-scale", type=float, default=0.01,
                     help="scale of the job targets (1 = the full 1e5..1e7 events)")
@@ -245,7 +279,8 @@ def main():
                                    "at%, 6711 vacancies, T=573 K, nu=6e12/s, 8-sector sublattice, T_sector=1e-6 s",
                        "events_per_step": E, "events_timed": d_events, "launches": launches},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "frac": achieved / HBM_PEAK_GBS,
+                         "traffic": PLAIN_TRAFFIC_BYTES_PER_EVENT * d_events / max(1, launches),
                          "algorithmic_bytes_per_event": nbytes / max(1, d_events),
                          "kernel_us_per_event": ksec / max(1, d_events) * 1e6}})
     if args.copy_calibration:
