@@ -293,6 +293,15 @@ int akmc_batch_sector_rates(akmc_batch* b, int32_t replica, int sector, int64_t 
 
 void akmc_batch_destroy(akmc_batch* b);
 
+/* Solute statistics of a lattice (host only, any site array of the encoding
+ * above, box in unit cells): Cu atoms (sites reading as Cu) grouped into
+ * clusters by 1nn connectivity; hist[k] = clusters of k atoms (the last bin
+ * collects hist_cap - 1 and more); n_isolated = the reference's isolated-Cu
+ * count (profiles/m_event_hook.cpp:19-55: sites in the Cu set without a 1nn
+ * site reading as Cu). */
+int akmc_cu_clusters(const uint8_t* sites, const int64_t box[3], int64_t hist_cap, int64_t* hist,
+                     int64_t* n_clusters, int64_t* n_cu, int64_t* n_isolated);
+
 /* The FP32 rate table the replica engine uses at temperature T (K) and
  * attempt frequency nu: the rate of barrier index m (Ea = m * q / 2 eV) is
  * tab[256 + ((m - m_min) >> 8)] * tab[(m - m_min) & 255] in FP32.  Writes

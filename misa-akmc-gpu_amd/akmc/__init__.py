@@ -344,6 +344,7 @@ def _bind_batch(L):
     L.akmc_batch_upload.argtypes = [vp, i32, vp, i64]
     L.akmc_batch_sector_rates.argtypes = [vp, i32, i32, i64, vp, vp, P(dbl), P(i64)]
     L.akmc_fp32_rate_table.argtypes = [dbl, dbl, vp, i32, P(i32), P(i32), P(dbl), vp, vp, vp]
+    L.akmc_cu_clusters.argtypes = [vp, vp, i64, vp, P(i64), P(i64), P(i64)]
     L.akmc_batch_destroy.argtypes = [vp]
     L.akmc_batch_destroy.restype = None
     L.akmc_server_options_default.argtypes = [P(ServerOptions)]
@@ -382,6 +383,18 @@ def fp32_rate_table(temperature, nu=6e12):
                                         ctypes.byref(m_min), ctypes.byref(q), e1q.ctypes.data, e2q.ctypes.data,
                                         e0q2.ctypes.data))
     return tab[:256 + nhi.value].copy(), nhi.value, m_min.value, q.value, e1q, e2q, e0q2
+
+
+def cu_clusters(sites, box, hist_cap=64):
+    """Cu clusters (1nn connectivity) of a site array: dict with the size
+    histogram, cluster and atom counts, and the isolated-Cu count."""
+    sites = np.ascontiguousarray(sites, dtype=np.uint8)
+    b = np.array(box, dtype=np.int64)
+    hist = np.zeros(hist_cap, dtype=np.int64)
+    nc, ncu, niso = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    _check(_blib().akmc_cu_clusters(sites.ctypes.data, b.ctypes.data, hist_cap, hist.ctypes.data, ctypes.byref(nc),
+                                    ctypes.byref(ncu), ctypes.byref(niso)))
+    return {"hist": hist, "clusters": nc.value, "cu": ncu.value, "isolated": niso.value}
 
 
 def batch_options(**kw):

@@ -1,0 +1,146 @@
+"""Precision study of the FP32 rate form (512^3 workload, 16 seeds).
+
+    python3 misa-akmc-gpu_amd/akmc/studies.py precision --L 512 --seeds 16 --events 10000000 --out result.json
+    python3 misa-akmc-gpu_amd/akmc/studies.py plain --L 64 --seeds 16 --events 1000000 --out result.json
+
+`precision`: the same seeds (initial lattices from creation::createRandom)
+advanced by the replica engine with FP32 rates and with FP64 rates (the
+reference's exact rate evaluation), then Cu-cluster statistics of every
+final lattice, the per-rate error of the FP32 form (every rate of every
+evaluation against FP64), and whether the two final lattices of a seed are
+identical.  `plain`: the plain FP64 engine (the reference's trajectory) and
+the replica engine's FP32 form on the same seeds, compared statistically
+(their random streams differ).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+if __name__ == "__main__":
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+
+import numpy as np
+
+import akmc
+from akmc import workloads
+
+
+def _stats(sites, box, hist_cap):
+    c = akmc.cu_clusters(sites, box, hist_cap)
+    sizes = np.arange(hist_cap)
+    n = max(1, c["clusters"])
+    return {"clusters": c["clusters"], "cu": c["cu"], "isolated": c["isolated"],
+            "mean_size": float((c["hist"] * sizes).sum() / n), "max_size": int(sizes[c["hist"] > 0].max()),
+            "hist": c["hist"].tolist()}
+
+
+def _summary(per):
+    out = {}
+    for k in ("clusters", "isolated", "mean_size", "max_size"):
+        v = np.array([p[k] for p in per], dtype=np.float64)
+        out[k] = {"mean": float(v.mean()), "std": float(v.std(ddof=1)) if len(v) > 1 else 0.0}
+    out["hist_total"] = np.sum([p["hist"] for p in per], axis=0).tolist()
+    return out
+
+
+def _welch(a, b, key):
+    x = np.array([p[key] for p in a], dtype=np.float64)
+    y = np.array([p[key] for p in b], dtype=np.float64)
+    se = math.sqrt(x.var(ddof=1) / len(x) + y.var(ddof=1) / len(y)) if len(x) > 1 else 0.0
+    return float((x.mean() - y.mean()) / se) if se > 0 else 0.0
+
+
+def precision(L, seeds, events, hist_cap, chunk, log):
+    cfgs = [workloads.precision_512(k, L) for k in range(seeds)]
+    box = cfgs[0]["box"]
+    res = {"workload": "BCC %d^3, Fe-1.34Cu-0.70Ni-1.30Mn at%%, %d vacancies, T=563 K, %d seeds, %d events each"
+                       % (L, cfgs[0]["va_count"], seeds, events)}
+    finals = {}
+    for name, prec in (("fp32", akmc.AKMC_PREC_FP32), ("fp64", akmc.AKMC_PREC_FP64)):
+        t0 = time.perf_counter()
+        b = akmc.Batch(cfgs, precision=prec, init=akmc.AKMC_INIT_REFERENCE, error_stats=int(prec == akmc.AKMC_PREC_FP32))
+        log("%s: %d lattices created in %.1f s" % (name, seeds, time.perf_counter() - t0))
+        if name == "fp32":
+            res["initial"] = _summary([_stats(b.download(i), box, hist_cap) for i in range(seeds)])
+        done = 0
+        t0 = time.perf_counter()
+        while done < events:
+            n = min(chunk, events - done)
+            b.run(n)
+            done += n
+            log("%s: %d / %d events per seed, %.1f s" % (name, done, events, time.perf_counter() - t0))
+        dt = time.perf_counter() - t0
+        rs = [b.replica_stats(i) for i in range(seeds)]
+        finals[name] = [b.download(i) for i in range(seeds)]
+        per = [_stats(s, box, hist_cap) for s in finals[name]]
+        res[name] = {"seconds": dt, "events_per_s": events * seeds / dt, "summary": _summary(per), "per_seed": per,
+                     "sim_time": [float(np.sum(r["evolution"])) / 8 for r in rs]}
+        if name == "fp32":
+            res["rate_error"] = {"max_rel": max(r["err_max"] for r in rs),
+                                 "mean_rel": sum(r["err_mean"] * r["err_n"] for r in rs) / max(1, sum(r["err_n"] for r in rs)),
+                                 "rates_compared": sum(r["err_n"] for r in rs),
+                                 "rates_below_flt_min": sum(r["err_tiny"] for r in rs)}
+        b.close()
+    res["identical_final_lattices"] = int(sum(np.array_equal(a, c) for a, c in zip(finals["fp32"], finals["fp64"])))
+    res["welch_t"] = {k: _welch(res["fp32"]["per_seed"], res["fp64"]["per_seed"], k)
+                      for k in ("clusters", "isolated", "mean_size")}
+    return res
+
+
+def plain(L, seeds, events, hist_cap, log):
+    cfgs = []
+    for k in range(seeds):
+        c = workloads.precision_512(k, L)
+        c["physics_time"] = 1e3
+        cfgs.append(c)
+    box = cfgs[0]["box"]
+    res = {"workload": "BCC %d^3, %d vacancies, T=563 K, %d seeds, %d events each" % (L, cfgs[0]["va_count"], seeds,
+                                                                                       events)}
+    per_plain = []
+    t0 = time.perf_counter()
+    for k, c in enumerate(cfgs):
+        e = akmc.Engine(c)
+        e.run(events)
+        per_plain.append(_stats(e.download(), box, hist_cap))
+        e.close()
+        log("plain: seed %d done, %.1f s" % (k, time.perf_counter() - t0))
+    res["plain_fp64"] = {"seconds": time.perf_counter() - t0, "summary": _summary(per_plain), "per_seed": per_plain}
+    t0 = time.perf_counter()
+    b = akmc.Batch(cfgs, precision=akmc.AKMC_PREC_FP32, init=akmc.AKMC_INIT_REFERENCE)
+    b.run(events)
+    per_b = [_stats(b.download(i), box, hist_cap) for i in range(seeds)]
+    b.close()
+    log("replica fp32 done, %.1f s" % (time.perf_counter() - t0))
+    res["replica_fp32"] = {"seconds": time.perf_counter() - t0, "summary": _summary(per_b), "per_seed": per_b}
+    res["welch_t"] = {k: _welch(per_b, per_plain, k) for k in ("clusters", "isolated", "mean_size")}
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("study", choices=["precision", "plain"])
+    ap.add_argument("--L", type=int, default=512)
+    ap.add_argument("--seeds", type=int, default=16)
+    ap.add_argument("--events", type=int, default=10_000_000)
+    ap.add_argument("--chunk", type=int, default=1_000_000)
+    ap.add_argument("--hist-cap", type=int, default=64)
+    ap.add_argument("--out", default="")
+    a = ap.parse_args()
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    if a.study == "precision":
+        res = precision(a.L, a.seeds, a.events, a.hist_cap, a.chunk, log)
+    else:
+        res = plain(a.L, a.seeds, a.events, a.hist_cap, log)
+    text = json.dumps(res)
+    if a.out:
+        with open(a.out, "w") as f:
+            f.write(text)
+    print(json.dumps({k: v for k, v in res.items() if k not in ("fp32", "fp64", "plain_fp64", "replica_fp32")}))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

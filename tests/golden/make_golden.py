@@ -2,7 +2,8 @@
 oracle (oracle/refcpu.cpp).  Each fixture holds the configuration, the first
 events' (from, to) site indices, a SHA-256 of the final site array, the
 sector clocks and the partial sector time.  Run from the repository root:
-    python3 tests/golden/make_golden.py
+    python3 tests/golden/make_golden.py        # traj_*.npz
+    python3 tests/golden/make_golden.py cfg0   # hashes_cfg0_40cube.npz (1e5 events, a few minutes)
 """
 import hashlib
 import json
@@ -26,7 +27,31 @@ CASES = {
 }
 
 
+# Config 0 of the benchmark suite at full length: 1e5 events of the 40^3
+# box, with the occupancy hashed every 1000 events.
+CFG0 = make_cfg(n=40, va=26, T=573.0, sector_T=3e-7, physics=1e3)
+CFG0_EVENTS, CFG0_EVERY = 100000, 1000
+
+
+def cfg0_hashes():
+    r = refcpu.RefCpu(CFG0)
+    digests = []
+    for k in range(CFG0_EVENTS // CFG0_EVERY):
+        r.run(CFG0_EVERY)
+        assert r.state()["events"] == (k + 1) * CFG0_EVERY
+        digests.append(np.frombuffer(hashlib.sha256(r.sites().tobytes()).digest(), dtype=np.uint8))
+    st = r.state()
+    np.savez_compressed(os.path.join(HERE, "hashes_cfg0_40cube.npz"), cfg=json.dumps(CFG0),
+                        events=np.int64(st["events"]), every=np.int64(CFG0_EVERY), sha256=np.stack(digests),
+                        evolution=st["evolution"], sector_time=np.float64(st["sector_time"]),
+                        step=np.int64(st["step"]))
+    print("cfg0 hashes", st["events"], "events")
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "cfg0":
+        cfg0_hashes()
+        return
     for name, (cfg, n) in CASES.items():
         r = refcpu.RefCpu(cfg)
         rc, lf, lt = r.run(n, log=True)

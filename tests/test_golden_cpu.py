@@ -48,3 +48,15 @@ def test_oracle_invariants():
     dy = (ty - fy + Y // 2) % Y - Y // 2
     dz = (tz - fz + Z // 2) % Z - Z // 2
     assert np.all(np.abs(dx) == 1) and np.all(np.abs(dy) <= 1) and np.all(np.abs(dz) <= 1)  # 1nn hops
+
+
+def test_oracle_matches_cfg0_hashes_prefix():
+    """The first 20,000 events of the config-0 fixture (1e5 events, hashed
+    every 1000) reproduce on the oracle."""
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "hashes_cfg0_40cube.npz"), allow_pickle=False)
+    r = refcpu.RefCpu(json.loads(str(z["cfg"])))
+    every = int(z["every"])
+    for k in range(20):
+        r.run(every)
+        got = np.frombuffer(hashlib.sha256(r.sites().tobytes()).digest(), dtype=np.uint8)
+        assert np.array_equal(got, z["sha256"][k]), k

@@ -192,3 +192,23 @@ def test_full_size_prefix_and_invariants():
     assert ((s1 >> 1) & 1).sum() == ((s0 >> 1) & 1).sum()
     assert s1.max() < 16
     eng.close()
+
+
+def test_config0_full_length_hashes():
+    """Config 0 at full length: 1e5 events of the 40^3 box, the occupancy
+    hash after every 1000 events, the sector clocks and the partial sector
+    time, against the oracle's fixture (tests/golden/make_golden.py cfg0)."""
+    say("test_config0_full_length_hashes")
+    path = os.path.join(os.path.dirname(__file__), "golden", "hashes_cfg0_40cube.npz")
+    z = np.load(path, allow_pickle=False)
+    cfg = json.loads(str(z["cfg"]))
+    every = int(z["every"])
+    eng = akmc.Engine(cfg)
+    for k, want in enumerate(z["sha256"]):
+        st = eng.run(every)
+        assert st["events"] == (k + 1) * every
+        got = np.frombuffer(hashlib.sha256(eng.download().tobytes()).digest(), dtype=np.uint8)
+        assert np.array_equal(got, want), "occupancy differs after %d events" % ((k + 1) * every)
+    assert np.array_equal(_bits(st["evolution"]), _bits(z["evolution"]))
+    assert st["sector_time"] == float(z["sector_time"]) and st["step"] == int(z["step"])
+    eng.close()

@@ -1,0 +1,25 @@
+"""Precision form on the GPU: FP32 rates against FP64 rates on the same seeds
+(the 512^3 study of misa-akmc-gpu_amd/akmc/studies.py at a reduced size).
+
+Bars: every FP32 rate within 1e-6 (relative) of the FP64 rate of the same
+stencil, and the Cu-cluster statistics of the final lattices the same in
+distribution; with the same seeds most final lattices are identical, since a
+selection can only differ when u * total falls within ~1e-7 of a boundary."""
+import sys
+
+import pytest
+
+from akmc import studies
+
+pytestmark = pytest.mark.gpu
+
+
+def test_precision_study_small():
+    print("test_precision_study_small", flush=True)
+    sys.stdout.flush()
+    res = studies.precision(L=32, seeds=6, events=20000, hist_cap=32, chunk=10000, log=lambda m: None)
+    assert res["rate_error"]["rates_compared"] > 10000
+    assert res["rate_error"]["max_rel"] <= 1e-6
+    assert res["identical_final_lattices"] >= 4
+    for k, t in res["welch_t"].items():
+        assert abs(t) < 4.0, (k, t)
