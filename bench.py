@@ -76,6 +76,31 @@ def cpu_baseline(events, seconds_budget):
                       "1 thread, initial occupancy excluded, %.1f s" % (done, dt)}
 
 
+def _oracle_replica(cfg, seconds):
+    """Events/s of the oracle on one replica (a worker process: no GPU)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import refcpu
+    ref = refcpu.RefCpu(cfg)
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        ref.run(5)
+    return ref.state()["events"], time.perf_counter() - t0
+
+
+def cpu_batched_baseline(cfgs, seconds):
+    """The oracle on independent replicas, one per host core (one process
+    each, started fresh), for `seconds`: aggregate events/s."""
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+    cores = max(1, min(16, os.cpu_count() or 1))
+    with ProcessPoolExecutor(max_workers=cores, mp_context=mp.get_context("spawn")) as ex:
+        res = list(ex.map(_oracle_replica, cfgs[:cores], [seconds] * cores))
+    value = sum(e / t for e, t in res)
+    return {"value": value, "unit": "events/s", "cores": cores, "kind": "port",
+            "sample": "oracle/refcpu.cpp on the first %d replicas of the same batch, one process per core, "
+                      "%.0f s each, %d events in total" % (cores, seconds, sum(e for e, _ in res))}
+
+
 def batched_leg(args, plain_value):
     """Replica engine throughput (events of all replicas per second)."""
     import akmc
@@ -149,7 +174,8 @@ def batched_leg(args, plain_value):
                          "frac": achieved / HBM_PEAK_GBS,
                          "traffic": REPLICA_TRAFFIC_BYTES_PER_EVENT * ev / max(1, launches),
                          "algorithmic_bytes_per_event": nbytes / max(1, ev)},
-            "rate_error": rate_error}
+            "rate_error": rate_error,
+            "cpu_baseline": None if args.no_cpu_baseline else cpu_batched_baseline(cfgs, args.cpu_batched_seconds)}
 
 
 def served_leg(args, batched_value):
@@ -213,6 +239,7 @@ def main():
     ap.add_argument("--cpu-events", type=int, default=400)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-batched-seconds", type=float, default=8.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run the streaming copy kernel (profiler counter calibration)")
     ap.add_argument("--no-plain", action="store_true")
