@@ -300,22 +300,36 @@ __device__ double tail_update(SectorLDS<VCAP, RateT>& S, int v0, int n, double P
       v = gend;
       continue;
     }
-    // Exact fold of the next 64 vacancies from the offending one: where the
-    // new sums crossed a binade boundary ahead of the old ones, every vacancy
-    // until the old sums cross too would fail the shift check.
+    // Serial exact fold from the offending vacancy until a folded vacancy's
+    // old and new sums share a binade again: where the new sums crossed a
+    // power of two ahead of the old ones (or behind them), the vacancies in
+    // between -- normally one to three -- fail the shift check; a tie or a
+    // binade boundary inside one vacancy's rates fails for that vacancy
+    // alone.  The shift resumes after it with the new exact difference, and
+    // the checks above validate every later vacancy as before.
     wave_sync();
-    const int fend = min(n, fail + 64);
-    const double Eo_end = S.pvac[fend - 1];
-    const double pstart = fail == v ? Pprev : S.pvac[fail - 1];
-    const double p = fold_range<VCAP, RateT>(S, fail, fend, pstart);
-    D = p - Eo_end;
-    Sold = Eo_end;
-    Pprev = p;
-    v = fend;
-    if (!(Eo_end + D == p && p - D == Eo_end)) {
-      wave_sync();
-      return v < n ? fold_range<VCAP, RateT>(S, v, n, p) : p;
+    double p = fail == v ? Pprev : S.pvac[fail - 1];
+    double old_w = 0.0;
+    int w = fail;
+    for (;;) {
+      old_w = S.pvac[w];
+      double r8[8];
+#pragma unroll
+      for (int b = 0; b < 8; b++) r8[b] = (double)S.rate[8 * w + b];
+#pragma unroll
+      for (int b = 0; b < 8; b++) p += r8[b];
+      if (l == 0) S.pvac[w] = p;
+      w++;
+      const bool same = (exponent_of(old_w) == exponent_of(p)) & (old_w >= 0x1p-969);
+      if (w >= n || same || w - fail >= 64) break;
     }
+    wave_sync();
+    if (w >= n) return p;
+    D = p - old_w;
+    Sold = old_w;
+    Pprev = p;
+    v = w;
+    if (!(old_w + D == p && p - D == old_w)) return fold_range<VCAP, RateT>(S, v, n, p);
   }
   wave_sync();
   return Pprev;
