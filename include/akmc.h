@@ -81,7 +81,7 @@ typedef struct akmc_stats {
   double sector_threshold;   /* threshold of the sector in progress */
   double sector_time;        /* partial time of the sector in progress */
   int32_t stop;              /* AKMC_STOP_* of the last akmc_run */
-  int32_t reserved;
+  int32_t sum_errors;        /* events whose running sums disagreed with a serial refold (akmc_set_check_sums) */
   double seconds;            /* wall time of the last akmc_run */
   double kernel_seconds;     /* device time of the event-loop launches of the last akmc_run */
   int64_t launches;          /* event-loop launches of the last akmc_run */
@@ -146,6 +146,14 @@ int akmc_get_event_log(akmc_engine* e, int64_t* from, int64_t* to, int64_t cap, 
  * the event loop consumes at most one chunk, so this sets the launch
  * granularity.  Only before the first akmc_run. */
 int akmc_set_rng_chunk(akmc_engine* e, int64_t events);
+
+/* Self-check of the running sums: when on, after every event the event loop
+ * refolds the current sector's rates serially in sweep order (the order of
+ * ABVIModel::calcRates, models/abvi/kmc.cpp:103) and compares every running
+ * sum and the total with the ones it keeps incrementally; disagreements are
+ * counted in akmc_stats.sum_errors.  A test aid with no counterpart in the
+ * reference: it costs a full sector fold per event. */
+int akmc_set_check_sums(akmc_engine* e, int on);
 
 /* Events [first, first + n) of the two random streams akmc_run consumes, one
  * pair per event: u = r::random() of a default-seeded mt19937

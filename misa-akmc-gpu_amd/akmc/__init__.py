@@ -51,7 +51,7 @@ class Stats(ctypes.Structure):
     _fields_ = [("events", ctypes.c_int64), ("step", ctypes.c_int64), ("sector_pos", ctypes.c_int32),
                 ("in_sector", ctypes.c_int32), ("evolution", ctypes.c_double * 8),
                 ("sector_threshold", ctypes.c_double), ("sector_time", ctypes.c_double),
-                ("stop", ctypes.c_int32), ("reserved", ctypes.c_int32), ("seconds", ctypes.c_double),
+                ("stop", ctypes.c_int32), ("sum_errors", ctypes.c_int32), ("seconds", ctypes.c_double),
                 ("kernel_seconds", ctypes.c_double), ("launches", ctypes.c_int64),
                 ("n_eval", ctypes.c_int64), ("n_fold", ctypes.c_int64), ("n_shift", ctypes.c_int64)]
 
@@ -61,7 +61,7 @@ class Stats(ctypes.Structure):
                 "sector_threshold": self.sector_threshold, "sector_time": self.sector_time,
                 "stop": self.stop, "seconds": self.seconds, "kernel_seconds": self.kernel_seconds,
                 "launches": self.launches, "n_eval": self.n_eval, "n_fold": self.n_fold,
-                "n_shift": self.n_shift}
+                "n_shift": self.n_shift, "sum_errors": self.sum_errors}
 
 
 _lib = None
@@ -93,6 +93,7 @@ def lib():
     L.akmc_set_event_log.argtypes = [vp, i64]
     L.akmc_get_event_log.argtypes = [vp, vp, vp, i64, P(i64)]
     L.akmc_set_rng_chunk.argtypes = [vp, i64]
+    L.akmc_set_check_sums.argtypes = [vp, ctypes.c_int]
     L.akmc_rng_streams.argtypes = [i64, i64, i64, vp, vp]
     L.akmc_sync.argtypes = [i32]
     L.akmc_copy_bench.argtypes = [i32, i64, i32, P(dbl)]
@@ -245,6 +246,10 @@ class Engine:
 
     def set_rng_chunk(self, events):
         _check(lib().akmc_set_rng_chunk(self.h, int(events)))
+
+    def set_check_sums(self, on=True):
+        """Self-check of the running sums after every event (a test aid)."""
+        _check(lib().akmc_set_check_sums(self.h, int(bool(on))))
 
     def set_event_log(self, capacity):
         _check(lib().akmc_set_event_log(self.h, int(capacity)))

@@ -146,6 +146,7 @@ struct akmc_engine {
   int64_t log_cap = 0;
   int64_t log_base = 0;
   bool lattice_ready = false;
+  int32_t check_sums = 0;
   akmc_stats last{};
 };
 
@@ -202,6 +203,7 @@ void fill_stats(akmc_engine* e, akmc_stats* s) {
   s->n_eval = st.n_eval;
   s->n_fold = st.n_fold;
   s->n_shift = st.n_shift;
+  s->sum_errors = st.sum_errors;
 }
 
 akmc::PlainArgs base_args(akmc_engine* e) {
@@ -216,6 +218,7 @@ akmc::PlainArgs base_args(akmc_engine* e) {
   a.nu = e->p.attempt_freq;
   a.kT = e->kT;
   a.rates_sector = -1;
+  a.check_sums = e->check_sums;
   return a;
 }
 
@@ -570,6 +573,12 @@ int akmc_set_rng_chunk(akmc_engine* e, int64_t events) {
     return fail(AKMC_ERR_INVALID, "chunk must be in [1, %lld]", (long long)kChunkMax);
   if (e->generated != 0) return fail(AKMC_ERR_INVALID, "the chunk size can only be set before the first run");
   e->chunk = events;
+  return AKMC_OK;
+}
+
+int akmc_set_check_sums(akmc_engine* e, int on) {
+  if (!e) return fail(AKMC_ERR_INVALID, "null engine");
+  e->check_sums = on ? 1 : 0;
   return AKMC_OK;
 }
 

@@ -29,7 +29,7 @@ struct PlainState {
   int32_t sect;       // ring position 0..7 (ring 0,7,2,5,3,4,1,6)
   int32_t in_sector;  // 1 while a sector is in progress
   int32_t status;     // AKMC_RUN_* of the last launch
-  int32_t pad;
+  int32_t sum_errors; // events whose running sums a serial refold disagreed with (check_sums)
   // Work counters (algorithmic traffic bookkeeping).
   int64_t n_eval;     // vacancy evaluations (55-site stencil gathers)
   int64_t n_fold;     // vacancies whose running sum was folded from their rates
@@ -58,7 +58,9 @@ struct PlainArgs {
   // sector (calcRates with its side effect) and writes the sorted keys,
   // rates and total, then returns.
   int32_t rates_sector;
-  int32_t pad;
+  // Self-check: after every update of the running sums, every lane refolds
+  // the sector serially in sweep order and compares (counted in sum_errors).
+  int32_t check_sums;
   uint32_t* out_keys;
   double* out_rates;
   double* out_total;

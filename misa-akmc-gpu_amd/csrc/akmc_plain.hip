@@ -128,6 +128,21 @@ __device__ void move_entry(PlainSector& S, int nv, int from_i, int to_i) {
   }
 }
 
+// Self-check: the running sums and the total against a serial refold of the
+// sector's rates in sweep order (every lane folds the same values, so the
+// result is wave-uniform).
+__device__ bool sums_agree(const PlainSector& S, int nv, double sum) {
+  wave_sync();
+  double p = 0.0;
+  bool ok = true;
+  for (int v = 0; v < nv; v++) {
+#pragma unroll
+    for (int b = 0; b < 8; b++) p += S.rate[8 * v + b];
+    ok = ok & (p == S.pvac[v]);
+  }
+  return ok & (p == (nv > 0 ? sum : 0.0));
+}
+
 __global__ __launch_bounds__(64) void plain_kernel(PlainArgs a) {
   __shared__ PlainSector S;
   __shared__ SharedTables T;
@@ -194,6 +209,7 @@ __global__ __launch_bounds__(64) void plain_kernel(PlainArgs a) {
       st.n_eval += nv;
       st.n_fold += nv;
       sum = fold_range<PLAIN_VCAP, double>(S, 0, nv, 0.0);
+      if (a.check_sums && !sums_agree(S, nv, sum)) st.sum_errors++;
       rebuild = false;
       if (a.rates_sector >= 0) {
         for (int i = l; i < nv && i < a.out_cap; i += 64) {
@@ -356,6 +372,7 @@ __global__ __launch_bounds__(64) void plain_kernel(PlainArgs a) {
       st.n_shift += nv - (c_hi + 1);
       sum = c_hi + 1 < nv ? tail_update<PLAIN_VCAP, double>(S, c_hi + 1, nv, Pn, Po) : Pn;
     }
+    if (a.check_sums && !sums_agree(S, nv, sum)) st.sum_errors++;
   }
   if (l == 0 && a.rates_sector < 0) {
     st.status = status;

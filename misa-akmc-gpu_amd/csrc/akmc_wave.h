@@ -97,6 +97,15 @@ __device__ __forceinline__ bool tie_free(uint64_t mask, int e) {
   return in_window & !((mask >> (t & 63)) & 1);
 }
 
+// Tie risk of a vacancy whose running sum went from So to Eo over its 8
+// slots: set unless both lie in one normal binade e and no rate can tie
+// there, i.e. unless a later exact shift inside that binade reproduces the
+// serial fold.
+__device__ __forceinline__ uint8_t fold_risk(double So, double Eo, uint64_t tm) {
+  const int e = exponent_of(So);
+  return (uint8_t)!((So >= 0x1p-969) & (exponent_of(Eo) == e) & tie_free(tm, e));
+}
+
 // ---------------------------------------------------------------- state
 constexpr int EVB = 8;  // vacancies evaluated per gather/compute pass
 
@@ -107,6 +116,7 @@ struct SectorLDS {
   RateT rate[VCAP * 8];
   double pvac[VCAP];       // running sum after each vacancy
   uint64_t tmask[VCAP];    // tie mask of each vacancy's rates
+  uint8_t risk[VCAP];      // 1 if shifting this vacancy's sums needs a serial fold (fold_risk)
   uint8_t own[VCAP];       // flag byte of the vacancy site
   uint8_t nb1[VCAP * 8];   // flag bytes of its 8 1nn sites
   uint8_t sten[EVB][STENCIL_MAX];
@@ -185,10 +195,14 @@ __device__ double fold_range(SectorLDS<VCAP, RateT>& S, int v0, int v1, double P
       double a[8];
 #pragma unroll
       for (int b = 0; b < 8; b++) a[b] = (double)S.rate[8 * v + b];
+      const uint64_t tm = S.tmask[v];
       double p = P;
 #pragma unroll
       for (int b = 0; b < 8; b++) p += a[b];
-      if (l == 0) S.pvac[v] = p;
+      if (l == 0) {
+        S.pvac[v] = p;
+        S.risk[v] = fold_risk(P, p, tm);
+      }
       P = p;
       v++;
       continue;
@@ -224,21 +238,33 @@ __device__ double fold_range(SectorLDS<VCAP, RateT>& S, int v0, int v1, double P
     // whether all of its slots stayed inside the binade.
     bad = bad | !(P + incl * u < limit);
     const int fb = first_set(ballot(active & bad));
-    if (active & (l < fb)) S.pvac[vi] = P + incl * u;
+    if (active & (l < fb)) {
+      // All of this vacancy's sums lie in P's binade e: only ties matter.
+      S.pvac[vi] = P + incl * u;
+      S.risk[vi] = (uint8_t)!tie_free(S.tmask[vi], e);
+    }
     if (fb == 64) {
       const int last = min(63, v1 - 1 - v);
       P = P + bcast_d(incl, last) * u;
       v += last + 1;
     } else {
-      const double excl = bcast_d(incl - local, fb);
-      double p = P + excl * u;  // exact: every slot before this vacancy stayed in the binade
+      // The sum before this vacancy is the inclusive sum of the lane before
+      // it, which stayed in the binade (below 2^52, so exact); incl - local
+      // here would round once the bad lane's own total passes 2^53.
+      const double excl = fb == 0 ? 0.0 : bcast_d(incl, fb - 1);
+      double p = P + excl * u;
       const int vb = v + fb;
+      const uint64_t tm = S.tmask[vb];
       double ab[8];
 #pragma unroll
       for (int b = 0; b < 8; b++) ab[b] = (double)S.rate[8 * vb + b];
+      const double p0 = p;
 #pragma unroll
       for (int b = 0; b < 8; b++) p += ab[b];
-      if (l == 0) S.pvac[vb] = p;
+      if (l == 0) {
+        S.pvac[vb] = p;
+        S.risk[vb] = fold_risk(p0, p, tm);
+      }
       P = p;
       v = vb + 1;
     }
@@ -250,86 +276,116 @@ __device__ double fold_range(SectorLDS<VCAP, RateT>& S, int v0, int v1, double P
 // Shifts the running sums of vacancies [v0, n) whose rates did not change
 // after the sum before them moved from Po to Pn.  pvac[v] still holds old
 // values on entry for v >= v0.  Returns the new total (pvac[n - 1]).
-// Works on groups of 256 vacancies: all old sums of a group are loaded
-// before anything of it is written, so the checks need no cross-lane chain.
+//
+// Region by region: with the old sum before v in binade e (S_old) and the
+// new one (P_new) in the same binade, D = P_new - S_old is a multiple of
+// that binade's ulp, and every following vacancy whose old sums lie in
+// [2^e, 2^(e+1) - max(D, 0)) and which carries no tie risk (risk[], kept by
+// the folds) shifts by exactly D.  One pass over the region loads, checks
+// (two compares per vacancy: the sums are sorted, so the first vacancy
+// whose end passes the bound ends the region) and writes 256 vacancies per
+// round.  The vacancies where it ends are folded serially, four per LDS
+// round trip, until old and new sums share a binade again, and the next
+// region starts with the new exact difference.
 template <int VCAP, typename RateT>
 __device__ double tail_update(SectorLDS<VCAP, RateT>& S, int v0, int n, double Pn, double Po) {
   const int l = lane_id();
-  double D = Pn - Po;
-  if (!(Po + D == Pn && Pn - D == Po)) return fold_range<VCAP, RateT>(S, v0, n, Pn);
   int v = v0;
   double Sold = Po;   // old running sum before v
   double Pprev = Pn;  // new running sum before v
+  bool capped = false;
   while (v < n) {
+    const double D = Pprev - Sold;
     if (D == 0.0) {
       wave_sync();
       return S.pvac[n - 1];  // the rest is unchanged
     }
-    constexpr int G = 4;
-    double Eo[G], So[G];
-    uint64_t tm[G];
+    const int e = exponent_of(Sold);
+    const bool start_ok = (Sold >= 0x1p-969) & (exponent_of(Pprev) == e) & (Sold + D == Pprev) & (Pprev - D == Sold);
+    if (!start_ok && capped) return fold_range<VCAP, RateT>(S, v, n, Pprev);
+    int stop = v;
+    if (start_ok) {
+      // D < 2^e here, so top - D is exact.
+      const double top = __builtin_ldexp(1.0, e + 1);
+      const double thr = D > 0.0 ? top - D : top;
+      stop = n;
+      for (int c = v; c < n; c += 256) {
+        double x[4];
+        uint8_t rk[4];
 #pragma unroll
-    for (int r = 0; r < G; r++) {
-      const int vc = min(v + 64 * r + l, n - 1);  // v >= 1 here, so vc - 1 >= 0
-      Eo[r] = S.pvac[vc];
-      So[r] = S.pvac[vc - 1];
-      tm[r] = S.tmask[vc];
-    }
-    if (l == 0) So[0] = Sold;
-    const int gend = min(n, v + 64 * G);
-    const double Eo_last = S.pvac[gend - 1];
-    int fail = gend;
+        for (int r = 0; r < 4; r++) {
+          const int ic = min(c + 64 * r + l, n - 1);
+          x[r] = S.pvac[ic];
+          rk[r] = S.risk[ic];
+        }
+        int f = n;
 #pragma unroll
-    for (int r = 0; r < G; r++) {
-      const int vi = v + 64 * r + l;
-      const double Sn = So[r] + D, En = Eo[r] + D;
-      const int e = exponent_of(So[r]);
-      const bool ok = (So[r] >= 0x1p-969) & (exponent_of(Eo[r]) == e) & (exponent_of(Sn) == e) &
-                      (exponent_of(En) == e) & tie_free(tm[r], e);
-      const uint64_t m = ballot((vi < n) & !ok);
-      if (fail == gend && m) fail = v + 64 * r + first_set(m);
-    }
+        for (int r = 0; r < 4; r++) {
+          const int i = c + 64 * r + l;
+          const uint64_t m = ballot((i < n) & !((x[r] < thr) & (rk[r] == 0)));
+          if (f == n && m) f = c + 64 * r + first_set(m);
+        }
 #pragma unroll
-    for (int r = 0; r < G; r++) {
-      const int vi = v + 64 * r + l;
-      if (vi < fail) S.pvac[vi] = Eo[r] + D;
+        for (int r = 0; r < 4; r++) {
+          const int i = c + 64 * r + l;
+          if (i < f) S.pvac[i] = x[r] + D;
+        }
+        if (f < n) {
+          stop = f;
+          break;
+        }
+      }
+      wave_sync();
+      if (stop >= n) return S.pvac[n - 1];
     }
-    if (fail == gend) {
-      Sold = Eo_last;
-      Pprev = Eo_last + D;
-      v = gend;
-      continue;
-    }
-    // Serial exact fold from the offending vacancy until a folded vacancy's
-    // old and new sums share a binade again: where the new sums crossed a
-    // power of two ahead of the old ones (or behind them), the vacancies in
-    // between -- normally one to three -- fail the shift check; a tie or a
-    // binade boundary inside one vacancy's rates fails for that vacancy
-    // alone.  The shift resumes after it with the new exact difference, and
-    // the checks above validate every later vacancy as before.
-    wave_sync();
-    double p = fail == v ? Pprev : S.pvac[fail - 1];
+    // Serial exact fold from `stop` until a folded vacancy's old and new sums
+    // share a binade again (normally one to three vacancies next to a power
+    // of two, or the risky vacancy alone).  Every lane folds the same four
+    // vacancies; lane j writes vacancy w + j.
+    double p = stop == v ? Pprev : S.pvac[stop - 1];
     double old_w = 0.0;
-    int w = fail;
+    int w = stop;
+    bool same = false;
     for (;;) {
-      old_w = S.pvac[w];
-      double r8[8];
+      double r4[4][8], o4[4];
+      uint64_t t4[4];
 #pragma unroll
-      for (int b = 0; b < 8; b++) r8[b] = (double)S.rate[8 * w + b];
+      for (int j = 0; j < 4; j++) {
+        const int wj = min(w + j, n - 1);
+        o4[j] = S.pvac[wj];
+        t4[j] = S.tmask[wj];
 #pragma unroll
-      for (int b = 0; b < 8; b++) p += r8[b];
-      if (l == 0) S.pvac[w] = p;
-      w++;
-      const bool same = (exponent_of(old_w) == exponent_of(p)) & (old_w >= 0x1p-969);
-      if (w >= n || same || w - fail >= 64) break;
+        for (int b = 0; b < 8; b++) r4[j][b] = (double)S.rate[8 * wj + b];
+      }
+      const int kmax = min(4, n - w);
+      int k = 0;
+      double pw = 0.0, pw0 = 0.0;
+      uint64_t tw = 0;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (j < kmax && !same && w + j - stop < 64) {
+          const double p0 = p;
+#pragma unroll
+          for (int b = 0; b < 8; b++) p += r4[j][b];
+          old_w = o4[j];
+          same = (exponent_of(old_w) == exponent_of(p)) & (old_w >= 0x1p-969);
+          if (l == j) { pw = p; pw0 = p0; tw = t4[j]; }
+          k = j + 1;
+        }
+      }
+      if (l < k) {
+        S.pvac[w + l] = pw;
+        S.risk[w + l] = fold_risk(pw0, pw, tw);
+      }
+      w += k;
+      if (w >= n || same || w - stop >= 64) break;
     }
     wave_sync();
     if (w >= n) return p;
-    D = p - old_w;
+    capped = !same;
     Sold = old_w;
     Pprev = p;
     v = w;
-    if (!(old_w + D == p && p - D == old_w)) return fold_range<VCAP, RateT>(S, v, n, p);
   }
   wave_sync();
   return Pprev;

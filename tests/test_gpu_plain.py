@@ -137,6 +137,36 @@ def test_golden_trajectories(path):
     assert st["sector_time"] == float(z["sector_time"])
 
 
+SUM_CASES = [c[1] for c in TRAJ_CASES] + [
+    make_cfg(n=24, va=200, T=520.0),
+    make_cfg(n=24, va=400, T=700.0, alloy=[("Fe", 7000), ("Cu", 1500), ("Mn", 1000), ("Ni", 500)]),
+    make_cfg(n=20, va=900, T=573.0),
+]
+
+
+def test_running_sums_self_check():
+    """The incrementally kept running sums (block folds, exact tail shifts,
+    serial folds at binade crossings and ties) against a serial refold of the
+    sector after every event, on the trajectory cases, the golden cases and
+    the 256^3 benchmark box."""
+    say("test_running_sums_self_check")
+    cfgs = list(SUM_CASES)
+    for path in GOLDEN:
+        cfgs.append(json.loads(str(np.load(path, allow_pickle=False)["cfg"])))
+    for cfg in cfgs:
+        eng = akmc.Engine(cfg)
+        eng.set_check_sums(True)
+        st = eng.run(6000)
+        assert st["sum_errors"] == 0, (cfg, st["sum_errors"], st["events"])
+        eng.close()
+    cfg = make_cfg(n=256, va=6711, T=573.0, sector_T=1e-6, physics=1.0)
+    eng = akmc.Engine(cfg)
+    eng.set_check_sums(True)
+    st = eng.run(20000)
+    assert st["events"] == 20000 and st["sum_errors"] == 0
+    eng.close()
+
+
 def test_time_loop_end_and_empty_box():
     say("test_time_loop_end_and_empty_box")
     cfg = make_cfg(n=12, va=0, physics=3e-5, sector_T=3e-7)
