@@ -392,6 +392,89 @@ __device__ double tail_update(SectorLDS<VCAP, RateT>& S, int v0, int n, double P
 }
 
 // ---------------------------------------------------------------- rates
+// Stencil indices of one lane's direction b = l & 7, packed four per word:
+// the 13 counted sites around the vacancy (8 1nn, then 5 2nn) and the 13
+// around its target t_b, for both parities, plus t_b itself.  Held in
+// registers so the species loads of a rate issue together.
+struct LaneStencil {
+  uint32_t v[2][4];
+  uint32_t t[2][4];
+  uint32_t x;  // byte par: stencil index of t_b
+};
+__device__ __forceinline__ LaneStencil lane_stencil(const StencilTables& st) {
+  const int b = lane_id() & 7;
+  LaneStencil R;
+  for (int par = 0; par < 2; par++) {
+    uint8_t bv[16], bt[16];
+    for (int k = 0; k < N1; k++) { bv[k] = st.w1[par][k]; bt[k] = st.t1[par][b][k]; }
+    for (int k = 0; k < N2; k++) { bv[N1 + k] = st.w2[par][k]; bt[N1 + k] = st.t2[par][b][k]; }
+    for (int k = N1 + N2; k < 16; k++) { bv[k] = 0; bt[k] = 0; }
+    for (int q = 0; q < 4; q++) {
+      R.v[par][q] = (uint32_t)bv[4 * q] | ((uint32_t)bv[4 * q + 1] << 8) | ((uint32_t)bv[4 * q + 2] << 16) |
+                    ((uint32_t)bv[4 * q + 3] << 24);
+      R.t[par][q] = (uint32_t)bt[4 * q] | ((uint32_t)bt[4 * q + 1] << 8) | ((uint32_t)bt[4 * q + 2] << 16) |
+                    ((uint32_t)bt[4 * q + 3] << 24);
+    }
+  }
+  R.x = (uint32_t)st.w1[0][b] | ((uint32_t)st.w1[1][b] << 8);
+  return R;
+}
+
+// rate_fp64 (akmc_device.h) for the stencil species sp[] of one vacancy,
+// direction b = l & 7, with the same FP64 operations in the same order; the
+// loads are staged (all species, then all bond terms) so that they overlap.
+__device__ __forceinline__ double rate_fp64_staged(const uint8_t* sp, int par, const LaneStencil& LS,
+                                                   const double* e1, const double* e2, const double* e0,
+                                                   double nu, double kT, const uint64_t* exp_tab, int& Xout) {
+  const int b = lane_id() & 7;
+  uint32_t vw[4], tw[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    vw[q] = par ? LS.v[1][q] : LS.v[0][q];
+    tw[q] = par ? LS.t[1][q] : LS.t[0][q];
+  }
+  const int X = sp[par ? (LS.x >> 8) & 255u : LS.x & 255u];
+  Xout = X;
+  int sv[N1 + N2], st[N1 + N2];
+#pragma unroll
+  for (int k = 0; k < N1 + N2; k++) {
+    sv[k] = sp[(vw[k >> 2] >> (8 * (k & 3))) & 255u];
+    st[k] = sp[(tw[k >> 2] >> (8 * (k & 3))) & 255u];
+  }
+  if (X == SP_V) return 0.0;
+  const double* v1 = e1 + SP_V * 5;
+  const double* v2 = e2 + SP_V * 5;
+  const double* x1 = e1 + X * 5;
+  const double* x2 = e2 + X * 5;
+  double tsb[N1 + N2], tdb[N1 + N2], tsa[N1 + N2], tda[N1 + N2];
+#pragma unroll
+  for (int k = 0; k < N1; k++) {
+    tsb[k] = v1[sv[k]];
+    tdb[k] = x1[st[k]];
+    tsa[k] = x1[k == b ? (int)SP_V : sv[k]];
+    tda[k] = v1[k == 7 - b ? X : st[k]];
+  }
+#pragma unroll
+  for (int k = N1; k < N1 + N2; k++) {
+    tsb[k] = v2[sv[k]];
+    tdb[k] = x2[st[k]];
+    tsa[k] = x2[sv[k]];
+    tda[k] = v2[st[k]];
+  }
+  double src_b = 0.0, des_b = 0.0, src_a = 0.0, des_a = 0.0;
+#pragma unroll
+  for (int k = 0; k < N1 + N2; k++) {
+    src_b += tsb[k];
+    des_b += tdb[k];
+    src_a += tsa[k];
+    des_a += tda[k];
+  }
+  const double e_before = src_b + des_b;
+  const double e_after = src_a + des_a;
+  const double ea = e0[X] + (e_after - e_before) / 2;
+  return nu * exp_glibc(-ea / kT, exp_tab);
+}
+
 // Evaluates the rates of the vacancies idx[0..n) (sorted indices; idx ==
 // nullptr means base + i), as the reference's calcRates evaluates a vacancy:
 // 8 direction rates, plus the side effect of deltaE on the vacancy's own
@@ -401,8 +484,9 @@ __device__ double tail_update(SectorLDS<VCAP, RateT>& S, int v0, int n, double P
 // visible yet).  pf/pf_vi: this lane's stencil byte (site index = lane) of
 // vacancy pf_vi, gathered earlier by the caller (pf_vi < 0: none).
 template <int VCAP, typename RateT, bool FP32>
-__device__ void eval_list(SectorLDS<VCAP, RateT>& S, const SharedTables& T, const LatticeView& L, const int32_t* idx,
-                          int base, int n, const Patch& pt, bool own_from_lds, uint8_t pf, int pf_vi) {
+__device__ void eval_list(SectorLDS<VCAP, RateT>& S, const SharedTables& T, const LaneStencil& LS, const LatticeView& L,
+                          const int32_t* idx, int base, int n, const Patch& pt, bool own_from_lds, uint8_t pf,
+                          int pf_vi) {
   const int l = lane_id();
   const int ns = T.st.n_sites;
   for (int bs = 0; bs < n; bs += EVB) {
@@ -448,12 +532,7 @@ __device__ void eval_list(SectorLDS<VCAP, RateT>& S, const SharedTables& T, cons
     int X = SP_V;
     if (act) {
       const int par = (int)(key_x(S.key[vi]) & 1);
-      X = S.sten[a][T.st.w1[par][b]];
-      if constexpr (FP32) {
-        r = 0.0;  // the FP32 rate is the batched engine's (akmc_batch.hip)
-      } else {
-        r = rate_fp64(S.sten[a], par, b, T.st, T.e1, T.e2, T.e0, L.nu, L.kT, T.exp_tab);
-      }
+      r = rate_fp64_staged(S.sten[a], par, LS, T.e1, T.e2, T.e0, L.nu, L.kT, T.exp_tab, X);
       S.rate[vi * 8 + b] = (RateT)r;
       S.tieb[a][b] = (int8_t)tie_index((double)(RateT)r);
     }
