@@ -148,9 +148,10 @@ __global__ __launch_bounds__(64) void plain_kernel(PlainArgs a) {
   __shared__ SharedTables T;
   const int l = lane_id();
   stage_tables(T);
-  const LaneStencil LS = lane_stencil(T.st);
+  LaneStencil LS = lane_stencil(T.st);
+  lane_linear(LS, T.st, a.dims);
   const LatticeView L{a.sites, a.vac_key, a.nvac, a.dims, a.nu, a.kT};
-  const Patch nopatch{0xFFFFFFFFu, 0xFFFFFFFFu, 0, 0, -1};
+  const Patch nopatch{0xFFFFFFFFu, 0xFFFFFFFFu, 0, 0, -1, ~0ull, ~0ull};
   __shared__ double evo[8];  // sector clocks (indexed by the running sector id)
   PlainState st = *a.state;  // every lane keeps an identical copy
   if (l < 8) evo[l] = a.state->evo[l];
@@ -260,10 +261,8 @@ __global__ __launch_bounds__(64) void plain_kernel(PlainArgs a) {
     // latency has been hidden behind the list update.
     uint8_t pf = 0;
     {
-      const int pt_par = par ^ 1;
-      if (l < T.st.n_sites)
-        pf = L.sites[key_index(L.d, key_add(L.d, to, T.st.off[pt_par][l][0], T.st.off[pt_par][l][1],
-                                             T.st.off[pt_par][l][2]))];
+      const uint64_t q = stencil_site(L.d, T.st, LS, (uint32_t)uni_i((int)to));
+      if (l < T.st.n_sites) pf = L.sites[q];
     }
     const uint8_t ff = S.own[vsel];
     const uint8_t gt = S.nb1[vsel * 8 + bsel];
@@ -337,7 +336,7 @@ __global__ __launch_bounds__(64) void plain_kernel(PlainArgs a) {
       }
     }
     int amin = 0x7FFFFFFF, amax = -1;
-    const Patch pt{from, to, nf_from, nf_to, stays ? knew : -1};
+    const Patch pt{from, to, nf_from, nf_to, stays ? knew : -1, key_index(L.d, from), key_index(L.d, to)};
     for (int b0 = ilo; b0 < ihi; b0 += 64) {
       const int v = b0 + l;
       bool hit = false;

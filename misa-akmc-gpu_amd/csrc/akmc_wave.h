@@ -50,6 +50,17 @@ __device__ __forceinline__ double bcast_d(double v, int lane) {
   return as_f64(((uint64_t)hi << 32) | lo);
 }
 
+// A value every lane holds identically, moved to scalar registers so that
+// the compiler treats it (and branches on it) as wave-uniform.  Only for
+// values computed the same way in all active lanes.
+__device__ __forceinline__ int uni_i(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ double uni_d(double x) {
+  const uint64_t b = as_u64(x);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b >> 32));
+  return as_f64(((uint64_t)hi << 32) | lo);
+}
+
 template <int CTRL, int ROW_MASK>
 __device__ __forceinline__ double dpp_d(double v) {
   const uint64_t b = as_u64(v);
@@ -149,6 +160,7 @@ struct Patch {
   uint32_t k0, k1;  // keys whose global bytes may be stale (the last hop)
   uint8_t f0, f1;
   int32_t moved;    // sorted index of the vacancy that just hopped (own = f1), or -1
+  uint64_t i0, i1;  // site indices of k0 and k1
 };
 
 // ---------------------------------------------------------------- searches
@@ -188,7 +200,9 @@ __device__ __forceinline__ int first_above(const double* pvac, int n, double tar
 template <int VCAP, typename RateT>
 __device__ double fold_range(SectorLDS<VCAP, RateT>& S, int v0, int v1, double P) {
   const int l = lane_id();
-  int v = v0;
+  int v = uni_i(v0);
+  v1 = uni_i(v1);
+  P = uni_d(P);
   while (v < v1) {
     if (v1 - v <= 2 || !(P >= 0x1p-969)) {
       // Short remainder or zero/tiny running sum: one vacancy serially.
@@ -199,6 +213,7 @@ __device__ double fold_range(SectorLDS<VCAP, RateT>& S, int v0, int v1, double P
       double p = P;
 #pragma unroll
       for (int b = 0; b < 8; b++) p += a[b];
+      p = uni_d(p);
       if (l == 0) {
         S.pvac[v] = p;
         S.risk[v] = fold_risk(P, p, tm);
@@ -261,6 +276,7 @@ __device__ double fold_range(SectorLDS<VCAP, RateT>& S, int v0, int v1, double P
       const double p0 = p;
 #pragma unroll
       for (int b = 0; b < 8; b++) p += ab[b];
+      p = uni_d(p);
       if (l == 0) {
         S.pvac[vb] = p;
         S.risk[vb] = fold_risk(p0, p, tm);
@@ -290,9 +306,10 @@ __device__ double fold_range(SectorLDS<VCAP, RateT>& S, int v0, int v1, double P
 template <int VCAP, typename RateT>
 __device__ double tail_update(SectorLDS<VCAP, RateT>& S, int v0, int n, double Pn, double Po) {
   const int l = lane_id();
-  int v = v0;
-  double Sold = Po;   // old running sum before v
-  double Pprev = Pn;  // new running sum before v
+  int v = uni_i(v0);
+  n = uni_i(n);
+  double Sold = uni_d(Po);   // old running sum before v
+  double Pprev = uni_d(Pn);  // new running sum before v
   bool capped = false;
   while (v < n) {
     const double D = Pprev - Sold;
@@ -336,13 +353,13 @@ __device__ double tail_update(SectorLDS<VCAP, RateT>& S, int v0, int n, double P
         }
       }
       wave_sync();
-      if (stop >= n) return S.pvac[n - 1];
+      if (stop >= n) return uni_d(S.pvac[n - 1]);
     }
     // Serial exact fold from `stop` until a folded vacancy's old and new sums
     // share a binade again (normally one to three vacancies next to a power
     // of two, or the risky vacancy alone).  Every lane folds the same four
     // vacancies; lane j writes vacancy w + j.
-    double p = stop == v ? Pprev : S.pvac[stop - 1];
+    double p = uni_d(stop == v ? Pprev : S.pvac[stop - 1]);
     double old_w = 0.0;
     int w = stop;
     bool same = false;
@@ -367,7 +384,8 @@ __device__ double tail_update(SectorLDS<VCAP, RateT>& S, int v0, int n, double P
           const double p0 = p;
 #pragma unroll
           for (int b = 0; b < 8; b++) p += r4[j][b];
-          old_w = o4[j];
+          p = uni_d(p);
+          old_w = uni_d(o4[j]);
           same = (exponent_of(old_w) == exponent_of(p)) & (old_w >= 0x1p-969);
           if (l == j) { pw = p; pw0 = p0; tw = t4[j]; }
           k = j + 1;
@@ -400,7 +418,34 @@ struct LaneStencil {
   uint32_t v[2][4];
   uint32_t t[2][4];
   uint32_t x;  // byte par: stencil index of t_b
+  int32_t lin[2];  // by parity: linear site-index offset of stencil site l (no wrap)
+  int32_t off[2];  // by parity: (dx, dy, dz) of stencil site l, one signed byte each
 };
+
+// Linear offsets of this lane's stencil site (site l, clamped to the
+// stencil) for a box: dx + X*(dy + Y*dz), valid when no coordinate wraps.
+__device__ __forceinline__ void lane_linear(LaneStencil& R, const StencilTables& st, const BoxDims& d) {
+  const int lk = min(lane_id(), st.n_sites - 1);
+  for (int par = 0; par < 2; par++) {
+    const int dx = st.off[par][lk][0], dy = st.off[par][lk][1], dz = st.off[par][lk][2];
+    R.lin[par] = dx + d.X * (dy + d.Y * dz);
+    R.off[par] = (int32_t)(((uint32_t)(uint8_t)dx) | ((uint32_t)(uint8_t)dy << 8) | ((uint32_t)(uint8_t)dz << 16));
+  }
+}
+
+// Site index of this lane's stencil site around the vacancy at key w (a
+// wave-uniform key): the vacancy's index plus the linear offset when the
+// 7 x 5 x 5 stencil box stays inside the lattice, else wrapped per
+// coordinate as key_add does.
+__device__ __forceinline__ uint64_t stencil_site(const BoxDims& d, const StencilTables& st, const LaneStencil& LS,
+                                                 uint32_t w) {
+  const int x = (int)key_x(w), y = (int)key_y(w), z = (int)key_z(w);
+  const int par = x & 1;
+  const bool inside = (x >= 3) & (x < d.X - 3) & (y >= 2) & (y < d.Y - 2) & (z >= 2) & (z < d.Z - 2);
+  if (inside) return (uint64_t)((int64_t)key_index(d, w) + (par ? LS.lin[1] : LS.lin[0]));
+  const int32_t o = par ? LS.off[1] : LS.off[0];
+  return key_index(d, key_add(d, w, (int8_t)(o & 255), (int8_t)((o >> 8) & 255), (int8_t)((o >> 16) & 255)));
+}
 __device__ __forceinline__ LaneStencil lane_stencil(const StencilTables& st) {
   const int b = lane_id() & 7;
   LaneStencil R;
@@ -493,26 +538,31 @@ __device__ void eval_list(SectorLDS<VCAP, RateT>& S, const SharedTables& T, cons
     const int bc = min(EVB, n - bs);
     // Gather: load i is (vacancy i of the batch, stencil site = lane).
     uint32_t f[EVB];
-    uint32_t qk[EVB];
-    const int lk = min(l, ns - 1);
+    // Site indices: the vacancy's index plus this lane's linear stencil
+    // offset, unless the stencil wraps around the box (then per coordinate).
+    uint64_t qi[EVB];
 #pragma unroll
     for (int i = 0; i < EVB; i++) {
-      const int vv = idx ? idx[bs + min(i, bc - 1)] : base + bs + min(i, bc - 1);
-      const uint32_t w = S.key[vv];
-      const int par = (int)(key_x(w) & 1);
-      qk[i] = key_add(L.d, w, T.st.off[par][lk][0], T.st.off[par][lk][1], T.st.off[par][lk][2]);
+      qi[i] = 0;
+      if (i < bc) {
+        const int vv = uni_i(idx ? idx[bs + i] : base + bs + i);
+        const uint32_t w = (uint32_t)uni_i((int)S.key[vv]);
+        qi[i] = stencil_site(L.d, T.st, LS, w);
+      }
     }
     // Wave-uniform conditions only, so the loads issue back to back.
 #pragma unroll
     for (int i = 0; i < EVB; i++) {
-      const int vv = idx ? idx[bs + min(i, bc - 1)] : base + bs + min(i, bc - 1);
       f[i] = pf;
-      if (i < bc && vv != pf_vi) f[i] = L.sites[key_index(L.d, qk[i])];
+      if (i < bc) {
+        const int vv = uni_i(idx ? idx[bs + i] : base + bs + i);
+        if (vv != pf_vi) f[i] = L.sites[qi[i]];
+      }
     }
 #pragma unroll
     for (int i = 0; i < EVB; i++) {
-      if (qk[i] == pt.k0) f[i] = pt.f0;
-      if (qk[i] == pt.k1) f[i] = pt.f1;
+      if (qi[i] == pt.i0) f[i] = pt.f0;
+      if (qi[i] == pt.i1) f[i] = pt.f1;
     }
 #pragma unroll
     for (int i = 0; i < EVB; i++) {
