@@ -37,7 +37,9 @@ __constant__ double c_e2[25] = AKMC_E2_TABLE;
 __constant__ double c_e0[5] = AKMC_E0_TABLE;
 
 static constexpr int PLAIN_VCAP = 1536;
-static const int kRing[8] = {0, 7, 2, 5, 3, 4, 1, 6};  // SectorMeta ring, sublattice.h:21-30
+// SectorMeta ring 0,7,2,5,3,4,1,6 (sublattice.h:21-30), one nibble per
+// position: arithmetic instead of a table load in the event loop.
+__device__ __forceinline__ int ring_sector(int pos) { return (int)((0x61435270u >> (4 * pos)) & 7u); }
 
 using PlainSector = SectorLDS<PLAIN_VCAP, double>;
 
@@ -159,7 +161,7 @@ __global__ __launch_bounds__(64) void plain_kernel(PlainArgs a) {
   bool rebuild = true;
   double sum = 0.0;
   int nv = 0;
-  int s = kRing[st.sect];
+  int s = ring_sector(st.sect);
   int32_t status = AKMC_RUN_OK;
   int64_t visits = 0;  // sector visits this launch (bounded so a launch always returns)
   // Random numbers of the next event, loaded one event ahead.
@@ -176,20 +178,20 @@ __global__ __launch_bounds__(64) void plain_kernel(PlainArgs a) {
     st.thr = 1.0;
     st.events = 0;
     for (int i = 0; i < 8; i++)
-      if (kRing[i] == a.rates_sector) st.sect = i;
+      if (ring_sector(i) == a.rates_sector) st.sect = i;
   }
   for (;;) {
     if (!st.in_sector) {
       if (st.step >= a.time_steps) { status = AKMC_RUN_TIME_DONE; break; }
       if (++visits > (int64_t)1 << 16) { status = AKMC_RUN_YIELD; break; }
-      s = kRing[st.sect];
+      s = ring_sector(st.sect);
       // step_threshold_time (sublattice.inl:72)
       st.thr = static_cast<double>(st.step + 1) * a.sector_T - evo[s];
       st.t = 0.0;
       st.in_sector = 1;
       rebuild = true;
     }
-    s = kRing[st.sect];
+    s = ring_sector(st.sect);
     if (!(st.t < st.thr)) {
       if (l == 0) evo[s] += st.t;  // sublattice.inl:140
       wave_sync();
