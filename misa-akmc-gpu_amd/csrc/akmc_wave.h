@@ -326,27 +326,27 @@ __device__ double tail_update(SectorLDS<VCAP, RateT>& S, int v0, int n, double P
       const double top = __builtin_ldexp(1.0, e + 1);
       const double thr = D > 0.0 ? top - D : top;
       stop = n;
+      // Lane l takes the four consecutive vacancies c + 4l .. c + 4l + 3.
       for (int c = v; c < n; c += 256) {
+        const int i0 = c + 4 * l;
         double x[4];
         uint8_t rk[4];
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int ic = min(c + 64 * r + l, n - 1);
-          x[r] = S.pvac[ic];
-          rk[r] = S.risk[ic];
+        for (int k = 0; k < 4; k++) {
+          const int ic = min(i0 + k, n - 1);
+          x[k] = S.pvac[ic];
+          rk[k] = S.risk[ic];
         }
-        int f = n;
+        int kf = 4;  // first of this lane's four that ends the region
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int i = c + 64 * r + l;
-          const uint64_t m = ballot((i < n) & !((x[r] < thr) & (rk[r] == 0)));
-          if (f == n && m) f = c + 64 * r + first_set(m);
-        }
+        for (int k = 3; k >= 0; k--)
+          if ((i0 + k < n) & !((x[k] < thr) & (rk[k] == 0))) kf = k;
+        const uint64_t m = ballot(kf < 4);
+        const int fl = first_set(m);
+        const int f = fl == 64 ? n : c + 4 * fl + __builtin_amdgcn_readlane(kf, fl);
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int i = c + 64 * r + l;
-          if (i < f) S.pvac[i] = x[r] + D;
-        }
+        for (int k = 0; k < 4; k++)
+          if (i0 + k < f) S.pvac[i0 + k] = x[k] + D;
         if (f < n) {
           stop = f;
           break;
