@@ -357,40 +357,35 @@ __device__ double tail_update(SectorLDS<VCAP, RateT>& S, int v0, int n, double P
     }
     // Serial exact fold from `stop` until a folded vacancy's old and new sums
     // share a binade again (normally one to three vacancies next to a power
-    // of two, or the risky vacancy alone).  Every lane folds the same four
-    // vacancies; lane j writes vacancy w + j.
+    // of two, or the risky vacancy alone).  Four vacancies per round: lane
+    // 8j + b loads slot b of vacancy w + j and lane j its old sum and tie
+    // mask; the fold reads them through readlane, and lane j writes vacancy
+    // w + j.
     double p = uni_d(stop == v ? Pprev : S.pvac[stop - 1]);
     double old_w = 0.0;
     int w = stop;
     bool same = false;
     for (;;) {
-      double r4[4][8], o4[4];
-      uint64_t t4[4];
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const int wj = min(w + j, n - 1);
-        o4[j] = S.pvac[wj];
-        t4[j] = S.tmask[wj];
-#pragma unroll
-        for (int b = 0; b < 8; b++) r4[j][b] = (double)S.rate[8 * wj + b];
-      }
+      const double rl = (double)S.rate[8 * min(w + ((l >> 3) & 3), n - 1) + (l & 7)];
+      const int wo = min(w + (l & 3), n - 1);
+      const double ol = S.pvac[wo];
+      const uint64_t tl = S.tmask[wo];
       const int kmax = min(4, n - w);
       int k = 0;
       double pw = 0.0, pw0 = 0.0;
-      uint64_t tw = 0;
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         if (j < kmax && !same && w + j - stop < 64) {
           const double p0 = p;
 #pragma unroll
-          for (int b = 0; b < 8; b++) p += r4[j][b];
-          p = uni_d(p);
-          old_w = uni_d(o4[j]);
+          for (int b = 0; b < 8; b++) p += bcast_d(rl, 8 * j + b);
+          old_w = bcast_d(ol, j);
           same = (exponent_of(old_w) == exponent_of(p)) & (old_w >= 0x1p-969);
-          if (l == j) { pw = p; pw0 = p0; tw = t4[j]; }
+          if (l == j) { pw = p; pw0 = p0; }
           k = j + 1;
         }
       }
+      const uint64_t tw = tl;
       if (l < k) {
         S.pvac[w + l] = pw;
         S.risk[w + l] = fold_risk(pw0, pw, tw);
