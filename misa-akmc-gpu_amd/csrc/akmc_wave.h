@@ -195,17 +195,49 @@ __device__ __forceinline__ int first_above(const double* pvac, int n, double tar
 }
 
 // ---------------------------------------------------------------- fold
+// Serial fold of the cnt <= 8 vacancies v .. v + cnt - 1 from P, in the
+// reference order: lane 8j + b loads slot b of vacancy v + j (one LDS load),
+// the fold reads the slots through readlane, and lane j writes vacancy
+// v + j's running sum and tie risk.  Returns the sum after the last.
+template <int VCAP, typename RateT>
+__device__ __forceinline__ double fold_serial(SectorLDS<VCAP, RateT>& S, int v, int cnt, double P) {
+  const int l = lane_id();
+  const int last = v + cnt - 1;
+  const double rl = (double)S.rate[8 * min(v + (l >> 3), last) + (l & 7)];
+  const uint64_t tl = S.tmask[min(v + (l & 7), last)];
+  double pw = 0.0, pw0 = 0.0;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    if (j < cnt) {
+      const double p0 = P;
+#pragma unroll
+      for (int b = 0; b < 8; b++) P += bcast_d(rl, 8 * j + b);
+      if (l == j) { pw = P; pw0 = p0; }
+    }
+  }
+  if (l < cnt) {
+    S.pvac[v + l] = pw;
+    S.risk[v + l] = fold_risk(pw0, pw, tl);
+  }
+  return uni_d(P);
+}
+
 // Exact running sums for vacancies [v0, v1) starting from P (the sum before
 // v0).  Writes pvac and returns the sum after v1 - 1.
 template <int VCAP, typename RateT>
-__device__ double fold_range(SectorLDS<VCAP, RateT>& S, int v0, int v1, double P) {
+__device__ __forceinline__ double fold_range(SectorLDS<VCAP, RateT>& S, int v0, int v1, double P) {
   const int l = lane_id();
   int v = uni_i(v0);
   v1 = uni_i(v1);
   P = uni_d(P);
   while (v < v1) {
-    if (v1 - v <= 2 || !(P >= 0x1p-969)) {
-      // Short remainder or zero/tiny running sum: one vacancy serially.
+    if (v1 - v <= 6) {
+      // A few vacancies: serially (cheaper than a scan pass).
+      P = fold_serial<VCAP, RateT>(S, v, v1 - v, P);
+      break;
+    }
+    if (!(P >= 0x1p-969)) {
+      // Zero or tiny running sum: one vacancy serially.
       double a[8];
 #pragma unroll
       for (int b = 0; b < 8; b++) a[b] = (double)S.rate[8 * v + b];
@@ -304,7 +336,7 @@ __device__ double fold_range(SectorLDS<VCAP, RateT>& S, int v0, int v1, double P
 // round trip, until old and new sums share a binade again, and the next
 // region starts with the new exact difference.
 template <int VCAP, typename RateT>
-__device__ double tail_update(SectorLDS<VCAP, RateT>& S, int v0, int n, double Pn, double Po) {
+__device__ __forceinline__ double tail_update(SectorLDS<VCAP, RateT>& S, int v0, int n, double Pn, double Po) {
   const int l = lane_id();
   int v = uni_i(v0);
   n = uni_i(n);
