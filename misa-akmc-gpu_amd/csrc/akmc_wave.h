@@ -108,13 +108,17 @@ __device__ __forceinline__ bool tie_free(uint64_t mask, int e) {
   return in_window & !((mask >> (t & 63)) & 1);
 }
 
-// Tie risk of a vacancy whose running sum went from So to Eo over its 8
-// slots: set unless both lie in one normal binade e and no rate can tie
-// there, i.e. unless a later exact shift inside that binade reproduces the
-// serial fold.
+// Shift risk of a vacancy whose running sum went from So to Eo over its 8
+// slots.  RISK_HARD: the sums do not lie in one normal binade e.  RISK_TIE:
+// some rate is an odd multiple of half the binade's ulp u, so a slot rounds
+// a tie; a shift by an even multiple of u keeps the parity of every partial
+// sum and with it the tie's rounding, an odd one does not.  A later exact
+// shift inside the binade reproduces the serial fold unless a flag applies.
+enum : uint8_t { RISK_TIE = 1, RISK_HARD = 2 };
 __device__ __forceinline__ uint8_t fold_risk(double So, double Eo, uint64_t tm) {
   const int e = exponent_of(So);
-  return (uint8_t)!((So >= 0x1p-969) & (exponent_of(Eo) == e) & tie_free(tm, e));
+  const bool hard = !((So >= 0x1p-969) & (exponent_of(Eo) == e));
+  return (uint8_t)(hard ? RISK_HARD : (tie_free(tm, e) ? 0 : RISK_TIE));
 }
 
 // ---------------------------------------------------------------- state
@@ -127,7 +131,7 @@ struct SectorLDS {
   RateT rate[VCAP * 8];
   double pvac[VCAP];       // running sum after each vacancy
   uint64_t tmask[VCAP];    // tie mask of each vacancy's rates
-  uint8_t risk[VCAP];      // 1 if shifting this vacancy's sums needs a serial fold (fold_risk)
+  uint8_t risk[VCAP];      // RISK_* flags of shifting this vacancy's sums (fold_risk)
   uint8_t own[VCAP];       // flag byte of the vacancy site
   uint8_t nb1[VCAP * 8];   // flag bytes of its 8 1nn sites
   uint8_t sten[EVB][STENCIL_MAX];
@@ -288,7 +292,7 @@ __device__ __forceinline__ double fold_range(SectorLDS<VCAP, RateT>& S, int v0, 
     if (active & (l < fb)) {
       // All of this vacancy's sums lie in P's binade e: only ties matter.
       S.pvac[vi] = P + incl * u;
-      S.risk[vi] = (uint8_t)!tie_free(S.tmask[vi], e);
+      S.risk[vi] = tie_free(S.tmask[vi], e) ? (uint8_t)0 : (uint8_t)RISK_TIE;
     }
     if (fb == 64) {
       const int last = min(63, v1 - 1 - v);
@@ -354,9 +358,11 @@ __device__ __forceinline__ double tail_update(SectorLDS<VCAP, RateT>& S, int v0,
     if (!start_ok && capped) return fold_range<VCAP, RateT>(S, v, n, Pprev);
     int stop = v;
     if (start_ok) {
-      // D < 2^e here, so top - D is exact.
+      // D < 2^e here, so top - D is exact, and D / u is an integer below 2^52.
       const double top = __builtin_ldexp(1.0, e + 1);
       const double thr = D > 0.0 ? top - D : top;
+      const double half = D * __builtin_ldexp(1.0, 51 - e);  // (D / u) / 2
+      const uint8_t rmask = half != __builtin_rint(half) ? (uint8_t)(RISK_TIE | RISK_HARD) : (uint8_t)RISK_HARD;
       stop = n;
       // Lane l takes the four consecutive vacancies c + 4l .. c + 4l + 3.
       for (int c = v; c < n; c += 256) {
@@ -372,7 +378,7 @@ __device__ __forceinline__ double tail_update(SectorLDS<VCAP, RateT>& S, int v0,
         int kf = 4;  // first of this lane's four that ends the region
 #pragma unroll
         for (int k = 3; k >= 0; k--)
-          if ((i0 + k < n) & !((x[k] < thr) & (rk[k] == 0))) kf = k;
+          if ((i0 + k < n) & !((x[k] < thr) & ((rk[k] & rmask) == 0))) kf = k;
         const uint64_t m = ballot(kf < 4);
         const int fl = first_set(m);
         const int f = fl == 64 ? n : c + 4 * fl + __builtin_amdgcn_readlane(kf, fl);
