@@ -310,6 +310,47 @@ void akmc_batch_destroy(akmc_batch* b);
 int akmc_cu_clusters(const uint8_t* sites, const int64_t box[3], int64_t hist_cap, int64_t* hist,
                      int64_t* n_clusters, int64_t* n_cu, int64_t* n_isolated);
 
+/* ---------------------------------------------------------------- outputs
+ * The reference's step hook (MEventHook::onStepFinished,
+ * profiles/m_event_hook.cpp:17-70) logs the isolated-Cu count every 100 steps
+ * (akmc_cu_clusters) and dumps the lattice every output.dump.interval steps.
+ * akmc_run_steps stops the time loop at a step boundary so a caller can run
+ * those hooks at the same points. */
+#define AKMC_STOP_STEPS 3   /* akmc_run_steps reached its step count */
+
+typedef struct akmc_output {
+  int64_t dump_interval;     /* output.dump.interval (0: no dumps) */
+  int64_t logs_interval;     /* output.logs.interval */
+  char dump_file_path[256];  /* output.dump.file_path, "{}" = the step */
+  char logs_file[256];       /* output.logs.logs_file */
+} akmc_output;
+
+/* The optional `output` section of config.yaml (ConfigParsing::parseOutput,
+ * profiles/config/config_parsing.cpp:216-229); all zero when absent. */
+int akmc_output_from_yaml(const char* path, akmc_output* o);
+
+/* fmt::format(dump_file_path, step) for patterns whose replacement fields are
+ * all "{}" (m_event_hook.cpp:62). */
+int akmc_dump_path(const char* pattern, int64_t step, char* out, int64_t cap);
+
+/* Writes a lattice dump in the reference's format (LatticeDump::dump,
+ * profiles/io/lattice_dump.cpp:14-63): the site count, an empty line, then
+ * "Type\tx\ty\tz" per site in z, y, x order, coordinates of the
+ * ghost-extended grid (ghost width `ghost` = ceil(cutoff / a0)), body sites at
+ * +0.5.  Type is the readout (stale tags are not written). */
+int akmc_write_dump(const uint8_t* sites, const int64_t box[3], int64_t ghost, const char* path);
+
+/* Reads a dump as the reference's XYZ reader does (XYZReader,
+ * tools/xyz_reader.cpp:15-80; particle, tools/particle.h): the box from the
+ * coordinate ranges, the ghost width from the smallest x.  sites == NULL
+ * returns the box only; otherwise cap must hold 2*box[0]*box[1]*box[2]. */
+int akmc_read_dump(const char* path, int64_t box[3], int64_t* ghost, uint8_t* sites, int64_t cap);
+
+/* Runs the event loop like akmc_run, but ends it when `steps` more time steps
+ * are complete (stop = AKMC_STOP_STEPS), or earlier at the end of the time
+ * loop or after max_events events. */
+int akmc_run_steps(akmc_engine* e, int64_t steps, int64_t max_events, akmc_stats* stats);
+
 /* The FP32 rate table the replica engine uses at temperature T (K) and
  * attempt frequency nu: the rate of barrier index m (Ea = m * q / 2 eV) is
  * tab[256 + ((m - m_min) >> 8)] * tab[(m - m_min) & 255] in FP32.  Writes

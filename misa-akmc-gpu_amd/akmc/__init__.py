@@ -17,6 +17,7 @@ AKMC_PREC_FP64 = 0
 AKMC_PREC_FP32 = 1
 AKMC_STOP_TIME = 1
 AKMC_STOP_EVENTS = 2
+AKMC_STOP_STEPS = 3
 MAX_ALLOY = 32
 
 # LatticeTypes::lat_type values (reference src/lattice/lattice_types.h:20-36).
@@ -64,6 +65,11 @@ class Stats(ctypes.Structure):
                 "n_shift": self.n_shift, "sum_errors": self.sum_errors}
 
 
+class Output(ctypes.Structure):
+    _fields_ = [("dump_interval", ctypes.c_int64), ("logs_interval", ctypes.c_int64),
+                ("dump_file_path", ctypes.c_char * 256), ("logs_file", ctypes.c_char * 256)]
+
+
 _lib = None
 
 
@@ -94,6 +100,11 @@ def lib():
     L.akmc_get_event_log.argtypes = [vp, vp, vp, i64, P(i64)]
     L.akmc_set_rng_chunk.argtypes = [vp, i64]
     L.akmc_set_check_sums.argtypes = [vp, ctypes.c_int]
+    L.akmc_run_steps.argtypes = [vp, i64, i64, P(Stats)]
+    L.akmc_output_from_yaml.argtypes = [ctypes.c_char_p, P(Output)]
+    L.akmc_dump_path.argtypes = [ctypes.c_char_p, i64, ctypes.c_char_p, i64]
+    L.akmc_write_dump.argtypes = [vp, vp, i64, ctypes.c_char_p]
+    L.akmc_read_dump.argtypes = [ctypes.c_char_p, vp, P(i64), vp, i64]
     L.akmc_rng_streams.argtypes = [i64, i64, i64, vp, vp]
     L.akmc_sync.argtypes = [i32]
     L.akmc_copy_bench.argtypes = [i32, i64, i32, P(dbl)]
@@ -227,6 +238,13 @@ class Engine:
     def run(self, max_events):
         st = Stats()
         _check(lib().akmc_run(self.h, int(max_events), ctypes.byref(st)))
+        return st.as_dict()
+
+    def run_steps(self, steps, max_events=1 << 62):
+        """Runs until `steps` more time steps are complete (stop AKMC_STOP_STEPS),
+        the time loop ends, or max_events events."""
+        st = Stats()
+        _check(lib().akmc_run_steps(self.h, int(steps), int(max_events), ctypes.byref(st)))
         return st.as_dict()
 
     def stats(self):
@@ -400,6 +418,70 @@ def cu_clusters(sites, box, hist_cap=64):
     _check(_blib().akmc_cu_clusters(sites.ctypes.data, b.ctypes.data, hist_cap, hist.ctypes.data, ctypes.byref(nc),
                                     ctypes.byref(ncu), ctypes.byref(niso)))
     return {"hist": hist, "clusters": nc.value, "cu": ncu.value, "isolated": niso.value}
+
+
+def output_from_yaml(path):
+    """The `output` section of a config.yaml (ConfigParsing::parseOutput)."""
+    o = Output()
+    _check(lib().akmc_output_from_yaml(str(path).encode(), ctypes.byref(o)))
+    return {"dump_interval": o.dump_interval, "logs_interval": o.logs_interval,
+            "dump_file_path": o.dump_file_path.decode(), "logs_file": o.logs_file.decode()}
+
+
+def dump_path(pattern, step):
+    """fmt::format(dump_file_path, step) for "{}" fields."""
+    out = ctypes.create_string_buffer(4096)
+    _check(lib().akmc_dump_path(str(pattern).encode(), int(step), out, 4096))
+    return out.value.decode()
+
+
+def write_dump(sites, box, ghost, path):
+    """Writes a lattice dump in the reference's format (LatticeDump::dump)."""
+    sites = np.ascontiguousarray(sites, dtype=np.uint8)
+    b = np.array(box, dtype=np.int64)
+    if sites.size != 2 * int(b[0]) * int(b[1]) * int(b[2]):
+        raise ValueError("sites do not match the box")
+    _check(lib().akmc_write_dump(sites.ctypes.data, b.ctypes.data, int(ghost), str(path).encode()))
+
+
+def read_dump(path):
+    """Reads a lattice dump as the reference's XYZ reader: (sites, box, ghost)."""
+    b = np.zeros(3, dtype=np.int64)
+    g = ctypes.c_int64()
+    _check(lib().akmc_read_dump(str(path).encode(), b.ctypes.data, ctypes.byref(g), None, 0))
+    sites = np.zeros(2 * int(b[0]) * int(b[1]) * int(b[2]), dtype=np.uint8)
+    _check(lib().akmc_read_dump(str(path).encode(), b.ctypes.data, ctypes.byref(g), sites.ctypes.data, sites.size))
+    return sites, tuple(int(x) for x in b), g.value
+
+
+def run_with_hooks(engine, box, ghost, steps, output, log=print, max_events=1 << 62):
+    """The reference's step hook (MEventHook::onStepFinished,
+    profiles/m_event_hook.cpp:17-70) around `steps` time steps: after every
+    step that is a multiple of 100, the isolated-Cu count is logged; after
+    every multiple of output["dump_interval"] (when non-zero) the lattice is
+    dumped to dump_path(output["dump_file_path"], step).  The engine runs
+    between hook points without stopping.  Returns the last run's stats."""
+    # onStepFinished(step) runs after time step `step` (counted from 0,
+    # sublattice.inl:61,178) completes, i.e. when the engine's step count
+    # reaches step + 1.
+    interval = int(output.get("dump_interval", 0))
+    st = engine.stats()
+    end = st["step"] + int(steps)
+    while st["step"] < end:
+        s = st["step"]
+        hook = -(-s // 100) * 100  # next completed-step index that is a multiple of 100
+        if interval:
+            hook = min(hook, -(-s // interval) * interval)
+        target = min(end, hook + 1)
+        st = engine.run_steps(target - s, max_events)
+        if st["step"] < target:
+            break  # the time loop or the event cap ended first
+        done = st["step"] - 1
+        if done % 100 == 0:
+            log(" step %d finished. now single_cu is %d." % (done, cu_clusters(engine.download(), box)["isolated"]))
+        if interval and done % interval == 0:
+            write_dump(engine.download(), box, ghost, dump_path(output["dump_file_path"], done))
+    return st
 
 
 def batch_options(**kw):

@@ -147,6 +147,7 @@ struct akmc_engine {
   int64_t log_base = 0;
   bool lattice_ready = false;
   int32_t check_sums = 0;
+  int64_t step_limit = INT64_MAX;  // akmc_run_steps: the time loop ends at this step
   akmc_stats last{};
 };
 
@@ -213,7 +214,7 @@ akmc::PlainArgs base_args(akmc_engine* e) {
   a.nvac = e->nvac;
   a.dims = e->dims;
   a.state = e->d_state;
-  a.time_steps = e->time_steps;
+  a.time_steps = std::min(e->time_steps, e->step_limit);
   a.sector_T = e->p.sector_time;
   a.nu = e->p.attempt_freq;
   a.kT = e->kT;
@@ -573,6 +574,19 @@ int akmc_set_rng_chunk(akmc_engine* e, int64_t events) {
     return fail(AKMC_ERR_INVALID, "chunk must be in [1, %lld]", (long long)kChunkMax);
   if (e->generated != 0) return fail(AKMC_ERR_INVALID, "the chunk size can only be set before the first run");
   e->chunk = events;
+  return AKMC_OK;
+}
+
+int akmc_run_steps(akmc_engine* e, int64_t steps, int64_t max_events, akmc_stats* stats) {
+  if (!e) return fail(AKMC_ERR_INVALID, "null engine");
+  if (steps < 0) return fail(AKMC_ERR_INVALID, "steps < 0");
+  const int64_t step0 = e->h_state.step;
+  e->step_limit = steps > INT64_MAX - step0 ? INT64_MAX : step0 + steps;
+  const int rc = akmc_run(e, max_events, nullptr);
+  e->step_limit = INT64_MAX;
+  if (rc != AKMC_OK) return rc;
+  if (e->last.stop == AKMC_STOP_TIME && e->h_state.step < e->time_steps) e->last.stop = AKMC_STOP_STEPS;
+  if (stats) *stats = e->last;
   return AKMC_OK;
 }
 

@@ -167,6 +167,51 @@ def test_running_sums_self_check():
     eng.close()
 
 
+def test_run_steps_stops_at_step_boundaries():
+    """akmc_run_steps ends the time loop after whole steps: the state equals
+    the oracle's at the end of a time loop of that many steps (physics time
+    (k - 0.5) T), bit for bit, and further runs continue the same trajectory."""
+    say("test_run_steps_stops_at_step_boundaries")
+    T = 3e-7
+    cfg = make_cfg(n=16, va=40, T=600.0, sector_T=T, physics=1.0)
+    eng = akmc.Engine(cfg)
+    done = 0
+    for k in (3, 4, 1):
+        st = eng.run_steps(k)
+        done += k
+        assert st["stop"] == akmc.AKMC_STOP_STEPS and st["step"] == done and st["in_sector"] == 0
+        ref = refcpu.RefCpu(make_cfg(n=16, va=40, T=600.0, sector_T=T, physics=(done - 0.5) * T))
+        ref.run(10 ** 9)
+        rs = ref.state()
+        assert rs["step"] == done and st["events"] == rs["events"]
+        assert np.array_equal(eng.download(), ref.sites())
+        assert np.array_equal(_bits(st["evolution"]), _bits(rs["evolution"]))
+    eng.close()
+
+
+def test_step_hooks_dump_and_log(tmp_path):
+    """The reference's step hook around a run: isolated-Cu log lines after
+    steps 0, 100, 200 and dumps every 100 steps, each dump equal to the
+    lattice at that step (readout)."""
+    say("test_step_hooks_dump_and_log")
+    cfg = make_cfg(n=12, va=8, T=650.0, sector_T=3e-8, physics=1.0)
+    eng = akmc.Engine(cfg)
+    logs = []
+    out = {"dump_interval": 100, "dump_file_path": str(tmp_path / "kmc.{}.out")}
+    st = akmc.run_with_hooks(eng, (12, 12, 12), 2, 201, out, log=logs.append)
+    assert st["step"] == 201
+    assert [l.split()[1] for l in logs] == ["0", "100", "200"]
+    assert sorted(os.listdir(tmp_path)) == ["kmc.0.out", "kmc.100.out", "kmc.200.out"]
+    sites, box, ghost = akmc.read_dump(tmp_path / "kmc.200.out")
+    cur = eng.download()
+    rb = np.zeros_like(cur)
+    for bit in (8, 4, 2, 1):
+        rb[(cur & bit) != 0] = bit
+    assert box == (12, 12, 12) and ghost == 2 and np.array_equal(sites, rb)
+    assert int(logs[-1].split()[-1].rstrip(".")) == akmc.cu_clusters(cur, box)["isolated"]
+    eng.close()
+
+
 def test_time_loop_end_and_empty_box():
     say("test_time_loop_end_and_empty_box")
     cfg = make_cfg(n=12, va=0, physics=3e-5, sector_T=3e-7)
