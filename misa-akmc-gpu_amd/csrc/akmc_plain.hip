@@ -209,7 +209,7 @@ __global__ __launch_bounds__(64) void plain_kernel(PlainArgs a) {
       __threadfence_block();
       nv = collect_sector(S, L, s);
       if (nv < 0) { status = AKMC_RUN_ERR_CAPACITY; break; }
-      eval_list<PLAIN_VCAP, double, false>(S, T, LS, L, nullptr, 0, nv, nopatch, false, 0, -1);
+      eval_list<PLAIN_VCAP, double>(S, T, LS, L, nullptr, 0, nv, nopatch, false, 0, -1);
       st.n_eval += nv;
       st.n_fold += nv;
       sum = fold_range<PLAIN_VCAP, double>(S, 0, nv, 0.0);
@@ -353,7 +353,7 @@ __global__ __launch_bounds__(64) void plain_kernel(PlainArgs a) {
         amin = min(amin, b0 + first_set(m));
         amax = max(amax, b0 + 63 - (int)__clzll(m));
         wave_sync();
-        eval_list<PLAIN_VCAP, double, false>(S, T, LS, L, S.list, 0, __popcll(m), pt, true, pf, stays ? knew : -1);
+        eval_list<PLAIN_VCAP, double>(S, T, LS, L, S.list, 0, __popcll(m), pt, true, pf, stays ? knew : -1);
         st.n_eval += __popcll(m);
       }
     }

@@ -561,7 +561,7 @@ __device__ __forceinline__ double rate_fp64_staged(const uint8_t* sp, int par, c
 // replaced by the patch values (the hop that was just written may not be
 // visible yet).  pf/pf_vi: this lane's stencil byte (site index = lane) of
 // vacancy pf_vi, gathered earlier by the caller (pf_vi < 0: none).
-template <int VCAP, typename RateT, bool FP32>
+template <int VCAP, typename RateT>
 __device__ void eval_list(SectorLDS<VCAP, RateT>& S, const SharedTables& T, const LaneStencil& LS, const LatticeView& L,
                           const int32_t* idx, int base, int n, const Patch& pt, bool own_from_lds, uint8_t pf,
                           int pf_vi) {
