@@ -351,6 +351,14 @@ int akmc_read_dump(const char* path, int64_t box[3], int64_t* ghost, uint8_t* si
  * loop or after max_events events. */
 int akmc_run_steps(akmc_engine* e, int64_t steps, int64_t max_events, akmc_stats* stats);
 
+/* Checkpoint and restart of a single-lattice engine (the reference's restart
+ * hook is empty, profiles/pkmc.cpp:183-184): the parameters, the sites with
+ * their stale tags, the sector-loop state (clocks, step, sector in progress,
+ * event count) and the random stream positions.  A loaded engine continues
+ * the same trajectory bit for bit, also from a stop inside a sector. */
+int akmc_checkpoint_save(akmc_engine* e, const char* path);
+int akmc_checkpoint_load(const char* path, int device, akmc_engine** out);
+
 /* The FP32 rate table the replica engine uses at temperature T (K) and
  * attempt frequency nu: the rate of barrier index m (Ea = m * q / 2 eV) is
  * tab[256 + ((m - m_min) >> 8)] * tab[(m - m_min) & 255] in FP32.  Writes

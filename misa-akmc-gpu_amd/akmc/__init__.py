@@ -101,6 +101,8 @@ def lib():
     L.akmc_set_rng_chunk.argtypes = [vp, i64]
     L.akmc_set_check_sums.argtypes = [vp, ctypes.c_int]
     L.akmc_run_steps.argtypes = [vp, i64, i64, P(Stats)]
+    L.akmc_checkpoint_save.argtypes = [vp, ctypes.c_char_p]
+    L.akmc_checkpoint_load.argtypes = [ctypes.c_char_p, i32, P(vp)]
     L.akmc_output_from_yaml.argtypes = [ctypes.c_char_p, P(Output)]
     L.akmc_dump_path.argtypes = [ctypes.c_char_p, i64, ctypes.c_char_p, i64]
     L.akmc_write_dump.argtypes = [vp, vp, i64, ctypes.c_char_p]
@@ -217,6 +219,22 @@ class Engine:
         self.n_sites = L.akmc_num_sites(self.h)
         if init_random:
             _check(L.akmc_init_random(self.h))
+
+    @classmethod
+    def load(cls, path, device=0):
+        """An engine restored from a checkpoint (akmc_checkpoint_load)."""
+        L = lib()
+        h = ctypes.c_void_p()
+        _check(L.akmc_checkpoint_load(str(path).encode(), int(device), ctypes.byref(h)))
+        self = cls.__new__(cls)
+        self._p = None
+        self.h = h
+        self.n_sites = L.akmc_num_sites(self.h)
+        return self
+
+    def save(self, path):
+        """Writes a checkpoint: a loaded engine continues this trajectory bit for bit."""
+        _check(lib().akmc_checkpoint_save(self.h, str(path).encode()))
 
     def close(self):
         if getattr(self, "h", None):

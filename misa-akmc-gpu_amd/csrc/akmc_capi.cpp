@@ -577,6 +577,77 @@ int akmc_set_rng_chunk(akmc_engine* e, int64_t events) {
   return AKMC_OK;
 }
 
+// Checkpoint file: magic, the sizes of the two records, akmc_params,
+// PlainState, the site count and one byte per site.  The vacancy list is
+// rebuilt from the sites (each sector's list is collected and sorted by key,
+// so the list order does not matter) and the two random streams are replayed
+// up to the next event's draws.
+static const char kCkptMagic[8] = {'A', 'K', 'M', 'C', 'C', 'K', 'P', '1'};
+
+int akmc_checkpoint_save(akmc_engine* e, const char* path) {
+  if (!e || !path) return fail(AKMC_ERR_INVALID, "null argument");
+  if (!e->lattice_ready) return fail(AKMC_ERR_INVALID, "no lattice to save");
+  std::vector<uint8_t> sites((size_t)e->nsites);
+  int rc = akmc_download(e, sites.data(), e->nsites);
+  if (rc) return rc;
+  FILE* f = fopen(path, "wb");
+  if (!f) return fail(AKMC_ERR_INVALID, "cannot open %s", path);
+  const uint32_t sz[2] = {(uint32_t)sizeof(akmc_params), (uint32_t)sizeof(akmc::PlainState)};
+  bool ok = fwrite(kCkptMagic, 1, 8, f) == 8 && fwrite(sz, sizeof(sz), 1, f) == 1 &&
+            fwrite(&e->p, sizeof(e->p), 1, f) == 1 && fwrite(&e->h_state, sizeof(e->h_state), 1, f) == 1 &&
+            fwrite(&e->nsites, sizeof(e->nsites), 1, f) == 1 &&
+            fwrite(sites.data(), 1, sites.size(), f) == sites.size();
+  ok = (fclose(f) == 0) && ok;
+  return ok ? AKMC_OK : fail(AKMC_ERR_INVALID, "write failed: %s", path);
+}
+
+int akmc_checkpoint_load(const char* path, int device, akmc_engine** out) {
+  if (!path || !out) return fail(AKMC_ERR_INVALID, "null argument");
+  *out = nullptr;
+  FILE* f = fopen(path, "rb");
+  if (!f) return fail(AKMC_ERR_INVALID, "cannot open %s", path);
+  char magic[8];
+  uint32_t sz[2];
+  akmc_params p;
+  akmc::PlainState st;
+  int64_t n = 0;
+  bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, kCkptMagic, 8) == 0 && fread(sz, sizeof(sz), 1, f) == 1 &&
+            sz[0] == sizeof(akmc_params) && sz[1] == sizeof(akmc::PlainState) && fread(&p, sizeof(p), 1, f) == 1 &&
+            fread(&st, sizeof(st), 1, f) == 1 && fread(&n, sizeof(n), 1, f) == 1 && n > 0;
+  std::vector<uint8_t> sites;
+  if (ok) {
+    sites.resize((size_t)n);
+    ok = fread(sites.data(), 1, sites.size(), f) == sites.size();
+  }
+  fclose(f);
+  if (!ok) return fail(AKMC_ERR_INVALID, "%s is not a checkpoint of this build", path);
+  akmc_engine* e = nullptr;
+  int rc = akmc_create(&p, device, &e);
+  if (rc) return rc;
+  if (n != e->nsites || st.events < 0) {
+    akmc_destroy(e);
+    return fail(AKMC_ERR_INVALID, "%s: inconsistent checkpoint", path);
+  }
+  rc = akmc_upload(e, sites.data(), n);
+  if (rc) { akmc_destroy(e); return rc; }
+  st.status = AKMC_RUN_OK;
+  e->h_state = st;
+  if (hipMemcpy(e->d_state, &e->h_state, sizeof(e->h_state), hipMemcpyHostToDevice) != hipSuccess) {
+    akmc_destroy(e);
+    return fail(AKMC_ERR_HIP, "state upload failed");
+  }
+  // Replay the streams: one draw of each per event performed so far.
+  for (int64_t i = 0; i < st.events; i++) {
+    (void)e->sel_rng();
+    (void)e->time_dis(e->time_rng);
+  }
+  e->generated = st.events;
+  e->chunk_base[0] = e->chunk_base[1] = -1;
+  fill_stats(e, &e->last);
+  *out = e;
+  return AKMC_OK;
+}
+
 int akmc_run_steps(akmc_engine* e, int64_t steps, int64_t max_events, akmc_stats* stats) {
   if (!e) return fail(AKMC_ERR_INVALID, "null engine");
   if (steps < 0) return fail(AKMC_ERR_INVALID, "steps < 0");

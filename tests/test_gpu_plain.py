@@ -212,6 +212,37 @@ def test_step_hooks_dump_and_log(tmp_path):
     eng.close()
 
 
+def test_checkpoint_restart_is_exact(tmp_path):
+    """Save after an event-cap stop inside a sector, load into a new engine
+    and continue: the same sites, sector clocks and time as one straight run,
+    and the same as the oracle."""
+    say("test_checkpoint_restart_is_exact")
+    cfg = make_cfg(n=20, va=60, T=620.0)
+    a = akmc.Engine(cfg)
+    st = a.run(3333)
+    assert st["in_sector"] == 1
+    path = tmp_path / "eng.ckpt"
+    a.save(path)
+    a.run(5000)
+    b = akmc.Engine.load(path)
+    assert b.stats()["events"] == 3333
+    b.run(5000)
+    sa, sb = a.stats(), b.stats()
+    assert np.array_equal(a.download(), b.download())
+    assert np.array_equal(_bits(sa["evolution"]), _bits(sb["evolution"]))
+    assert sa["sector_time"] == sb["sector_time"] and sa["step"] == sb["step"] and sa["events"] == sb["events"]
+    ref = refcpu.RefCpu(cfg)
+    ref.run(8333)
+    assert np.array_equal(b.download(), ref.sites())
+    assert np.array_equal(_bits(sb["evolution"]), _bits(ref.state()["evolution"]))
+    bad = tmp_path / "bad.ckpt"
+    bad.write_bytes(b"not a checkpoint")
+    with pytest.raises(akmc.AkmcError):
+        akmc.Engine.load(bad)
+    a.close()
+    b.close()
+
+
 def test_time_loop_end_and_empty_box():
     say("test_time_loop_end_and_empty_box")
     cfg = make_cfg(n=12, va=0, physics=3e-5, sector_T=3e-7)
