@@ -69,6 +69,20 @@ __device__ __forceinline__ double dpp_d(double v) {
   return as_f64(((uint64_t)hi << 32) | lo);
 }
 
+// OR of a 64-bit value over each group of eight lanes (8k .. 8k + 7), in
+// every lane of the group: quad_perm [1,0,3,2], quad_perm [2,3,0,1], then
+// row_half_mirror (lane i <-> 7 - i within each half row).
+__device__ __forceinline__ uint64_t or8_u64(uint64_t v) {
+  uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+  lo |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)lo, 0xB1, 0xF, 0xF, false);
+  hi |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)hi, 0xB1, 0xF, 0xF, false);
+  lo |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)lo, 0x4E, 0xF, 0xF, false);
+  hi |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)hi, 0x4E, 0xF, 0xF, false);
+  lo |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)lo, 0x141, 0xF, 0xF, false);
+  hi |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)hi, 0x141, 0xF, 0xF, false);
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // Inclusive scan across the wave (Hillis-Steele inside each 16-lane row with
 // row_shr 1/2/4/8, then row_bcast 15/31 across rows).  Exact for integers
 // whose totals stay below 2^53, which is all it is used for.
@@ -136,7 +150,6 @@ struct SectorLDS {
   uint8_t nb1[VCAP * 8];   // flag bytes of its 8 1nn sites
   uint8_t sten[EVB][STENCIL_MAX];
   uint8_t cflag[EVB];      // gathered flag byte of each batch vacancy's own site
-  int8_t tieb[EVB][8];     // tie-mask bit index of each batch rate
   int32_t list[64];
   int32_t nv;
   int32_t pad;
@@ -617,8 +630,11 @@ __device__ void eval_list(SectorLDS<VCAP, RateT>& S, const SharedTables& T, cons
       const int par = (int)(key_x(S.key[vi]) & 1);
       r = rate_fp64_staged(S.sten[a], par, LS, T.e1, T.e2, T.e0, L.nu, L.kT, T.exp_tab, X);
       S.rate[vi * 8 + b] = (RateT)r;
-      S.tieb[a][b] = (int8_t)tie_index((double)(RateT)r);
     }
+    // Tie mask of each vacancy: OR of its eight rates' bits, across the eight
+    // lanes of the vacancy.
+    const int ti = act ? tie_index((double)(RateT)r) : -1;
+    const uint64_t tmv = or8_u64(ti >= 0 ? 1ull << ti : 0ull);
     // Side effect: species present among the available targets of each vacancy.
     const uint64_t mcu = ballot(act & (X == SP_CU));
     const uint64_t mmn = ballot(act & (X == SP_MN));
@@ -635,13 +651,7 @@ __device__ void eval_list(SectorLDS<VCAP, RateT>& S, const SharedTables& T, cons
       const uint8_t nf = (uint8_t)(f0 & ~clr);
       if (nf != f0) L.sites[key_index(L.d, S.key[vi])] = nf;
       S.own[vi] = nf;
-      uint64_t tm = 0;
-#pragma unroll
-      for (int k = 0; k < 8; k++) {
-        const int t = S.tieb[a][k];
-        if (t >= 0) tm |= 1ull << t;
-      }
-      S.tmask[vi] = tm;
+      S.tmask[vi] = tmv;
     }
     wave_sync();
   }
