@@ -148,6 +148,8 @@ struct akmc_engine {
   bool lattice_ready = false;
   int32_t check_sums = 0;
   int64_t step_limit = INT64_MAX;  // akmc_run_steps: the time loop ends at this step
+  uint8_t* d_sector = nullptr;     // device-memory sector state (sectors beyond the LDS capacity)
+  int32_t sector_cap = 0;
   akmc_stats last{};
 };
 
@@ -187,8 +189,25 @@ int upload_vacancies(akmc_engine* e, const uint8_t* sites) {
     HIP_TRY(hipMalloc(&e->d_vac, (size_t)e->vac_capacity * sizeof(uint32_t)));
   }
   e->nvac = (int32_t)keys.size();
+  if (e->d_sector && e->nvac > e->sector_cap) {
+    HIP_TRY(hipFree(e->d_sector));
+    e->d_sector = nullptr;
+    e->sector_cap = 0;
+  }
   if (!keys.empty())
     HIP_TRY(hipMemcpy(e->d_vac, keys.data(), keys.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  return AKMC_OK;
+}
+
+// Switches the engine to the device-memory sector state (plain_kernel_hbm),
+// sized for every vacancy in one sector.
+int enable_hbm_sectors(akmc_engine* e) {
+  if (e->d_sector) return AKMC_OK;
+  const int64_t cap = std::max<int64_t>(e->nvac, 1);
+  int64_t off[9];
+  akmc::plain_sector_layout(cap, off);
+  HIP_TRY(hipMalloc(&e->d_sector, (size_t)off[8]));
+  e->sector_cap = (int32_t)cap;
   return AKMC_OK;
 }
 
@@ -220,6 +239,8 @@ akmc::PlainArgs base_args(akmc_engine* e) {
   a.kT = e->kT;
   a.rates_sector = -1;
   a.check_sums = e->check_sums;
+  a.sector_ws = e->d_sector;
+  a.sector_cap = e->sector_cap;
   return a;
 }
 
@@ -364,6 +385,7 @@ void akmc_destroy(akmc_engine* e) {
   (void)hipFree(e->d_vac);
   (void)hipFree(e->d_state);
   (void)hipFree(e->d_log);
+  (void)hipFree(e->d_sector);
   for (int b = 0; b < 2; b++) {
     (void)hipHostFree(e->h_u[b]);
     (void)hipHostFree(e->h_l[b]);
@@ -553,8 +575,15 @@ int akmc_run(akmc_engine* e, int64_t max_events, akmc_stats* stats) {
     if (st == AKMC_RUN_ERR_NO_EVENT)
       return fail(AKMC_ERR_NO_EVENT, "no event selected at event %lld (u * sum >= sum; the reference asserts here)",
                   (long long)e->h_state.events);
+    if (st == AKMC_RUN_ERR_CAPACITY && !e->d_sector) {
+      // A sector outgrew the LDS form: continue with the device-memory one
+      // (the launch stopped before the sector's first event).
+      int rc = enable_hbm_sectors(e);
+      if (rc) return rc;
+      continue;
+    }
     if (st == AKMC_RUN_ERR_CAPACITY)
-      return fail(AKMC_ERR_CAPACITY, "a sector holds more than %d vacancies", akmc_plain_vcap());
+      return fail(AKMC_ERR_CAPACITY, "a sector holds more than %d vacancies", e->sector_cap);
     return fail(AKMC_ERR_HIP, "event loop returned status %d", st);
   }
   const auto t1 = std::chrono::steady_clock::now();
@@ -750,7 +779,7 @@ int akmc_sector_rates(akmc_engine* e, int sector, int64_t cap, int64_t* sites, d
   if (!e || !n || !total || sector < 0 || sector > 7 || cap < 0) return fail(AKMC_ERR_INVALID, "bad argument");
   if (!e->lattice_ready) return fail(AKMC_ERR_INVALID, "no lattice");
   HIP_TRY(hipSetDevice(e->device));
-  const int32_t ocap = (int32_t)std::min<int64_t>(cap, akmc_plain_vcap());
+  const int32_t ocap = (int32_t)std::min<int64_t>(cap, std::max<int64_t>(akmc_plain_vcap(), e->nvac));
   uint32_t* d_keys = nullptr;
   double* d_rates = nullptr;
   double* d_total = nullptr;
@@ -762,6 +791,9 @@ int akmc_sector_rates(akmc_engine* e, int sector, int64_t cap, int64_t* sites, d
   HIP_TRY(hipMalloc(&d_n, sizeof(int32_t)));
   HIP_TRY(hipMalloc(&d_tmp, sizeof(akmc::PlainState)));
   HIP_TRY(hipMemcpy(d_tmp, &e->h_state, sizeof(e->h_state), hipMemcpyHostToDevice));
+  int rc = AKMC_OK;
+  int32_t hn = 0;
+  for (int attempt = 0; attempt < 2; attempt++) {
   akmc::PlainArgs a = base_args(e);
   a.state = d_tmp;
   a.rates_sector = sector;
@@ -774,15 +806,20 @@ int akmc_sector_rates(akmc_engine* e, int sector, int64_t cap, int64_t* sites, d
   a.out_total = d_total;
   a.out_n = d_n;
   a.out_cap = ocap;
-  int rc = AKMC_OK;
   if (akmc_plain_launch(&a, e->stream) != 0) rc = fail(AKMC_ERR_HIP, "rates launch failed");
   hipError_t he = hipStreamSynchronize(e->stream);
   if (rc == AKMC_OK && he != hipSuccess) rc = fail(AKMC_ERR_HIP, "rates kernel: %s", hipGetErrorString(he));
-  int32_t hn = 0;
   if (rc == AKMC_OK) {
     (void)hipMemcpy(&hn, d_n, sizeof(hn), hipMemcpyDeviceToHost);
     (void)hipMemcpy(total, d_total, sizeof(double), hipMemcpyDeviceToHost);
-    if (hn < 0) rc = fail(hn == AKMC_RUN_ERR_CAPACITY ? AKMC_ERR_CAPACITY : AKMC_ERR_HIP, "rates kernel status %d", hn);
+    if (hn == AKMC_RUN_ERR_CAPACITY && !e->d_sector && attempt == 0) {
+      rc = enable_hbm_sectors(e);
+      if (rc == AKMC_OK) continue;
+    }
+    if (rc == AKMC_OK && hn < 0)
+      rc = fail(hn == AKMC_RUN_ERR_CAPACITY ? AKMC_ERR_CAPACITY : AKMC_ERR_HIP, "rates kernel status %d", hn);
+  }
+  break;
   }
   if (rc == AKMC_OK) {
     *n = hn;

@@ -66,8 +66,24 @@ struct PlainArgs {
   double* out_total;
   int32_t* out_n;
   int32_t out_cap;
-  int32_t pad2;
+  // Sector state in device memory (plain_kernel_hbm) for sectors holding more
+  // vacancies than the LDS form: plain_sector_layout(sector_cap) bytes.
+  int32_t sector_cap;
+  uint8_t* sector_ws;
 };
+
+// Byte offsets of the device-memory sector arrays for a capacity of `cap`
+// vacancies: key, slot, rate (8 per vacancy), pvac, tmask, risk, own, nb1
+// (8 per vacancy); off[8] is the total.  Every array starts on 256 bytes.
+__host__ __device__ inline void plain_sector_layout(int64_t cap, int64_t off[9]) {
+  const int64_t sz[8] = {4 * cap, 4 * cap, 64 * cap, 8 * cap, 8 * cap, cap, cap, 8 * cap};
+  int64_t o = 0;
+  for (int i = 0; i < 8; i++) {
+    off[i] = o;
+    o += (sz[i] + 255) / 256 * 256;
+  }
+  off[8] = o;
+}
 
 }  // namespace akmc
 

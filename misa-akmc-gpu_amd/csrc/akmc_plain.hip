@@ -56,7 +56,8 @@ __device__ void stage_tables(SharedTables& T) {
 
 // Collects the vacancies of sector s from the global list and sorts them by
 // key (sweep order).  Returns the count, or -1 if it exceeds the capacity.
-__device__ int collect_sector(PlainSector& S, const LatticeView& L, int s) {
+template <class SecT>
+__device__ int collect_sector(SecT& S, const LatticeView& L, int s, int cap) {
   const int l = lane_id();
   int n = 0;
   for (int b = 0; b < L.nvac; b += 64) {
@@ -65,10 +66,10 @@ __device__ int collect_sector(PlainSector& S, const LatticeView& L, int s) {
     const bool in = j < L.nvac && key_sector(L.d, k) == s;
     const uint64_t m = ballot(in);
     const int r = n + __popcll(m & ((1ull << l) - 1));
-    if (in && r < PLAIN_VCAP) { S.key[r] = k; S.slot[r] = j; }
+    if (in && r < cap) { S.key[r] = k; S.slot[r] = j; }
     n += __popcll(m);
   }
-  if (n > PLAIN_VCAP) return -1;
+  if (n > cap) return -1;
   wave_sync();
   // Bitonic network in its all-ascending form: each stage first compares i
   // with its mirror i ^ (k - 1), then with i ^ d for d = k/4 .. 1.  Every
@@ -96,7 +97,8 @@ __device__ int collect_sector(PlainSector& S, const LatticeView& L, int s) {
 
 // Moves entry `from_i` to position `to_i` (shifting the entries between), or
 // deletes it when to_i < 0.  pvac is not moved (it is repaired afterwards).
-__device__ void move_entry(PlainSector& S, int nv, int from_i, int to_i) {
+template <class SecT>
+__device__ void move_entry(SecT& S, int nv, int from_i, int to_i) {
   const int l = lane_id();
   int lo, hi, delta;
   if (to_i < 0) { lo = from_i + 1; hi = nv - 1; delta = -1; }
@@ -133,7 +135,8 @@ __device__ void move_entry(PlainSector& S, int nv, int from_i, int to_i) {
 // Self-check: the running sums and the total against a serial refold of the
 // sector's rates in sweep order (every lane folds the same values, so the
 // result is wave-uniform).
-__device__ bool sums_agree(const PlainSector& S, int nv, double sum) {
+template <class SecT>
+__device__ bool sums_agree(const SecT& S, int nv, double sum) {
   wave_sync();
   double p = 0.0;
   bool ok = true;
@@ -145,16 +148,16 @@ __device__ bool sums_agree(const PlainSector& S, int nv, double sum) {
   return ok & (p == (nv > 0 ? sum : 0.0));
 }
 
-__global__ __launch_bounds__(64) void plain_kernel(PlainArgs a) {
-  __shared__ PlainSector S;
-  __shared__ SharedTables T;
+// The event loop over a sector state S of capacity `cap` (LDS or device
+// memory); T and evo[8] are LDS.
+template <class SecT>
+__device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap, SharedTables& T, double* evo) {
   const int l = lane_id();
   stage_tables(T);
   LaneStencil LS = lane_stencil(T.st);
   lane_linear(LS, T.st, a.dims);
   const LatticeView L{a.sites, a.vac_key, a.nvac, a.dims, a.nu, a.kT};
   const Patch nopatch{0xFFFFFFFFu, 0xFFFFFFFFu, 0, 0, -1, ~0ull, ~0ull};
-  __shared__ double evo[8];  // sector clocks (indexed by the running sector id)
   PlainState st = *a.state;  // every lane keeps an identical copy
   if (l < 8) evo[l] = a.state->evo[l];
   wave_sync();
@@ -207,12 +210,12 @@ __global__ __launch_bounds__(64) void plain_kernel(PlainArgs a) {
       // writes visible first.  (Within a sector the engine never reads back
       // a byte it wrote in the same event; those are patched from registers.)
       __threadfence_block();
-      nv = collect_sector(S, L, s);
+      nv = collect_sector(S, L, s, cap);
       if (nv < 0) { status = AKMC_RUN_ERR_CAPACITY; break; }
-      eval_list<PLAIN_VCAP, double>(S, T, LS, L, nullptr, 0, nv, nopatch, false, 0, -1);
+      eval_list(S, T, LS, L, nullptr, 0, nv, nopatch, false, 0, -1);
       st.n_eval += nv;
       st.n_fold += nv;
-      sum = fold_range<PLAIN_VCAP, double>(S, 0, nv, 0.0);
+      sum = fold_range(S, 0, nv, 0.0);
       if (a.check_sums && !sums_agree(S, nv, sum)) st.sum_errors++;
       rebuild = false;
       if (a.rates_sector >= 0) {
@@ -353,7 +356,7 @@ __global__ __launch_bounds__(64) void plain_kernel(PlainArgs a) {
         amin = min(amin, b0 + first_set(m));
         amax = max(amax, b0 + 63 - (int)__clzll(m));
         wave_sync();
-        eval_list<PLAIN_VCAP, double>(S, T, LS, L, S.list, 0, __popcll(m), pt, true, pf, stays ? knew : -1);
+        eval_list(S, T, LS, L, S.list, 0, __popcll(m), pt, true, pf, stays ? knew : -1);
         st.n_eval += __popcll(m);
       }
     }
@@ -363,16 +366,16 @@ __global__ __launch_bounds__(64) void plain_kernel(PlainArgs a) {
     } else if (!stays) {
       const int c_lo = min(min(vsel, amin), nv);
       st.n_fold += nv - c_lo;
-      sum = c_lo < nv ? fold_range<PLAIN_VCAP, double>(S, c_lo, nv, c_lo > 0 ? S.pvac[c_lo - 1] : 0.0)
+      sum = c_lo < nv ? fold_range(S, c_lo, nv, c_lo > 0 ? S.pvac[c_lo - 1] : 0.0)
                       : S.pvac[nv - 1];
     } else {
       const int c_lo = min(min(vsel, knew), amin);
       const int c_hi = max(max(vsel, knew), amax);
       const double Po = S.pvac[c_hi];
-      const double Pn = fold_range<PLAIN_VCAP, double>(S, c_lo, c_hi + 1, c_lo > 0 ? S.pvac[c_lo - 1] : 0.0);
+      const double Pn = fold_range(S, c_lo, c_hi + 1, c_lo > 0 ? S.pvac[c_lo - 1] : 0.0);
       st.n_fold += c_hi + 1 - c_lo;
       st.n_shift += nv - (c_hi + 1);
-      sum = c_hi + 1 < nv ? tail_update<PLAIN_VCAP, double>(S, c_hi + 1, nv, Pn, Po) : Pn;
+      sum = c_hi + 1 < nv ? tail_update(S, c_hi + 1, nv, Pn, Po) : Pn;
     }
     if (a.check_sums && !sums_agree(S, nv, sum)) st.sum_errors++;
   }
@@ -384,10 +387,37 @@ __global__ __launch_bounds__(64) void plain_kernel(PlainArgs a) {
   if (l == 0 && a.rates_sector >= 0 && status != AKMC_RUN_OK) *a.out_n = status;
 }
 
+__global__ __launch_bounds__(64) void plain_kernel(PlainArgs a) {
+  __shared__ PlainSector S;
+  __shared__ SharedTables T;
+  __shared__ double evo[8];  // sector clocks (indexed by the running sector id)
+  plain_body(a, S, PLAIN_VCAP, T, evo);
+}
+
+// The same loop with the sector arrays in device memory, for sectors of more
+// than PLAIN_VCAP vacancies.
+__global__ __launch_bounds__(64) void plain_kernel_hbm(PlainArgs a) {
+  __shared__ SharedTables T;
+  __shared__ double evo[8];
+  __shared__ uint8_t sten[EVB][STENCIL_MAX];
+  __shared__ uint8_t cflag[EVB];
+  __shared__ int32_t list[64];
+  int64_t off[9];
+  plain_sector_layout(a.sector_cap, off);
+  uint8_t* w = a.sector_ws;
+  SectorRef S{reinterpret_cast<uint32_t*>(w + off[0]), reinterpret_cast<int32_t*>(w + off[1]),
+              reinterpret_cast<double*>(w + off[2]),   reinterpret_cast<double*>(w + off[3]),
+              reinterpret_cast<uint64_t*>(w + off[4]), w + off[5], w + off[6], w + off[7], sten, cflag, list};
+  plain_body(a, S, a.sector_cap, T, evo);
+}
+
 }  // namespace akmc
 
 extern "C" int akmc_plain_launch(const akmc::PlainArgs* args, hipStream_t stream) {
-  hipLaunchKernelGGL(akmc::plain_kernel, dim3(1), dim3(64), 0, stream, *args);
+  if (args->sector_ws)
+    hipLaunchKernelGGL(akmc::plain_kernel_hbm, dim3(1), dim3(64), 0, stream, *args);
+  else
+    hipLaunchKernelGGL(akmc::plain_kernel, dim3(1), dim3(64), 0, stream, *args);
   return (int)hipGetLastError();
 }
 

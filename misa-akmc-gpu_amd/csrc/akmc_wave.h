@@ -140,6 +140,7 @@ constexpr int EVB = 8;  // vacancies evaluated per gather/compute pass
 
 template <int VCAP, typename RateT>
 struct SectorLDS {
+  using rate_t = RateT;
   uint32_t key[VCAP];      // sorted lattice keys of the sector's vacancies
   int32_t slot[VCAP];      // index into the lattice's global vacancy list
   RateT rate[VCAP * 8];
@@ -153,6 +154,24 @@ struct SectorLDS {
   int32_t list[64];
   int32_t nv;
   int32_t pad;
+};
+
+// The same sector state with the per-vacancy arrays in device memory (any
+// capacity: sectors too crowded for LDS) and the per-pass staging arrays in
+// LDS.  The helpers below take either form.
+struct SectorRef {
+  using rate_t = double;
+  uint32_t* key;
+  int32_t* slot;
+  double* rate;
+  double* pvac;
+  uint64_t* tmask;
+  uint8_t* risk;
+  uint8_t* own;
+  uint8_t* nb1;
+  uint8_t (*sten)[STENCIL_MAX];
+  uint8_t* cflag;
+  int32_t* list;
 };
 
 // Tables every engine of a workgroup reads (staged once into LDS).
@@ -216,8 +235,8 @@ __device__ __forceinline__ int first_above(const double* pvac, int n, double tar
 // reference order: lane 8j + b loads slot b of vacancy v + j (one LDS load),
 // the fold reads the slots through readlane, and lane j writes vacancy
 // v + j's running sum and tie risk.  Returns the sum after the last.
-template <int VCAP, typename RateT>
-__device__ __forceinline__ double fold_serial(SectorLDS<VCAP, RateT>& S, int v, int cnt, double P) {
+template <class SecT>
+__device__ __forceinline__ double fold_serial(SecT& S, int v, int cnt, double P) {
   const int l = lane_id();
   const int last = v + cnt - 1;
   const double rl = (double)S.rate[8 * min(v + (l >> 3), last) + (l & 7)];
@@ -241,8 +260,8 @@ __device__ __forceinline__ double fold_serial(SectorLDS<VCAP, RateT>& S, int v, 
 
 // Exact running sums for vacancies [v0, v1) starting from P (the sum before
 // v0).  Writes pvac and returns the sum after v1 - 1.
-template <int VCAP, typename RateT>
-__device__ __forceinline__ double fold_range(SectorLDS<VCAP, RateT>& S, int v0, int v1, double P) {
+template <class SecT>
+__device__ __forceinline__ double fold_range(SecT& S, int v0, int v1, double P) {
   const int l = lane_id();
   int v = uni_i(v0);
   v1 = uni_i(v1);
@@ -250,7 +269,7 @@ __device__ __forceinline__ double fold_range(SectorLDS<VCAP, RateT>& S, int v0, 
   while (v < v1) {
     if (v1 - v <= 6) {
       // A few vacancies: serially (cheaper than a scan pass).
-      P = fold_serial<VCAP, RateT>(S, v, v1 - v, P);
+      P = fold_serial(S, v, v1 - v, P);
       break;
     }
     if (!(P >= 0x1p-969)) {
@@ -352,8 +371,8 @@ __device__ __forceinline__ double fold_range(SectorLDS<VCAP, RateT>& S, int v0, 
 // round.  The vacancies where it ends are folded serially, four per LDS
 // round trip, until old and new sums share a binade again, and the next
 // region starts with the new exact difference.
-template <int VCAP, typename RateT>
-__device__ __forceinline__ double tail_update(SectorLDS<VCAP, RateT>& S, int v0, int n, double Pn, double Po) {
+template <class SecT>
+__device__ __forceinline__ double tail_update(SecT& S, int v0, int n, double Pn, double Po) {
   const int l = lane_id();
   int v = uni_i(v0);
   n = uni_i(n);
@@ -368,7 +387,7 @@ __device__ __forceinline__ double tail_update(SectorLDS<VCAP, RateT>& S, int v0,
     }
     const int e = exponent_of(Sold);
     const bool start_ok = (Sold >= 0x1p-969) & (exponent_of(Pprev) == e) & (Sold + D == Pprev) & (Pprev - D == Sold);
-    if (!start_ok && capped) return fold_range<VCAP, RateT>(S, v, n, Pprev);
+    if (!start_ok && capped) return fold_range(S, v, n, Pprev);
     int stop = v;
     if (start_ok) {
       // D < 2^e here, so top - D is exact, and D / u is an integer below 2^52.
@@ -574,8 +593,8 @@ __device__ __forceinline__ double rate_fp64_staged(const uint8_t* sp, int par, c
 // replaced by the patch values (the hop that was just written may not be
 // visible yet).  pf/pf_vi: this lane's stencil byte (site index = lane) of
 // vacancy pf_vi, gathered earlier by the caller (pf_vi < 0: none).
-template <int VCAP, typename RateT>
-__device__ void eval_list(SectorLDS<VCAP, RateT>& S, const SharedTables& T, const LaneStencil& LS, const LatticeView& L,
+template <class SecT>
+__device__ void eval_list(SecT& S, const SharedTables& T, const LaneStencil& LS, const LatticeView& L,
                           const int32_t* idx, int base, int n, const Patch& pt, bool own_from_lds, uint8_t pf,
                           int pf_vi) {
   const int l = lane_id();
@@ -629,11 +648,11 @@ __device__ void eval_list(SectorLDS<VCAP, RateT>& S, const SharedTables& T, cons
     if (act) {
       const int par = (int)(key_x(S.key[vi]) & 1);
       r = rate_fp64_staged(S.sten[a], par, LS, T.e1, T.e2, T.e0, L.nu, L.kT, T.exp_tab, X);
-      S.rate[vi * 8 + b] = (RateT)r;
+      S.rate[vi * 8 + b] = (typename SecT::rate_t)r;
     }
     // Tie mask of each vacancy: OR of its eight rates' bits, across the eight
     // lanes of the vacancy.
-    const int ti = act ? tie_index((double)(RateT)r) : -1;
+    const int ti = act ? tie_index((double)(typename SecT::rate_t)r) : -1;
     const uint64_t tmv = or8_u64(ti >= 0 ? 1ull << ti : 0ull);
     // Side effect: species present among the available targets of each vacancy.
     const uint64_t mcu = ballot(act & (X == SP_CU));

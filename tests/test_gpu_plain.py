@@ -265,13 +265,26 @@ def test_time_loop_end_and_empty_box():
     assert np.array_equal(eng.download(), ref.sites())
 
 
-def test_sector_capacity_error():
-    say("test_sector_capacity_error")
-    cfg = make_cfg(n=32, va=16000)  # ~2000 vacancies per sector > 1536
+def test_sectors_beyond_lds_capacity():
+    """Sectors of more than the LDS form's 1,536 vacancies run with the sector
+    state in device memory: rates and trajectory bit-exact vs the oracle."""
+    say("test_sectors_beyond_lds_capacity")
+    cfg = make_cfg(n=32, va=16000, T=600.0)  # ~2,000 vacancies per sector
     eng = akmc.Engine(cfg)
-    with pytest.raises(akmc.AkmcError) as e:
-        eng.run(10)
-    assert e.value.code == -3
+    ref = refcpu.RefCpu(cfg)
+    for s in (0, 5):
+        gs, gr, gt = eng.sector_rates(s, cap=4096)
+        rs, rr, rt = ref.sector_rates(s, cap=4096)
+        assert len(gs) > 1536
+        assert np.array_equal(gs, rs) and np.array_equal(_bits(gr), _bits(rr)) and gt == rt
+    eng.close()
+    st = compare_run(cfg, 400)
+    assert st["events"] == 400
+    eng = akmc.Engine(cfg)
+    eng.set_check_sums(True)
+    st = eng.run(1500)
+    assert st["sum_errors"] == 0
+    eng.close()
 
 
 def test_full_size_prefix_and_invariants():
