@@ -200,34 +200,36 @@ struct Patch {
 };
 
 // ---------------------------------------------------------------- searches
-// First index i in [0, n) with key[i] >= target, n if none (key sorted).
-__device__ __forceinline__ int lower_bound_key(const uint32_t* key, int n, uint32_t target) {
+// First index i in [0, n) with pred(a[i]), n if none, for a sorted so that
+// pred is false and then true.  Each round splits the remaining range into
+// 64 chunks and keeps the first chunk whose last element satisfies pred (all
+// elements before it fail): two rounds up to 4,096 elements, three up to
+// 262,144.
+template <typename T, class Pred>
+__device__ __forceinline__ int first_true(const T* a, int n, Pred pred) {
   if (n <= 0) return 0;
   const int l = lane_id();
-  const int S = (n + 63) >> 6;
-  const int ci = min(n - 1, l * S + S - 1);
-  const uint32_t kc = key[ci];
-  const int f = first_set(ballot((l * S < n) & (kc >= target)));
-  if (f == 64) return n;
-  const int fi = f * S + l;
-  const uint32_t kf = key[min(fi, n - 1)];
-  const int g = first_set(ballot((l < S) & (fi < n) & (kf >= target)));
-  return g == 64 ? min(n, (f + 1) * S) : f * S + g;
+  int lo = 0, len = n;
+  while (len > 64) {
+    const int S = (len + 63) >> 6;
+    const int ci = lo + min(len - 1, l * S + S - 1);
+    const int f = first_set(ballot((l * S < len) & pred(a[ci])));
+    if (f == 64) return n;  // only possible in the first round
+    lo += f * S;
+    len = min(S, len - f * S);
+  }
+  const int g = first_set(ballot((l < len) & pred(a[lo + min(l, len - 1)])));
+  return g == 64 ? n : lo + g;
+}
+
+// First index i in [0, n) with key[i] >= target, n if none (key sorted).
+__device__ __forceinline__ int lower_bound_key(const uint32_t* key, int n, uint32_t target) {
+  return first_true(key, n, [=](uint32_t k) { return k >= target; });
 }
 
 // First index v in [0, n) with pvac[v] > target, n if none.
 __device__ __forceinline__ int first_above(const double* pvac, int n, double target) {
-  if (n <= 0) return 0;
-  const int l = lane_id();
-  const int S = (n + 63) >> 6;
-  const int ci = min(n - 1, l * S + S - 1);
-  const double pc = pvac[ci];
-  const int f = first_set(ballot((l * S < n) & (pc > target)));
-  if (f == 64) return n;
-  const int fi = f * S + l;
-  const double pf = pvac[min(fi, n - 1)];
-  const int g = first_set(ballot((l < S) & (fi < n) & (pf > target)));
-  return g == 64 ? min(n, (f + 1) * S) : f * S + g;
+  return first_true(pvac, n, [=](double p) { return p > target; });
 }
 
 // ---------------------------------------------------------------- fold

@@ -285,6 +285,16 @@ def test_sectors_beyond_lds_capacity():
     st = eng.run(1500)
     assert st["sum_errors"] == 0
     eng.close()
+    # More than 4,096 vacancies per sector: three search rounds over the sums.
+    cfg = make_cfg(n=48, va=40000, T=620.0)
+    eng = akmc.Engine(cfg)
+    ref = refcpu.RefCpu(cfg)
+    gs, gr, gt = eng.sector_rates(3, cap=8192)
+    rs, rr, rt = ref.sector_rates(3, cap=8192)
+    assert len(gs) > 4096
+    assert np.array_equal(gs, rs) and np.array_equal(_bits(gr), _bits(rr)) and gt == rt
+    eng.close()
+    compare_run(cfg, 150)
 
 
 def test_full_size_prefix_and_invariants():
