@@ -106,3 +106,14 @@ def test_output_section(tmp_path):
     q = tmp_path / "noout.yaml"
     q.write_text("box:\n  size: [8, 8, 8]\n")
     assert akmc.output_from_yaml(q)["dump_interval"] == 0
+
+
+def test_checkpoint_load_rejects_other_files(tmp_path):
+    """The checkpoint reader checks the file before it creates an engine (no
+    device is touched for a file that is not a checkpoint of this build)."""
+    p = tmp_path / "x.ckpt"
+    p.write_bytes(b"AKMCCKP0" + bytes(64))
+    with pytest.raises(akmc.AkmcError):
+        akmc.Engine.load(p)
+    with pytest.raises(akmc.AkmcError):
+        akmc.Engine.load(tmp_path / "missing.ckpt")
