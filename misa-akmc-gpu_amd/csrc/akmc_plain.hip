@@ -236,8 +236,8 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
 
     // ---- select: first (vacancy, dir) whose running sum exceeds u * sum
     // (ModelAdapter::selectAndPerform, model_adapter.h:95-101; ABVIModel::select, kmc.cpp:242-309).
-    const double target = u_next * sum;
-    const double lg = l_next;
+    const double target = uni_d(u_next) * sum;
+    const double lg = uni_d(l_next);
     const int vsel = first_above(S.pvac, nv, target);
     if (vsel >= nv) { status = AKMC_RUN_ERR_NO_EVENT; break; }
     int bsel = -1;
@@ -252,11 +252,12 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
         if (bsel < 0 && p > target) bsel = b;
       }
     }
+    bsel = uni_i(bsel);
     if (bsel < 0) { status = AKMC_RUN_ERR_NO_EVENT; break; }
 
     // ---- perform (ABVIModel::perform, kmc.cpp:358-396): the vacancy moves to
     // the target; the target's first species tag of Cu, Mn, Ni moves back.
-    const uint32_t from = S.key[vsel];
+    const uint32_t from = (uint32_t)uni_i((int)S.key[vsel]);
     const int par = (int)(key_x(from) & 1);
     int ox, oy, oz;
     nn1_offset(par, bsel, &ox, &oy, &oz);
