@@ -776,7 +776,7 @@ __global__ __launch_bounds__(256) void replica_kernel(BatchArgs a) {
   uint8_t* lds_sec = region + RSTAGE_BYTES;
   const int stride = (int)gridDim.x * waves;
   for (int item = (int)blockIdx.x * waves + wave; item < a.n_items; item += stride)
-    run_item<RateT, FP32, LDS>(a, T, G, lds_sec, a.items[item]);
+    run_item<RateT, FP32, LDS>(a, T, G, lds_sec, __builtin_amdgcn_readfirstlane(a.items[item]));
 }
 
 // Records the launch start tick (queue[1]) for the time budget.
