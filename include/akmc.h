@@ -168,6 +168,24 @@ int akmc_rng_streams(int64_t seed_time_inc, int64_t first, int64_t n, double* u,
 int akmc_exp_host(const double* x, double* y, int64_t n);
 int akmc_exp_device(int device, const double* x, double* y, int64_t n);
 
+/* Self-interstitial (dumbbell) rates of the abvi model on a typed lattice.
+ * types: one LatticeTypes value per real site (uint16: V 0, Fe 1, Cu 2, Ni 4,
+ * Mn 8, dumbbell XY = X << 8 | Y with X <= Y; src/lattice/lattice_types.h:
+ * 20-36), index x + 2 box[0] (y + box[1] z), periodic box of at least 3
+ * cells per axis.  sites: the n_itl dumbbell sites, strictly ascending (the
+ * order of ItlList::mp, models/abvi/defect/itl_list.h); orient: their
+ * orientations, -6 .. 6 without 0 (models/abvi/orientation.h:12-30).
+ * Outputs avail[n_itl] (Itl::beforeRatesUpdate's avail_trans_dir,
+ * itl_list.cpp:9-29) and rates[8 n_itl] (Itl::updateRates with
+ * ItlRatesSolver::rate as the callback, itl_list.cpp:31-50,
+ * rate/itl_rates_solver.cpp:16-80, rate/bonds/energy_dumbbell.cpp:7-206):
+ * every mask first, then every rate.  The reference has no caller of this
+ * model (models/abvi/kmc.cpp:393-395), and its lattice no longer holds
+ * dumbbell types; this is the model's rate step for callers that do. */
+int akmc_itl_rates(const int64_t box[3], const uint16_t* types, int64_t n_itl, const int64_t* sites,
+                   const int8_t* orient, double attempt_freq, double temperature, int device, uint8_t* avail,
+                   double* rates);
+
 /* Waits for all work on a device (benchmark bracketing). */
 int akmc_sync(int device);
 

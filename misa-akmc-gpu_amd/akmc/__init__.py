@@ -13,6 +13,7 @@ _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(_PKG, "lib", "libakmc.so")
 
 AKMC_OK = 0
+AKMC_ERR_INVALID = -1
 AKMC_PREC_FP64 = 0
 AKMC_PREC_FP32 = 1
 AKMC_STOP_TIME = 1
@@ -112,6 +113,7 @@ def lib():
     L.akmc_copy_bench.argtypes = [i32, i64, i32, P(dbl)]
     L.akmc_exp_host.argtypes = [vp, vp, i64]
     L.akmc_exp_device.argtypes = [i32, vp, vp, i64]
+    L.akmc_itl_rates.argtypes = [vp, vp, i64, vp, vp, dbl, dbl, i32, vp, vp]
     L.akmc_destroy.argtypes = [vp]
     L.akmc_destroy.restype = None
     _lib = L
@@ -632,3 +634,34 @@ class Server:
 
     def clock(self):
         return _blib().akmc_server_clock(self.h)
+
+
+# Dumbbell (self-interstitial) types, LatticeTypes values
+# (src/lattice/lattice_types.h:20-36).
+LAT_V, LAT_FE, LAT_CU, LAT_NI, LAT_MN = 0, 1, 2, 4, 8
+
+
+def dumbbell(a, b):
+    """LatticeTypes value of the dumbbell of atoms a and b (combineToInter)."""
+    a, b = min(a, b), max(a, b)
+    return (a << 8) | b
+
+
+def itl_rates(box, types, sites, orient, attempt_freq, temperature, device=0):
+    """Rates of the abvi dumbbell model (akmc_itl_rates): (avail[n],
+    rates[n, 8]) for the dumbbells at `sites` (ascending) with orientations
+    `orient`, on a periodic lattice of LatticeTypes values `types`."""
+    b = np.ascontiguousarray(box, dtype=np.int64)
+    t = np.ascontiguousarray(types, dtype=np.uint16)
+    s = np.ascontiguousarray(sites, dtype=np.int64)
+    o = np.ascontiguousarray(orient, dtype=np.int8)
+    if t.size != 2 * int(b[0]) * int(b[1]) * int(b[2]):
+        raise AkmcError(AKMC_ERR_INVALID, "types holds %d sites, the box %d" % (t.size, 2 * b[0] * b[1] * b[2]))
+    if o.size != s.size:
+        raise AkmcError(AKMC_ERR_INVALID, "one orientation per dumbbell")
+    av = np.zeros(s.size, dtype=np.uint8)
+    r = np.zeros((s.size, 8), dtype=np.float64)
+    _check(lib().akmc_itl_rates(b.ctypes.data, t.ctypes.data, s.size, s.ctypes.data, o.ctypes.data,
+                                float(attempt_freq), float(temperature), int(device), av.ctypes.data,
+                                r.ctypes.data))
+    return av, r

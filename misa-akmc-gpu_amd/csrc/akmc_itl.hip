@@ -1,0 +1,375 @@
+// Self-interstitial (dumbbell) transition rates of the abvi model on a typed
+// lattice, and the C entry point akmc_itl_rates.
+//
+// Lattice: one reference LatticeTypes value per real site (uint16: V 0, Fe 1,
+// Cu 2, Ni 4, Mn 8, dumbbell XY = X << 8 | Y with X <= Y;
+// src/lattice/lattice_types.h:20-36), index x + 2nx (y + ny z) with x
+// doubled, periodic.  The dumbbells are listed in ascending site order, the
+// order of the reference's ItlList::mp (a std::map keyed by lattice id,
+// models/abvi/defect/itl_list.h), each with an orientation in [-6, 6] \ {0}
+// (models/abvi/orientation.h:12-30).
+//
+// Per dumbbell, as Itl::beforeRatesUpdate then Itl::updateRates with
+// ItlRatesSolver::rate as the callback (models/abvi/defect/itl_list.cpp:9-50,
+// models/abvi/rate/rates_solver.cpp:13-26, itl_rates_solver.cpp:16-80):
+//   avail  = orientation directions whose 1nn holds a single atom
+//   for each available direction b: the jumping atom g (the high or low end
+//   of XY by the orientation), dE = e_after - e_before with
+//     e = count(src) + count(dst) + Edumb(all dumbbells)
+//   before and after g moves (src keeps the other atom, dst becomes a
+//   dumbbell), k = nu exp(-(e0(g) + dE / 2) / kB T), stored halved in the
+//   direction's two rate slots.
+// Edumb (models/abvi/rate/bonds/energy_dumbbell.cpp:7-206) is a left fold over
+// the whole list, so each rate re-folds the list: the kernel keeps every
+// dumbbell's own term from the unchanged lattice and recomputes only the terms
+// of dumbbells within reach of src or dst; the fold is exact to the bit.
+//
+// All stored availability masks are computed before any rate (the reference
+// reads each dumbbell's stored mask inside FeX_comp, energy_dumbbell.cpp:144).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "../../include/akmc.h"
+#include "akmc_exp.h"
+#include "akmc_exp_table.h"
+#include "akmc_host.h"
+#include "akmc_model.h"
+
+using akmc_host::fail;
+
+namespace akmc {
+namespace itl {
+
+__constant__ uint64_t c_exp_tab[256] = AKMC_EXP_TAB_INIT;
+__constant__ double c_e1[25] = AKMC_E1_TABLE;
+__constant__ double c_e2[25] = AKMC_E2_TABLE;
+// Direction masks by orientation o + 6: the high end's (X of XY) and the low
+// end's (orientation.cc:180-236).
+__constant__ uint8_t c_high[13] = {0x50, 0xA0, 0x44, 0x88, 0x30, 0xC0, 0, 0x03, 0x0C, 0x11, 0x22, 0x05, 0x0A};
+__constant__ uint8_t c_low[13] = {0x0A, 0x05, 0x22, 0x11, 0x0C, 0x03, 0, 0xC0, 0x30, 0x88, 0x44, 0xA0, 0x50};
+
+enum : int { T_V = 0, T_FE = 1, T_CU = 2, T_NI = 4, T_MN = 8 };
+constexpr uint8_t STATUS_2NN = 0x2F;  // period_lattice_list.cpp:15-18
+
+__device__ __forceinline__ bool is_dumbbell(int t) { return t > 0x0008; }
+__device__ __forceinline__ bool is_atom_or_v(int t) { return t <= 0x0008; }
+__device__ __forceinline__ bool is_atom(int t) { return t <= 0x0008 && t != 0; }
+__device__ __forceinline__ int species(int t) {  // single type -> bond-table index
+  return t == T_V ? SP_V : t == T_FE ? SP_FE : t == T_CU ? SP_CU : t == T_NI ? SP_NI : SP_MN;
+}
+
+struct Dims {
+  int32_t X, Y, Z;
+};
+
+__device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : v >= n ? v - n : v; }
+__device__ __forceinline__ int64_t site_of(const Dims& d, int x, int y, int z) {
+  return (int64_t)wrap(x, d.X) + (int64_t)d.X * ((int64_t)wrap(y, d.Y) + (int64_t)d.Y * wrap(z, d.Z));
+}
+// 1nn b of (x, y, z) in the reference's order (period_lattice_list.cpp:20-44).
+__device__ __forceinline__ int64_t nn1(const Dims& d, int x, int y, int z, int b) {
+  const int lo = (x & 1) ? 0 : -1;
+  return site_of(d, x + ((b & 4) ? 1 : -1), y + lo + ((b >> 1) & 1), z + lo + (b & 1));
+}
+// 2nn k (period_lattice_list.cpp:223-232): (-2,0,0) (0,-1,0) (0,0,-1) (0,0,1) (0,1,0) (2,0,0).
+__device__ __forceinline__ int64_t nn2(const Dims& d, int x, int y, int z, int k) {
+  const int dx = k == 0 ? -2 : k == 5 ? 2 : 0;
+  const int dy = k == 1 ? -1 : k == 4 ? 1 : 0;
+  const int dz = k == 2 ? -1 : k == 3 ? 1 : 0;
+  return site_of(d, x + dx, y + dy, z + dz);
+}
+
+// The lattice with up to two sites replaced (the state after a jump).
+struct View {
+  const uint16_t* t;
+  int64_t s0, s1;
+  int v0, v1;
+  __device__ __forceinline__ int at(int64_t s) const { return s == s0 ? v0 : s == s1 ? v1 : (int)t[s]; }
+};
+
+// BondsCounter::count (bonds_counter.cpp:35-81): pair bonds of an atom of type
+// src at (x, y, z) with its 1nn and counted 2nn; dumbbells carry no bond.
+__device__ double count(const Dims& d, const View& L, int x, int y, int z, int src) {
+  const int ps = species(src);
+  double e = 0.0;
+#pragma unroll
+  for (int b = 0; b < N1; b++) {
+    const int t = L.at(nn1(d, x, y, z, b));
+    if (is_atom_or_v(t)) e += c_e1[species(t) * 5 + ps];
+  }
+#pragma unroll
+  for (int k = 0; k < N2ALL; k++) {
+    if (!((STATUS_2NN >> k) & 1)) continue;
+    const int t = L.at(nn2(d, x, y, z, k));
+    if (is_atom_or_v(t)) e += c_e2[species(t) * 5 + ps];
+  }
+  return e;
+}
+
+// One dumbbell's term of bond::Edumb, emix + etens + ecomp + eb / 2
+// (energy_dumbbell.cpp:29-134, FeX_comp :139-206), on the lattice L.
+__device__ double edumb_term(const Dims& d, const View& L, int x, int y, int z, int type, int o, uint8_t avail) {
+  int n1[N1], n2[N2ALL];
+#pragma unroll
+  for (int b = 0; b < N1; b++) n1[b] = L.at(nn1(d, x, y, z, b));
+#pragma unroll
+  for (int k = 0; k < N2ALL; k++) n2[k] = ((STATUS_2NN >> k) & 1) ? L.at(nn2(d, x, y, z, k)) : 0;
+  double emix = 0, etens = 0, ecomp = 0, eb = 0;
+  const int lo = type & 0xFF;
+  switch (type) {
+    case 0x0102: emix = -0.46; break;
+    case 0x0104: emix = -0.19; break;
+    case 0x0108: emix = 0.39; break;
+    case 0x0202: emix = -0.36; break;
+    case 0x0204: emix = -0.5; break;
+    case 0x0208: emix = -0.11; break;
+    case 0x0404: emix = -0.3; break;
+    case 0x0408: emix = 0.08; break;
+    case 0x0808: emix = 0.41; break;
+    default: break;
+  }
+  if (type == 0x0102 || type == 0x0104 || type == 0x0108) {
+    // FeX: tension of the solute X towards its own jump targets.
+    double c = 0, ebx = 0;
+    const uint8_t comp = avail & c_low[o + 6];
+    for (int b = 0; b < N1; b++) {
+      if (!((comp >> b) & 1)) continue;
+      const int t = n1[b];
+      if (is_dumbbell(t)) {
+        ebx += 0.70;
+      } else if (lo == T_CU) {
+        if (t == T_CU) c += -0.18;
+        else if (t == T_NI) c += -0.35;
+        else if (t == T_MN) c += -0.46;
+      } else if (lo == T_NI) {
+        if (t == T_CU) c += -0.21;
+        else if (t == T_NI) c += -0.32;
+        else if (t == T_MN) c += -0.30;
+      } else {
+        if (t == T_CU) c += 0.28;
+        else if (t == T_NI) c += 0.29;
+        else if (t == T_MN) c += 0.28;
+      }
+    }
+    for (int k = 0; k < N2ALL; k++)
+      if (((STATUS_2NN >> k) & 1) && is_dumbbell(n2[k])) ebx += 0.48;
+    ecomp = c + ebx / 2;
+  } else if (type == 0x0101) {
+    const uint8_t dirs = c_high[o + 6] | c_low[o + 6];
+    for (int b = 0; b < N1; b++) {
+      const int t = n1[b];
+      const bool jump = (dirs >> b) & 1;
+      if (is_dumbbell(t)) eb += 0.70;
+      else if (t == T_CU) etens += jump ? -0.13 : 0.13;
+      else if (t == T_NI) etens += jump ? 0.15 : 0.02;
+      else if (t == T_MN) etens += jump ? 0.02 : -0.36;
+    }
+    // The 2nn status bits select entries of the 1nn list here, as written in
+    // energy_dumbbell.cpp:118-122.
+    for (int k = 0; k < N2ALL; k++)
+      if (((STATUS_2NN >> k) & 1) && is_dumbbell(n1[k])) eb += 0.48;
+  } else if (type > 0x0108) {
+    // Dumbbells of two solutes: 0.70 per 1nn and 0.48 per 2nn dumbbell,
+    // interleaved by neighbour index.
+    double c = 0;
+    for (int b = 0; b < N1; b++) {
+      if (is_dumbbell(n1[b])) c += 0.70;
+      if (b < N2ALL && ((STATUS_2NN >> b) & 1) && is_dumbbell(n2[b])) c += 0.48;
+    }
+    eb += c;
+  }
+  return emix + etens + ecomp + eb / 2;
+}
+
+struct Args {
+  Dims d;
+  const uint16_t* types;
+  const int64_t* sites;
+  const int8_t* orient;
+  int64_t n;
+  double nu, kT;
+  int4* info;      // per dumbbell: x, y, z, type | avail << 16 | (o + 6) << 24
+  double* term;    // per dumbbell: its Edumb term on the unchanged lattice
+  uint8_t* avail;  // out
+  double* rates;   // out, 8 per dumbbell
+};
+
+// Availability masks and Edumb terms of all dumbbells (one thread each).
+__global__ void itl_prep_kernel(Args a) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.n) return;
+  const int64_t s = a.sites[k];
+  const int x = (int)(s % a.d.X), y = (int)((s / a.d.X) % a.d.Y), z = (int)(s / ((int64_t)a.d.X * a.d.Y));
+  const int type = a.types[s];
+  const int o = a.orient[k];
+  const View L{a.types, -1, -1, 0, 0};
+  uint8_t atoms = 0;
+  for (int b = 0; b < N1; b++)
+    if (is_atom(L.at(nn1(a.d, x, y, z, b)))) atoms |= (uint8_t)(1u << b);
+  const uint8_t av = (uint8_t)((c_high[o + 6] | c_low[o + 6]) & atoms);
+  a.avail[k] = av;
+  a.term[k] = edumb_term(a.d, L, x, y, z, type, o, av);
+  a.info[k] = make_int4(x, y, z, type | ((int)av << 16) | ((o + 6) << 24));
+}
+
+// True if (x, y, z) is within the reach of one Edumb term of (cx, cy, cz):
+// |dx| <= 2 (doubled x), |dy| <= 1, |dz| <= 1, periodically.
+__device__ __forceinline__ bool near(const Dims& d, int x, int y, int z, int cx, int cy, int cz) {
+  int dx = x - cx, dy = y - cy, dz = z - cz;
+  dx = dx > d.X / 2 ? dx - d.X : dx < -(d.X / 2) ? dx + d.X : dx;
+  dy = dy > d.Y / 2 ? dy - d.Y : dy < -(d.Y / 2) ? dy + d.Y : dy;
+  dz = dz > d.Z / 2 ? dz - d.Z : dz < -(d.Z / 2) ? dz + d.Z : dz;
+  return dx >= -2 && dx <= 2 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1;
+}
+
+// Rates: eight lanes per dumbbell, lane b for direction b.
+__global__ void itl_rate_kernel(Args a) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t k = g >> 3;
+  const int b = (int)(g & 7);
+  const bool live = k < a.n;
+  const int4 me = live ? a.info[k] : make_int4(0, 0, 0, 0);
+  const int type = me.w & 0xFFFF;
+  const uint8_t av = (uint8_t)((me.w >> 16) & 0xFF);
+  const int o = (me.w >> 24) - 6;
+  double rate = 0.0;
+  if (live && ((av >> b) & 1)) {
+    const int64_t src = site_of(a.d, me.x, me.y, me.z);
+    const int lo = (me.x & 1) ? 0 : -1;
+    const int tx = me.x + ((b & 4) ? 1 : -1), ty = me.y + lo + ((b >> 1) & 1), tz = me.z + lo + (b & 1);
+    const int64_t dst = site_of(a.d, tx, ty, tz);
+    const int dx = wrap(tx, a.d.X), dy = wrap(ty, a.d.Y), dz = wrap(tz, a.d.Z);
+    // orientation::tranAtom (orientation.cc:238-253): X of XY in the high
+    // end's directions, Y in the low end's.
+    const int ghost = ((c_high[o + 6] >> b) & 1) ? (type >> 8) : (type & 0xFF);
+    const int tdst = a.types[dst];
+    // LatticeTypes::diff and combineToInter (lattice_types.cpp:21-48).
+    const int src_after = (type & 0xFF) == ghost ? (type >> 8) : (type >> 8) == ghost ? (type & 0xFF) : type;
+    const int dst_after = tdst <= ghost ? (tdst << 8) | ghost : (ghost << 8) | tdst;
+    const View L0{a.types, -1, -1, 0, 0};
+    const View L1{a.types, src, dst, src_after, dst_after};
+    const double e_src_b = count(a.d, L0, me.x, me.y, me.z, ghost);
+    const double e_des_b = count(a.d, L0, dx, dy, dz, tdst);
+    const double e_src_a = count(a.d, L1, me.x, me.y, me.z, src_after);
+    const double e_des_a = count(a.d, L1, dx, dy, dz, ghost);
+    double sb = 0, sa = 0;
+    for (int64_t j = 0; j < a.n; j++) {
+      const double t = a.term[j];
+      const int4 q = a.info[j];
+      double ta = t;
+      if (near(a.d, q.x, q.y, q.z, me.x, me.y, me.z) || near(a.d, q.x, q.y, q.z, dx, dy, dz)) {
+        const int64_t sj = site_of(a.d, q.x, q.y, q.z);
+        ta = edumb_term(a.d, L1, q.x, q.y, q.z, L1.at(sj), (q.w >> 24) - 6, (uint8_t)((q.w >> 16) & 0xFF));
+      }
+      sb += t;
+      sa += ta;
+    }
+    const double e_before = e_src_b + e_des_b + sb;
+    const double e_after = e_src_a + e_des_a + sa;
+    const double e0 = ghost == T_FE ? 0.32 : ghost == T_CU ? 0.36 : ghost == T_NI ? 0.45 : ghost == T_MN ? 0.31 : 0.0;
+    const double ea = e0 + (e_after - e_before) / 2;
+    rate = a.nu * exp_glibc(-ea / a.kT, c_exp_tab);
+  }
+  // Slot b holds half the rate of the (b / 2)-th orientation direction when
+  // that direction is available (Itl::ratesIndex, itl_list.cpp:67-74).
+  const uint8_t dirs = live ? (uint8_t)(c_high[o + 6] | c_low[o + 6]) : 0;
+  int dsel = 0;
+  for (int i = 0, seen = 0; i < N1; i++)
+    if ((dirs >> i) & 1) {
+      if (seen == (b >> 1)) dsel = i;
+      seen++;
+    }
+  const double rd = __shfl(rate, dsel, 8);
+  if (live) a.rates[8 * k + b] = ((av >> dsel) & 1) ? rd / 2 : 0.0;
+}
+
+}  // namespace itl
+}  // namespace akmc
+
+namespace {
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t n) { return hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)); }
+};
+
+bool valid_single(int t) { return t == 0 || t == 1 || t == 2 || t == 4 || t == 8; }
+bool valid_dumbbell(int t) {
+  const int hi = t >> 8, lo = t & 0xFF;
+  return hi != 0 && valid_single(hi) && valid_single(lo) && hi <= lo && (t >> 16) == 0;
+}
+
+}  // namespace
+
+extern "C" int akmc_itl_rates(const int64_t box[3], const uint16_t* types, int64_t n_itl, const int64_t* sites,
+                              const int8_t* orient, double attempt_freq, double temperature, int device,
+                              uint8_t* avail, double* rates) {
+  if (!box || !types || n_itl < 0 || (n_itl > 0 && (!sites || !orient || !avail || !rates)))
+    return fail(AKMC_ERR_INVALID, "null argument");
+  for (int i = 0; i < 3; i++)
+    if (box[i] < 3 || box[i] > (1 << 20)) return fail(AKMC_ERR_INVALID, "box dimension %d is %lld (3 .. 2^20)", i, (long long)box[i]);
+  if (!(temperature > 0) || !(attempt_freq >= 0) || !std::isfinite(attempt_freq))
+    return fail(AKMC_ERR_INVALID, "bad temperature or attempt frequency");
+  const int64_t X = 2 * box[0], nsites = X * box[1] * box[2];
+  for (int64_t s = 0; s < nsites; s++)
+    if (!valid_single(types[s]) && !valid_dumbbell(types[s]))
+      return fail(AKMC_ERR_INVALID, "site %lld holds 0x%x, not a lattice type", (long long)s, (unsigned)types[s]);
+  for (int64_t k = 0; k < n_itl; k++) {
+    if (sites[k] < 0 || sites[k] >= nsites) return fail(AKMC_ERR_INVALID, "dumbbell %lld: site out of the box", (long long)k);
+    if (k > 0 && sites[k] <= sites[k - 1])
+      return fail(AKMC_ERR_INVALID, "dumbbell sites must be strictly ascending (at %lld)", (long long)k);
+    if (!valid_dumbbell(types[sites[k]]))
+      return fail(AKMC_ERR_INVALID, "dumbbell %lld: site %lld holds 0x%x, not a dumbbell", (long long)k,
+                  (long long)sites[k], (unsigned)types[sites[k]]);
+    if (orient[k] == 0 || orient[k] < -6 || orient[k] > 6)
+      return fail(AKMC_ERR_INVALID, "dumbbell %lld: orientation %d", (long long)k, (int)orient[k]);
+  }
+  if (n_itl == 0) return AKMC_OK;
+  HIP_TRY(hipSetDevice(device));
+  DevBuf<uint16_t> d_types;
+  DevBuf<int64_t> d_sites;
+  DevBuf<int8_t> d_orient;
+  DevBuf<int4> d_info;
+  DevBuf<double> d_term, d_rates;
+  DevBuf<uint8_t> d_avail;
+  HIP_TRY(d_types.alloc(nsites));
+  HIP_TRY(d_sites.alloc(n_itl));
+  HIP_TRY(d_orient.alloc(n_itl));
+  HIP_TRY(d_info.alloc(n_itl));
+  HIP_TRY(d_term.alloc(n_itl));
+  HIP_TRY(d_avail.alloc(n_itl));
+  HIP_TRY(d_rates.alloc(8 * n_itl));
+  HIP_TRY(hipMemcpy(d_types.p, types, nsites * sizeof(uint16_t), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d_sites.p, sites, n_itl * sizeof(int64_t), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d_orient.p, orient, n_itl * sizeof(int8_t), hipMemcpyHostToDevice));
+  akmc::itl::Args a;
+  a.d = akmc::itl::Dims{(int32_t)X, (int32_t)box[1], (int32_t)box[2]};
+  a.types = d_types.p;
+  a.sites = d_sites.p;
+  a.orient = d_orient.p;
+  a.n = n_itl;
+  a.nu = attempt_freq;
+  a.kT = AKMC_KB * temperature;  // RatesSolver::arrhenius: kB * T first
+  a.info = d_info.p;
+  a.term = d_term.p;
+  a.avail = d_avail.p;
+  a.rates = d_rates.p;
+  const int threads = 256;
+  hipLaunchKernelGGL(akmc::itl::itl_prep_kernel, dim3((unsigned)((n_itl + threads - 1) / threads)), dim3(threads), 0,
+                     nullptr, a);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(akmc::itl::itl_rate_kernel, dim3((unsigned)((8 * n_itl + threads - 1) / threads)), dim3(threads),
+                     0, nullptr, a);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(avail, d_avail.p, n_itl, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(rates, d_rates.p, 8 * n_itl * sizeof(double), hipMemcpyDeviceToHost));
+  return AKMC_OK;
+}

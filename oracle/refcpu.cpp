@@ -601,3 +601,404 @@ double ref_rate(void* h, int64_t real_src, int dir) {
 }
 
 }  // extern "C"
+
+// ============================================================ dumbbells (SIA)
+// The abvi self-interstitial (dumbbell) model, restated on a typed periodic
+// lattice: one LatticeTypes value per real site (the reference's array
+// lattice, src/lattice/lattices_list.h:345), dumbbells listed in ascending
+// site order.  ItlList::mp is a std::map keyed by lattice id
+// (models/abvi/defect/itl_list.h), and ascending local id on the
+// ghost-extended grid is ascending real index x + 2N(y + N z).  The
+// reference's hash-set lattice no longer allocates that array
+// (lattices_list.cpp:13), so this part of the reference cannot run as
+// shipped; its semantics are those of the sources cited below.
+namespace refitl {
+
+using refcpu::bond1;
+using refcpu::bond2;
+using refcpu::make_bond;
+using refcpu::Fe;
+using refcpu::Cu;
+using refcpu::Ni;
+using refcpu::Mn;
+
+// LatticeTypes predicates and combinations (src/lattice/lattice_types.h:48-95,
+// lattice_types.cpp:21-48).
+static inline bool is_dumbbell(int t) { return t > 0x0008; }
+static inline bool is_atom(int t) { return t <= 0x0008 && t != 0; }
+static inline int high_end(int t) { return t >> 8; }
+static inline int low_end(int t) { return t & 0xFF; }
+static inline int combine_to_inter(int a, int b) {
+  if (a > b) { const int t = a; a = b; b = t; }
+  return (a << 8) | b;
+}
+static inline int diff(int t, int b) {
+  if (low_end(t) == b) return high_end(t);
+  if (high_end(t) == b) return low_end(t);
+  return t;
+}
+
+// orientation (models/abvi/orientation.h:12-30; orientation.cc:7-21 jump
+// table, :143-236 direction masks).  o in [-6, 6], 0 = unknown.
+static const char kJump[13][8] = {
+    {'_', 'Y', '_', 'Y', 'X', '_', 'X', '_'},  // -6
+    {'Y', '_', 'Y', '_', '_', 'X', '_', 'X'},  // -5
+    {'_', 'Y', 'X', '_', '_', 'Y', 'X', '_'},  // -4
+    {'Y', '_', '_', 'X', 'Y', '_', '_', 'X'},  // -3
+    {'_', '_', 'Y', 'Y', 'X', 'X', '_', '_'},  // -2
+    {'Y', 'Y', '_', '_', '_', '_', 'X', 'X'},  // -1
+    {'_', '_', '_', '_', '_', '_', '_', '_'},  // 0
+    {'X', 'X', '_', '_', '_', '_', 'Y', 'Y'},  // 1
+    {'_', '_', 'X', 'X', 'Y', 'Y', '_', '_'},  // 2
+    {'X', '_', '_', 'Y', 'X', '_', '_', 'Y'},  // 3
+    {'_', 'X', 'Y', '_', '_', 'X', 'Y', '_'},  // 4
+    {'X', '_', 'X', '_', '_', 'Y', '_', 'Y'},  // 5
+    {'_', 'X', '_', 'X', 'Y', '_', 'Y', '_'},  // 6
+};
+static uint8_t avail_all(int o) {
+  switch (o) {
+  case 1: case -1: return 0xC3;
+  case 2: case -2: return 0x3C;
+  case 3: case -3: return 0x99;
+  case 4: case -4: return 0x66;
+  case 5: case -5: return 0xA5;
+  case 6: case -6: return 0x5A;
+  default: return 0;
+  }
+}
+static uint8_t avail_high(int o) {
+  switch (o) {
+  case 1: return 0x03; case -1: return 0xC0;
+  case 2: return 0x0C; case -2: return 0x30;
+  case 3: return 0x11; case -3: return 0x88;
+  case 4: return 0x22; case -4: return 0x44;
+  case 5: return 0x05; case -5: return 0xA0;
+  case 6: return 0x0A; case -6: return 0x50;
+  default: return 0;
+  }
+}
+static uint8_t avail_low(int o) {
+  switch (o) {
+  case 1: return 0xC0; case -1: return 0x03;
+  case 2: return 0x30; case -2: return 0x0C;
+  case 3: return 0x88; case -3: return 0x11;
+  case 4: return 0x44; case -4: return 0x22;
+  case 5: return 0xA0; case -5: return 0x05;
+  case 6: return 0x50; case -6: return 0x0A;
+  default: return 0;
+  }
+}
+// orientation::tranAtom (orientation.cc:238-253): the atom of dumbbell type t
+// that jumps towards 1nn tag b; t itself when neither may.
+static int tran_atom(int o, int t, int b) {
+  switch (kJump[o + 6][b]) {
+  case 'X': return high_end(t);
+  case 'Y': return low_end(t);
+  default: return t;
+  }
+}
+// orientation::random (orientation.h:38-42) from one 32-bit draw r:
+// rand32(0, 12) = r % 12, then 1..12 -> +r/2 for even, -(r+1)/2 for odd.
+static int orientation_from_draw(uint32_t r) {
+  const uint32_t v = r % 12u + 1u;
+  return v % 2 == 0 ? (int)(v / 2) : -(int)((v + 1) / 2);
+}
+
+// Itl (models/abvi/defect/itl_list.cpp).
+// availTranDirs (:9-23): orientation directions whose 1nn is a single atom.
+static uint8_t itl_avail(int o, uint8_t status_1nn, const int nb[8]) {
+  uint8_t atoms = 0;
+  for (int b = 0; b < 8; b++)
+    if (((status_1nn >> b) & 1) && is_atom(nb[b])) atoms |= (uint8_t)(1 << b);
+  return (uint8_t)(avail_all(o) & atoms);
+}
+// ratesIndex (:67-74): two slots per orientation direction, in direction order.
+static int rates_index(int b, uint8_t dirs, bool up) {
+  int below = 0;
+  for (int k = 0; k < b; k++) below += (dirs >> k) & 1;
+  return up ? 2 * below + 1 : 2 * below;
+}
+// get1nnIdByRatesIndex (:76-91).
+static int first_nn_by_rates_index(int idx, uint8_t dirs) {
+  idx /= 2;
+  int i = 0;
+  for (int b = 0; b < 8; b++)
+    if ((dirs >> b) & 1) {
+      if (i == idx) return b;
+      i++;
+    }
+  return 0;
+}
+// getRatesStatus (:52-65).
+static uint8_t rates_status(int o, uint8_t avail) {
+  uint8_t st = 0;
+  const uint8_t dirs = avail_all(o);
+  int i = 0;
+  for (int b = 0; b < 8; b++)
+    if ((dirs >> b) & 1) {
+      if ((avail >> b) & 1) st |= (uint8_t)((1 << i) | (1 << (i + 1)));
+      i += 2;
+    }
+  return st;
+}
+
+// The typed periodic lattice.
+struct Typed {
+  int64_t X, Y, Z;     // 2 nx, ny, nz
+  uint16_t* t;         // one LatticeTypes value per real site
+  int64_t wrap_site(int64_t s, int dx, int dy, int dz) const {
+    int64_t x = s % X, y = (s / X) % Y, z = s / (X * Y);
+    x = ((x + dx) % X + X) % X;
+    y = ((y + dy) % Y + Y) % Y;
+    z = ((z + dz) % Z + Z) % Z;
+    return x + X * (y + Y * z);
+  }
+  // get1nn / get2nn order (src/lattice/period_lattice_list.cpp:20-44,223-232).
+  int64_t nn1(int64_t s, int b) const {
+    const int par = (int)((s % X) & 1);
+    const int lo = par ? 0 : -1;
+    return wrap_site(s, (b & 4) ? 1 : -1, lo + ((b >> 1) & 1), lo + (b & 1));
+  }
+  int64_t nn2(int64_t s, int k) const {
+    static const int o[6][3] = {{-2, 0, 0}, {0, -1, 0}, {0, 0, -1}, {0, 0, 1}, {0, 1, 0}, {2, 0, 0}};
+    return wrap_site(s, o[k][0], o[k][1], o[k][2]);
+  }
+};
+static const uint8_t kStatus1 = 0xFF, kStatus2 = 0x2F;  // period_lattice_list.cpp:10-18
+
+// BondsCounter::count (models/abvi/rate/bonds/bonds_counter.cpp:35-81) on the
+// typed lattice: dumbbell neighbours carry no pair bond.
+static double count(const Typed& L, int64_t s, int src) {
+  double e = 0.0;
+  for (int b = 0; b < 8; b++)
+    if ((kStatus1 >> b) & 1) {
+      const int t = L.t[L.nn1(s, b)];
+      if (is_atom(t) || t == 0) e += bond1(make_bond(t, src));
+    }
+  for (int b = 0; b < 6; b++)
+    if ((kStatus2 >> b) & 1) {
+      const int t = L.t[L.nn2(s, b)];
+      if (is_atom(t) || t == 0) e += bond2(make_bond(t, src));
+    }
+  return e;
+}
+
+struct Itl {
+  int64_t site;
+  int o;
+  uint8_t avail;
+  double rates[8];
+};
+
+// bond::FeX_comp (models/abvi/rate/bonds/energy_dumbbell.cpp:139-206).
+static double fex_comp(const Typed& L, int type, const Itl& itl, const int n1[8], const int n2[6]) {
+  double ecomp = 0, eb = 0;
+  const uint8_t comp = itl.avail & avail_low(itl.o);
+  const int x = low_end(type);
+  for (int b = 0; b < 8; b++) {
+    if (!((comp >> b) & 1)) continue;
+    const int t = n1[b];
+    switch (x) {
+    case Cu:
+      if (is_dumbbell(t)) eb += 0.70;
+      else if (t == Cu) ecomp += -0.18;
+      else if (t == Ni) ecomp += -0.35;
+      else if (t == Mn) ecomp += -0.46;
+      break;
+    case Ni:
+      if (is_dumbbell(t)) eb += 0.70;
+      else if (t == Cu) ecomp += -0.21;
+      else if (t == Ni) ecomp += -0.32;
+      else if (t == Mn) ecomp += -0.30;
+      break;
+    case Mn:
+      if (is_dumbbell(t)) eb += 0.70;
+      else if (t == Cu) ecomp += 0.28;
+      else if (t == Ni) ecomp += 0.29;
+      else if (t == Mn) ecomp += 0.28;
+      break;
+    default:
+      break;
+    }
+  }
+  (void)L;
+  for (int b = 0; b < 8; b++)
+    if (b < 6 && ((kStatus2 >> b) & 1) && is_dumbbell(n2[b])) eb += 0.48;
+  return ecomp + eb / 2;
+}
+
+// bond::Edumb (energy_dumbbell.cpp:7-137): the dumbbell energy of the whole
+// list, a left fold in list order.
+static double edumb(const Typed& L, const std::vector<Itl>& list) {
+  double edumb = 0;
+  for (const Itl& itl : list) {
+    const int type = L.t[itl.site];
+    int n1[8], n2[6];
+    for (int b = 0; b < 8; b++) n1[b] = L.t[L.nn1(itl.site, b)];
+    for (int b = 0; b < 6; b++) n2[b] = L.t[L.nn2(itl.site, b)];
+    auto contribute = [&]() {
+      double c = 0;
+      for (int b = 0; b < 8; b++) {
+        if (((kStatus1 >> b) & 1) && is_dumbbell(n1[b])) c += 0.70;
+        if (b < 6 && ((kStatus2 >> b) & 1) && is_dumbbell(n2[b])) c += 0.48;
+      }
+      return c;
+    };
+    double emix = 0, etens = 0, ecomp = 0, eb = 0;
+    switch (type) {
+    case 0x0102: emix = -0.46; ecomp = fex_comp(L, type, itl, n1, n2); break;
+    case 0x0202: emix = -0.36; eb += contribute(); break;
+    case 0x0104: emix = -0.19; ecomp = fex_comp(L, type, itl, n1, n2); break;
+    case 0x0204: emix = -0.5; eb += contribute(); break;
+    case 0x0404: emix = -0.3; eb += contribute(); break;
+    case 0x0108: emix = 0.39; ecomp = fex_comp(L, type, itl, n1, n2); break;
+    case 0x0208: emix = -0.11; eb += contribute(); break;
+    case 0x0408: emix = 0.08; eb += contribute(); break;
+    case 0x0808: emix = 0.41; eb += contribute(); break;
+    case 0x0101: {
+      etens = 0;
+      const uint8_t dirs = avail_all(itl.o);
+      for (int b = 0; b < 8; b++) {
+        if (!((kStatus1 >> b) & 1)) continue;
+        const int t = n1[b];
+        if (!((dirs >> b) & 1)) {
+          if (is_dumbbell(t)) eb += 0.70;
+          else if (t == Cu) etens += 0.13;
+          else if (t == Ni) etens += 0.02;
+          else if (t == Mn) etens += -0.36;
+        } else {
+          if (is_dumbbell(t)) eb += 0.70;
+          else if (t == Cu) etens += -0.13;
+          else if (t == Ni) etens += 0.15;
+          else if (t == Mn) etens += 0.02;
+        }
+      }
+      // The reference tests the 1nn list under the 2nn status bits here
+      // (energy_dumbbell.cpp:118-122); kept as written.
+      for (int b = 0; b < 8; b++)
+        if (((kStatus2 >> b) & 1) && is_dumbbell(n1[b])) eb += 0.48;
+      break;
+    }
+    default: emix = 0; etens = 0; ecomp = 0; eb = 0; break;
+    }
+    edumb += emix + etens + ecomp + eb / 2;
+  }
+  return edumb;
+}
+
+// ItlRatesSolver::e0 (models/abvi/rate/itl_rates_solver.cpp:16-29).
+static double e0_itl(int ghost) {
+  switch (ghost) {
+  case Fe: return 0.32;
+  case Cu: return 0.36;
+  case Ni: return 0.45;
+  case Mn: return 0.31;
+  default: return 0;
+  }
+}
+
+// ItlRatesSolver::deltaE (itl_rates_solver.cpp:31-80): both sites' bond
+// energies and the whole list's dumbbell energy, before and after the
+// jumping atom leaves the dumbbell at src and joins the atom at dst.
+static double delta_e(Typed& L, const std::vector<Itl>& list, int64_t src, int64_t dst, int ghost) {
+  double e_before = 0;
+  {
+    const double e_src = count(L, src, ghost);
+    const double e_des = count(L, dst, L.t[dst]);
+    const double e_d = edumb(L, list);
+    e_before = e_src + e_des + e_d;
+  }
+  const uint16_t cs = L.t[src], cd = L.t[dst];
+  L.t[src] = (uint16_t)diff(cs, ghost);
+  L.t[dst] = (uint16_t)combine_to_inter(cd, ghost);
+  double e_after = 0;
+  {
+    const double e_src = count(L, src, L.t[src]);
+    const double e_des = count(L, dst, ghost);
+    const double e_d = edumb(L, list);
+    e_after = e_src + e_des + e_d;
+  }
+  L.t[src] = cs;
+  L.t[dst] = cd;
+  return e_after - e_before;
+}
+
+// Itl::beforeRatesUpdate for every dumbbell, then Itl::updateRates for every
+// dumbbell with RatesSolver::rate (rates_solver.cpp:13-26) as the callback
+// (itl_list.cpp:25-50): each available direction's rate, halved, in the
+// direction's two slots.
+static void rates(Typed& L, std::vector<Itl>& list, double nu, double T) {
+  for (Itl& itl : list) {
+    int nb[8];
+    for (int b = 0; b < 8; b++) nb[b] = L.t[L.nn1(itl.site, b)];
+    for (double& r : itl.rates) r = 0;
+    itl.avail = itl_avail(itl.o, kStatus1, nb);
+  }
+  for (Itl& itl : list) {
+    const uint8_t dirs = avail_all(itl.o);
+    for (int b = 0; b < 8; b++) {
+      if (!((itl.avail >> b) & 1)) continue;
+      const int64_t dst = L.nn1(itl.site, b);
+      const int ghost = tran_atom(itl.o, L.t[itl.site], b);
+      const double ea = e0_itl(ghost) + delta_e(L, list, itl.site, dst, ghost) / 2;
+      const double rate = nu * exp(-ea / (refcpu::kBoltzmann * T));
+      itl.rates[rates_index(b, dirs, false)] = rate / 2;
+      itl.rates[rates_index(b, dirs, true)] = rate / 2;
+    }
+  }
+}
+
+}  // namespace refitl
+
+extern "C" {
+
+// Orientation tables: which 0 = all, 1 = high end (X of XY), 2 = low end.
+int ref_ori_avail(int o, int which) {
+  return which == 1 ? refitl::avail_high(o) : which == 2 ? refitl::avail_low(o) : refitl::avail_all(o);
+}
+int ref_ori_jump(int o, int b) { return refitl::kJump[o + 6][b]; }
+int ref_ori_tran_atom(int o, int type, int b) { return refitl::tran_atom(o, type, b); }
+int ref_ori_from_draw(uint32_t r) { return refitl::orientation_from_draw(r); }
+int ref_itl_avail(int o, int status, const int* nb) { return refitl::itl_avail(o, (uint8_t)status, nb); }
+int ref_itl_rates_index(int b, int dirs, int up) { return refitl::rates_index(b, (uint8_t)dirs, up != 0); }
+int ref_itl_first_nn(int idx, int dirs) { return refitl::first_nn_by_rates_index(idx, (uint8_t)dirs); }
+int ref_itl_rates_status(int o, int avail) { return refitl::rates_status(o, (uint8_t)avail); }
+// Itl::updateRates with a given value per direction in place of the rate
+// callback (the unit tests' test_rates_updater, itl_rates_test.cpp).
+void ref_itl_update_rates(int o, int avail, const double* cb, double* rates) {
+  const uint8_t dirs = refitl::avail_all(o);
+  for (int b = 0; b < 8; b++)
+    if ((avail >> b) & 1) {
+      rates[refitl::rates_index(b, dirs, false)] = cb[b] / 2;
+      rates[refitl::rates_index(b, dirs, true)] = cb[b] / 2;
+    }
+}
+int ref_combine_to_inter(int a, int b) { return refitl::combine_to_inter(a, b); }
+int ref_type_diff(int t, int b) { return refitl::diff(t, b); }
+
+// Dumbbell energy of a typed periodic lattice (box in cells; sites ascending).
+double ref_edumb(const int64_t* box, uint16_t* types, int64_t n, const int64_t* sites, const int8_t* ori,
+                 const uint8_t* avail) {
+  refitl::Typed L{2 * box[0], box[1], box[2], types};
+  std::vector<refitl::Itl> list((size_t)n);
+  for (int64_t k = 0; k < n; k++) list[k] = refitl::Itl{sites[k], ori[k], avail[k], {0}};
+  return refitl::edumb(L, list);
+}
+double ref_count_typed(const int64_t* box, uint16_t* types, int64_t site, int src) {
+  refitl::Typed L{2 * box[0], box[1], box[2], types};
+  return refitl::count(L, site, src);
+}
+// All dumbbells' availability masks and rate slots.
+void ref_itl_rates(const int64_t* box, uint16_t* types, int64_t n, const int64_t* sites, const int8_t* ori,
+                   double nu, double T, uint8_t* avail, double* rates) {
+  refitl::Typed L{2 * box[0], box[1], box[2], types};
+  std::vector<refitl::Itl> list((size_t)n);
+  for (int64_t k = 0; k < n; k++) list[k] = refitl::Itl{sites[k], ori[k], 0, {0}};
+  refitl::rates(L, list, nu, T);
+  for (int64_t k = 0; k < n; k++) {
+    avail[k] = list[k].avail;
+    for (int b = 0; b < 8; b++) rates[8 * k + b] = list[k].rates[b];
+  }
+}
+
+}  // extern "C"

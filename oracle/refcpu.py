@@ -209,6 +209,9 @@ def refsrc():
     L = ctypes.CDLL(_REF_SRC)
     L.refsrc_random_atoms_type.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint, ctypes.c_int64]
     L.refsrc_make_bond.argtypes = [ctypes.c_int, ctypes.c_int]
+    for f in ("refsrc_combine_to_inter", "refsrc_diff"):
+        getattr(L, f).argtypes = [ctypes.c_int, ctypes.c_int]
+    L.refsrc_type_info.argtypes = [ctypes.c_int]
     return L
 
 
@@ -228,3 +231,98 @@ def streams(seed_time_inc, n):
     l = np.zeros(n, dtype=np.float64)
     L.ref_streams(ctypes.c_int64(seed_time_inc), ctypes.c_int64(n), u.ctypes.data, l.ctypes.data)
     return u, l
+
+
+# ------------------------------------------------------------------ dumbbells
+class Itl:
+    """The oracle's restatement of the abvi dumbbell model (orientation.cc,
+    defect/itl_list.cpp, rate/itl_rates_solver.cpp, rate/bonds/energy_dumbbell.cpp)."""
+
+    def __init__(self):
+        L = lib()
+        for f, args in (("ref_ori_avail", [ctypes.c_int, ctypes.c_int]),
+                        ("ref_ori_jump", [ctypes.c_int, ctypes.c_int]),
+                        ("ref_ori_tran_atom", [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+                        ("ref_ori_from_draw", [ctypes.c_uint32]),
+                        ("ref_itl_avail", [ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+                        ("ref_itl_rates_index", [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+                        ("ref_itl_first_nn", [ctypes.c_int, ctypes.c_int]),
+                        ("ref_itl_rates_status", [ctypes.c_int, ctypes.c_int]),
+                        ("ref_combine_to_inter", [ctypes.c_int, ctypes.c_int]),
+                        ("ref_type_diff", [ctypes.c_int, ctypes.c_int])):
+            getattr(L, f).argtypes = args
+            getattr(L, f).restype = ctypes.c_int
+        L.ref_itl_update_rates.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.ref_edumb.argtypes = [ctypes.c_void_p] * 2 + [ctypes.c_int64] + [ctypes.c_void_p] * 3
+        L.ref_edumb.restype = ctypes.c_double
+        L.ref_count_typed.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int]
+        L.ref_count_typed.restype = ctypes.c_double
+        L.ref_itl_rates.argtypes = ([ctypes.c_void_p] * 2 + [ctypes.c_int64] + [ctypes.c_void_p] * 2 +
+                                    [ctypes.c_double, ctypes.c_double] + [ctypes.c_void_p] * 2)
+        self.L = L
+
+    def avail(self, o, which=0):
+        return self.L.ref_ori_avail(o, which)
+
+    def jump(self, o, b):
+        return chr(self.L.ref_ori_jump(o, b))
+
+    def tran_atom(self, o, t, b):
+        return self.L.ref_ori_tran_atom(o, t, b)
+
+    def from_draw(self, r):
+        return self.L.ref_ori_from_draw(r)
+
+    def itl_avail(self, o, status, nb):
+        a = (ctypes.c_int * 8)(*nb)
+        return self.L.ref_itl_avail(o, status, a)
+
+    def rates_index(self, b, dirs, up):
+        return self.L.ref_itl_rates_index(b, dirs, int(up))
+
+    def first_nn(self, idx, dirs):
+        return self.L.ref_itl_first_nn(idx, dirs)
+
+    def rates_status(self, o, avail):
+        return self.L.ref_itl_rates_status(o, avail)
+
+    def update_rates(self, o, avail, cb, rates):
+        cb = np.ascontiguousarray(cb, dtype=np.float64)
+        r = np.ascontiguousarray(rates, dtype=np.float64).copy()
+        self.L.ref_itl_update_rates(o, avail, cb.ctypes.data, r.ctypes.data)
+        return r
+
+    def combine(self, a, b):
+        return self.L.ref_combine_to_inter(a, b)
+
+    def diff(self, t, b):
+        return self.L.ref_type_diff(t, b)
+
+    @staticmethod
+    def _box(box):
+        return np.ascontiguousarray(box, dtype=np.int64)
+
+    def edumb(self, box, types, sites, ori, avail):
+        b = self._box(box)
+        t = np.ascontiguousarray(types, dtype=np.uint16).copy()
+        s = np.ascontiguousarray(sites, dtype=np.int64)
+        o = np.ascontiguousarray(ori, dtype=np.int8)
+        a = np.ascontiguousarray(avail, dtype=np.uint8)
+        return self.L.ref_edumb(b.ctypes.data, t.ctypes.data, s.size, s.ctypes.data, o.ctypes.data, a.ctypes.data)
+
+    def count(self, box, types, site, src):
+        b = self._box(box)
+        t = np.ascontiguousarray(types, dtype=np.uint16).copy()
+        return self.L.ref_count_typed(b.ctypes.data, t.ctypes.data, int(site), int(src))
+
+    def rates(self, box, types, sites, ori, nu, T):
+        """(avail[n], rates[n, 8]) of every listed dumbbell."""
+        b = self._box(box)
+        t = np.ascontiguousarray(types, dtype=np.uint16).copy()
+        s = np.ascontiguousarray(sites, dtype=np.int64)
+        o = np.ascontiguousarray(ori, dtype=np.int8)
+        av = np.zeros(s.size, dtype=np.uint8)
+        r = np.zeros((s.size, 8), dtype=np.float64)
+        self.L.ref_itl_rates(b.ctypes.data, t.ctypes.data, s.size, s.ctypes.data, o.ctypes.data,
+                             float(nu), float(T), av.ctypes.data, r.ctypes.data)
+        return av, r

@@ -1,0 +1,76 @@
+"""Dumbbell (self-interstitial) rates on the GPU (akmc_itl_rates) against the
+oracle's restatement of the reference's model, bit for bit."""
+import math
+
+import numpy as np
+import pytest
+
+import akmc
+from oracle import refcpu
+
+pytestmark = pytest.mark.gpu
+
+DUMBBELLS = [0x0101, 0x0102, 0x0104, 0x0108, 0x0202, 0x0204, 0x0208, 0x0404, 0x0408, 0x0808]
+
+
+def typed_lattice(box, seed, f_itl, solutes=(0.01, 0.02, 0.01), f_v=2e-4):
+    """Fe with Cu, Ni, Mn atoms, vacancies and dumbbells at random sites."""
+    rng = np.random.default_rng(seed)
+    n = 2 * box[0] * box[1] * box[2]
+    u = rng.random(n)
+    t = np.full(n, 1, dtype=np.uint16)
+    c = np.cumsum([solutes[0], solutes[1], solutes[2], f_v])
+    t[u < c[3]] = 0
+    t[u < c[2]] = 8
+    t[u < c[1]] = 4
+    t[u < c[0]] = 2
+    k = max(1, int(round(f_itl * n)))
+    sites = np.sort(rng.choice(n, size=k, replace=False)).astype(np.int64)
+    t[sites] = rng.choice(np.array(DUMBBELLS, dtype=np.uint16), size=k)
+    orient = rng.choice(np.array([-6, -5, -4, -3, -2, -1, 1, 2, 3, 4, 5, 6], dtype=np.int8), size=k)
+    return t, sites, orient
+
+
+def _same(a, b):
+    return np.array_equal(np.asarray(a, np.float64).view(np.uint64), np.asarray(b, np.float64).view(np.uint64))
+
+
+def _check(box, t, sites, orient, nu=6e12, T=573.0):
+    print("itl case: box", box, "dumbbells", len(sites), flush=True)
+    av_g, r_g = akmc.itl_rates(box, t, sites, orient, nu, T)
+    av_o, r_o = refcpu.Itl().rates(box, t, sites, orient, nu, T)
+    assert np.array_equal(av_g, av_o)
+    assert _same(r_g, r_o), np.argwhere(r_g != r_o)[:5]
+    return av_g, r_g
+
+
+def test_fefe_in_pure_iron_closed_form():
+    box = (5, 4, 3)
+    t = np.full(2 * 5 * 4 * 3, 1, dtype=np.uint16)
+    s = 2 + 10 * (1 + 4 * 1)
+    t[s] = 0x0101
+    for o in (1, -2, 3, -4, 5, -6):
+        av, r = _check(box, t, [s], [o])
+        assert list(r[0]) == [6e12 * math.exp(-0.32 / (8.617330350e-5 * 573.0)) / 2] * 8
+
+
+@pytest.mark.parametrize("box,seed,f_itl", [((3, 3, 3), 1, 0.08), ((4, 5, 6), 2, 0.05), ((8, 8, 8), 3, 0.03),
+                                            ((6, 4, 9), 4, 0.10)])
+def test_random_lattices_bit_exact(box, seed, f_itl):
+    t, sites, orient = typed_lattice(box, seed, f_itl, solutes=(0.05, 0.05, 0.05), f_v=0.02)
+    av, r = _check(box, t, sites, orient, T=500.0 + 10 * seed)
+    assert (r > 0).sum() > 0
+
+
+def test_job_sized_lattice():
+    # A serving job's box (L = 64, 524,288 sites) with 1e-4 dumbbells.
+    box = (64, 64, 64)
+    t, sites, orient = typed_lattice(box, 11, 1e-4)
+    _check(box, t, sites, orient, T=650.0)
+
+
+def test_many_dumbbells():
+    box = (16, 16, 16)
+    t, sites, orient = typed_lattice(box, 12, 0.02)
+    assert len(sites) > 150
+    _check(box, t, sites, orient)

@@ -1,0 +1,290 @@
+"""The oracle's dumbbell (self-interstitial) model against the reference's own
+unit tests (tests/unit-abvi/orientation_test.cpp, itl_test.cpp,
+defect/itl_rates_test.cpp), the reference's compiled lattice_types.cpp where
+it is built (oracle/_ref), and hand-derived energies of small configurations
+(bond::Edumb, ItlRatesSolver::deltaE)."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import refcpu
+
+V, FE, CU, NI, MN = 0, 1, 2, 4, 8
+FEFE, FECU, FENI, FEMN, CUCU = 0x0101, 0x0102, 0x0104, 0x0108, 0x0202
+KB = 8.617330350e-5
+S_110, R_110, S_011 = 1, -1, 3
+DUMBBELLS = (FEFE, FECU, FENI, FEMN, CUCU, 0x0204, 0x0208, 0x0404, 0x0408, 0x0808)
+
+
+@pytest.fixture(scope="module")
+def itl():
+    return refcpu.Itl()
+
+
+def test_orientation_masks(itl):
+    # orientation_availTransDirsHighLow_test
+    for i in range(-6, 6):
+        assert itl.avail(i, 2) | itl.avail(i, 1) == itl.avail(i, 0)
+    for i in range(1, 6):
+        assert itl.avail(i, 1) == itl.avail(-i, 2)
+        assert itl.avail(i, 2) == itl.avail(-i, 1)
+        assert itl.avail(i, 2) | itl.avail(-i, 2) == itl.avail(i, 0)
+        assert itl.avail(i, 1) | itl.avail(-i, 1) == itl.avail(i, 0)
+    # every orientation: four directions, two opposite pairs (b and 7 - b)
+    for i in list(range(-6, 0)) + list(range(1, 7)):
+        m = itl.avail(i, 0)
+        assert bin(m).count("1") == 4
+        assert all(((m >> b) & 1) == ((m >> (7 - b)) & 1) for b in range(8))
+        assert itl.avail(i, 1) & itl.avail(i, 2) == 0
+    assert itl.avail(0, 0) == itl.avail(0, 1) == itl.avail(0, 2) == 0
+
+
+def test_jump_table_matches_masks(itl):
+    # The jump table (orientation.cc:7-21) names X exactly where the high-end
+    # mask is set and Y where the low-end mask is.
+    for o in range(-6, 7):
+        for b in range(8):
+            j = itl.jump(o, b)
+            assert (j == "X") == bool((itl.avail(o, 1) >> b) & 1)
+            assert (j == "Y") == bool((itl.avail(o, 2) >> b) & 1)
+
+
+def test_tran_atom(itl):
+    # orientation_availTransDirs_test
+    for b, want in ((0, FE), (1, FE), (6, CU), (7, CU)):
+        assert itl.tran_atom(S_110, FECU, b) == want
+    for b, want in ((0, CU), (1, CU), (6, FE), (7, FE)):
+        assert itl.tran_atom(R_110, FECU, b) == want
+    for b in (2, 3, 4, 5):
+        assert itl.tran_atom(S_110, FECU, b) == FECU  # no jump: the type itself
+
+
+def test_random_orientation(itl):
+    # orientation_random_test, and the draw -> orientation map is onto all 12
+    got = [itl.from_draw(r) for r in range(24)]
+    assert all(o != 0 and -6 <= o <= 6 for o in got)
+    assert sorted(set(got)) == [-6, -5, -4, -3, -2, -1, 1, 2, 3, 4, 5, 6]
+    assert got[:4] == [-1, 1, -2, 2]
+
+
+def test_itl_avail_dirs(itl):
+    # itl_availTranDirs_test
+    nb = [FE] * 8
+    assert itl.itl_avail(S_110, 0xFF, nb) == 0xC3
+    nb[0] = FEFE
+    assert itl.itl_avail(S_110, 0xFF, nb) == 0xC2
+    nb[0] = V
+    assert itl.itl_avail(S_110, 0xFF, nb) == 0xC2
+    nb[0] = FE
+    nb[3] = FECU
+    assert itl.itl_avail(S_110, 0xFF, nb) == 0xC3
+    # itl_availTranDirs_status_test
+    nb = [FE, FECU, FE, FE, FE, FE, FECU, FE]
+    for status, want in ((0xFF, 0x81), (0xF0, 0x80), (0xF3, 0x81), (0xFC, 0x80), (0x0C, 0x00)):
+        assert itl.itl_avail(S_110, status, nb) == want
+
+
+def test_rates_index(itl):
+    # itl_ratesIndex_test
+    cases = [(0, 0xC3, 0, 0), (0, 0xC3, 1, 1), (1, 0xC3, 0, 2), (1, 0xC3, 1, 3), (2, 0xC3, 0, 4),
+             (2, 0xC3, 1, 5), (3, 0xC3, 0, 4), (3, 0xC3, 1, 5), (6, 0xC3, 0, 4), (6, 0xC3, 1, 5),
+             (7, 0xC3, 0, 6), (7, 0xC3, 1, 7), (0, 0xC0, 0, 0), (0, 0xC0, 1, 1), (6, 0xC0, 0, 0),
+             (6, 0xC0, 1, 1), (7, 0xC0, 0, 2), (7, 0xC0, 1, 3)]
+    for b, dirs, up, want in cases:
+        assert itl.rates_index(b, dirs, up) == want
+    # itl_get1nnIdByRatesIndex_test
+    for b, up, dirs in ((0, 0, 0xC3), (0, 1, 0xC3), (1, 1, 0xC3), (6, 1, 0xC3), (7, 1, 0xC3), (6, 1, 0xC0),
+                        (7, 1, 0xC0)):
+        assert itl.first_nn(itl.rates_index(b, dirs, up), dirs) == b
+
+
+def test_rates_status(itl):
+    # itl_getRatesStatus_test
+    for avail, want in ((0xC3, 0xFF), (0x83, 0xCF), (0x43, 0x3F), (0x03, 0x0F), (0x01, 0x03), (0x02, 0x0C),
+                        (0xF3, 0xFF), (0xFF, 0xFF), (0xCC, 0xF0)):
+        assert itl.rates_status(S_110, avail) == want
+
+
+def test_update_rates(itl):
+    # itl_updateRates_with_beforeRatesUpdate_test: callback = direction + 1
+    cb = np.arange(8, dtype=np.float64) + 1
+    nb = [FE] * 8
+    av = itl.itl_avail(S_110, 0xFF, nb)
+    r = itl.update_rates(S_110, av, cb, np.zeros(8))
+    assert list(r) == [1 / 2, 1 / 2, 2 / 2, 2 / 2, 7 / 2, 7 / 2, 8 / 2, 8 / 2]
+    nb[1] = FECU
+    av = itl.itl_avail(S_110, 0xFF, nb)
+    r = itl.update_rates(S_110, av, cb, np.zeros(8))
+    assert list(r) == [1 / 2, 1 / 2, 0, 0, 7 / 2, 7 / 2, 8 / 2, 8 / 2]
+    # itl_updateRates_test: callback = direction; unavailable slots untouched
+    cb = np.arange(8, dtype=np.float64)
+    assert list(itl.update_rates(S_110, 0xC3, cb, np.zeros(8))) == [0, 0, 1 / 2, 1 / 2, 6 / 2, 6 / 2, 7 / 2, 7 / 2]
+    assert list(itl.update_rates(S_110, 0xC1, cb, -np.ones(8))) == [0, 0, -1, -1, 6 / 2, 6 / 2, 7 / 2, 7 / 2]
+    assert list(itl.update_rates(S_110, 0x00, cb, -np.ones(8))) == [-1] * 8
+
+
+def test_type_helpers_against_reference_source(itl):
+    singles = (V, FE, CU, NI, MN)
+    for a in singles[1:]:
+        for b in singles[1:]:
+            t = itl.combine(a, b)
+            assert t == (min(a, b) << 8 | max(a, b))
+            assert itl.diff(t, a) == b and itl.diff(t, b) == a
+        assert itl.diff(FECU, NI) == FECU if a else True
+    L = refcpu.refsrc()
+    if L is None:
+        pytest.skip("oracle/_ref not built (the reference sources are not mounted here)")
+    for a in singles:
+        for b in singles:
+            assert L.refsrc_combine_to_inter(a, b) == itl.combine(a, b)
+    for t in singles + DUMBBELLS:
+        info = L.refsrc_type_info(t)
+        assert bool(info & 1) == (t > 8)
+        assert bool(info & 2) == (0 < t <= 8)
+        for b in singles:
+            assert L.refsrc_diff(t, b) == itl.diff(t, b)
+
+
+# ---------------------------------------------------------------- energies
+BOX = (4, 4, 4)
+X, Y, Z = 2 * BOX[0], BOX[1], BOX[2]
+
+
+def site(x, y, z):
+    return (x % X) + X * ((y % Y) + Y * (z % Z))
+
+
+def nn1(s, b):
+    x, y, z = s % X, (s // X) % Y, s // (X * Y)
+    lo = 0 if x & 1 else -1
+    return site(x + (1 if b & 4 else -1), y + lo + ((b >> 1) & 1), z + lo + (b & 1))
+
+
+NN2 = [(-2, 0, 0), (0, -1, 0), (0, 0, -1), (0, 0, 1), (0, 1, 0), (2, 0, 0)]
+
+
+def nn2(s, k):
+    x, y, z = s % X, (s // X) % Y, s // (X * Y)
+    dx, dy, dz = NN2[k]
+    return site(x + dx, y + dy, z + dz)
+
+
+def fold(terms):
+    e = 0.0
+    for t in terms:
+        e += t
+    return e
+
+
+def test_edumb_single_and_pairs(itl):
+    types = np.full(X * Y * Z, FE, dtype=np.uint16)
+    s = site(2, 1, 1)
+    types[s] = FECU
+    # a lone FeCu dumbbell among Fe atoms: emix only
+    assert itl.edumb(BOX, types, [s], [S_110], [0xC3]) == ((-0.46 + 0.0) + 0.0) + 0.0 / 2
+    # a Cu atom in the dumbbell's Cu jump directions (low end of s_110: 6, 7)
+    types[nn1(s, 6)] = CU
+    assert itl.edumb(BOX, types, [s], [S_110], [0xC3]) == ((-0.46 + 0.0) + (-0.18 + 0.0 / 2)) + 0.0 / 2
+    # ... only counted where the stored availability mask allows
+    assert itl.edumb(BOX, types, [s], [S_110], [0x83]) == ((-0.46 + 0.0) + 0.0) + 0.0 / 2
+    # two CuCu dumbbells at 1nn: each sees the other once (0.70, halved)
+    types = np.full(X * Y * Z, FE, dtype=np.uint16)
+    a = site(2, 1, 1)
+    b = nn1(a, 7)
+    types[a] = types[b] = CUCU
+    lst = sorted([a, b])
+    term = ((-0.36 + 0.0) + 0.0) + (0.0 + 0.70) / 2
+    assert itl.edumb(BOX, types, lst, [S_110, S_110], [0, 0]) == (0.0 + term) + term
+
+
+def test_edumb_fefe_tension_and_quirk(itl):
+    types = np.full(X * Y * Z, FE, dtype=np.uint16)
+    s = site(2, 1, 1)
+    types[s] = FEFE
+    # s_110 moves along 0, 1, 6, 7: Cu at 2 (not a jump direction), Ni at 6 (one)
+    types[nn1(s, 2)] = CU
+    types[nn1(s, 6)] = NI
+    etens = fold([0.13, 0.15])
+    assert itl.edumb(BOX, types, [s], [S_110], [0]) == ((0.0 + etens) + 0.0) + 0.0 / 2
+    # a dumbbell at 1nn index 0: 0.70 as a 1nn, and 0.48 again because the
+    # 2nn bits are tested against the 1nn list (energy_dumbbell.cpp:118-122)
+    types[nn1(s, 6)] = FE
+    types[nn1(s, 0)] = CUCU
+    eb = 0.70 + 0.48
+    assert itl.edumb(BOX, types, [s], [S_110], [0]) == ((0.0 + 0.13) + 0.0) + (0.0 + eb) / 2
+    # a dumbbell at the 2nn site 0 (-2, 0, 0) is not seen by FeFe at all
+    types[nn1(s, 0)] = FE
+    x, y, z = s % X, (s // X) % Y, s // (X * Y)
+    types[site(x - 2, y, z)] = CUCU
+    assert itl.edumb(BOX, types, [s], [S_110], [0]) == ((0.0 + 0.13) + 0.0) + 0.0 / 2
+
+
+def _count(types, s, src, e1, e2):
+    """BondsCounter::count on the typed lattice: dumbbell neighbours skipped."""
+    terms = []
+    for b in range(8):
+        t = int(types[nn1(s, b)])
+        if t <= 8:
+            terms.append(e1[(min(t, src), max(t, src))])
+    for k in (0, 1, 2, 3, 5):
+        t = int(types[nn2(s, k)])
+        if t <= 8:
+            terms.append(e2[(min(t, src), max(t, src))])
+    return fold(terms)
+
+
+E1 = {(1, 1): -0.778, (0, 1): -0.161, (1, 2): -0.609, (2, 2): -0.581}
+E2 = {(1, 1): -0.389, (0, 1): -0.161, (1, 2): -0.344, (2, 2): -0.389}
+
+
+def test_typed_count_skips_dumbbells(itl):
+    types = np.full(X * Y * Z, FE, dtype=np.uint16)
+    s = site(3, 2, 1)
+    assert itl.count(BOX, types, s, FE) == fold([-0.778] * 8 + [-0.389] * 5)
+    types[nn1(s, 4)] = FECU
+    types[nn2(s, 3)] = CUCU
+    types[nn1(s, 1)] = CU
+    assert itl.count(BOX, types, s, FE) == _count(types, s, FE, E1, E2)
+    assert itl.count(BOX, types, s, FE) == fold([-0.778, -0.609] + [-0.778] * 5 + [-0.389] * 4)
+
+
+@pytest.mark.parametrize("T", [300.0, 573.0])
+def test_fefe_rates_in_pure_iron(itl, T):
+    # One FeFe dumbbell among Fe atoms: the bond energies before and after are
+    # the same two sums in the other order and the dumbbell energy is zero on
+    # both sides, so dE = 0 and every jump has rate nu exp(-0.32 / kT) / 2.
+    types = np.full(X * Y * Z, FE, dtype=np.uint16)
+    for o in (1, -1, 2, 3, -4, 5, -6):
+        s = site(3, 2, 1)
+        types[s] = FEFE
+        av, r = itl.rates(BOX, types, [s], [o], 6e12, T)
+        assert av[0] == itl.avail(o, 0)
+        want = 6e12 * math.exp(-0.32 / (KB * T)) / 2
+        assert list(r[0]) == [want] * 8
+
+
+def test_fecu_rates_in_pure_iron(itl):
+    nu, T = 6e12, 573.0
+    types = np.full(X * Y * Z, FE, dtype=np.uint16)
+    s = site(2, 1, 1)
+    types[s] = FECU
+    av, r = itl.rates(BOX, types, [s], [S_110], nu, T)
+    assert av[0] == 0xC3
+    S8 = lambda a: fold([E1[a]] * 8 + [E2[a]] * 5)  # noqa: E731
+    S7 = lambda a: fold([E1[a]] * 7 + [E2[a]] * 5)  # noqa: E731
+    # Cu jumps (directions 6, 7): Cu joins an Fe atom, Fe stays behind.
+    de_cu = ((S7((1, 1)) + S8((1, 2))) + 0.0) - ((S8((1, 2)) + S7((1, 1))) + (0.0 + (((-0.46 + 0.0) + 0.0) + 0.0 / 2)))
+    k_cu = nu * math.exp(-(0.36 + de_cu / 2) / (KB * T))
+    # Fe jumps (directions 0, 1): Cu stays behind as an atom next to the new FeFe.
+    for b, lo in ((0, 0), (1, 2)):
+        t = nn1(s, b)
+        after = types.copy()
+        after[s] = CU
+        after[t] = FEFE
+        e_src_after = _count(after, s, CU, E1, E2)
+        e_des_after = _count(after, t, FE, E1, E2)
+        de_fe = ((e_src_after + e_des_after) + 0.0) - ((S8((1, 1)) + S7((1, 1))) + (0.0 + (-0.46)))
+        k_fe = nu * math.exp(-(0.32 + de_fe / 2) / (KB * T))
+        assert r[0][lo] == r[0][lo + 1] == k_fe / 2
+    assert r[0][4] == r[0][5] == r[0][6] == r[0][7] == k_cu / 2
