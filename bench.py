@@ -10,7 +10,8 @@ upload are outside the timed region, so the lattice is resident in HBM when
 timing starts.  value = events performed in the timed steps / seconds.
 
 Beside the plain line, the same JSON line carries two more legs of the same
-metric (disable with --no-batched / --no-served):
+metric (disable with --no-batched / --no-served) and the dumbbell-rate step
+(--no-itl):
   batched  the replica engine on 1,024 independent replicas (edge U{16..96},
            random Fe-Cu-Ni-Mn compositions, T U[500,700] K, 2e-4 vacancies),
            FP32 rates with FP64 sums, autotuned, hipGraph replays; a "step"
@@ -19,6 +20,8 @@ metric (disable with --no-batched / --no-served):
            vacancies U[1e-4,5e-4], targets log-uniform in [1e5,1e7] events scaled
            by --serve-scale), admitted by a 250 GB HBM budget; reports
            throughput and p50/p99 completion latency.
+  itl      the abvi dumbbell model's rate step (akmc_itl_rates) on a 256^3
+           typed lattice with 1e-5 dumbbells: rates per second of kernel time.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--events-per-step E]
 """
@@ -232,6 +235,44 @@ def served_leg(args, batched_value):
             "data": "synthetic jobs; vacancy-only (self-interstitial dumbbells are not modelled)"}
 
 
+def itl_leg(args):
+    """Dumbbell (self-interstitial) rates on one serving job's lattice: the
+    rate step of the abvi dumbbell model (akmc_itl_rates), device time of its
+    kernels, and the oracle on the same input as the CPU baseline."""
+    import akmc
+    from akmc import workloads
+    L = args.itl_edge
+    box = (L, L, L)
+    types, sites, orient = workloads.typed_lattice(box, 0x5EED0004, args.itl_fraction)
+    akmc.itl_rates(box, types, sites, orient, 6e12, 573.0)  # warmup
+    ksec, calls, nrates = 0.0, 0, 0
+    t0 = time.perf_counter()
+    for _ in range(args.itl_calls):
+        tm = {}
+        av, r = akmc.itl_rates(box, types, sites, orient, 6e12, 573.0, timing=tm)
+        ksec += tm["kernel_seconds"]
+        calls += 1
+        nrates += int(sum(bin(int(a)).count("1") for a in av))
+    wall = time.perf_counter() - t0
+    leg = {"value": nrates / ksec, "unit": "dumbbell rates/s", "dtype": "f64",
+           "config": {"workload": "abvi dumbbell rates: BCC %d^3 cells (%d sites), Fe-1.34Cu-0.70Ni-1.30Mn at%%, "
+                                  "%d dumbbells (fraction %g), T=573 K" % (L, types.size, sites.size, args.itl_fraction),
+                      "calls": calls, "rates_per_call": nrates // max(1, calls)},
+           "kernel_ms_per_call": ksec / calls * 1e3, "wall_ms_per_call": wall / calls * 1e3,
+           "note": "kernel time of the two kernels; a call also checks and uploads the whole typed lattice"}
+    if not args.no_cpu_baseline:
+        from oracle import refcpu
+        o = refcpu.Itl()
+        t0 = time.perf_counter()
+        av_o, r_o = o.rates(box, types, sites, orient, 6e12, 573.0)
+        dt = time.perf_counter() - t0
+        leg["cpu_baseline"] = {"value": nrates // max(1, calls) / dt, "unit": "dumbbell rates/s", "cores": 1,
+                               "kind": "port", "sample": "oracle/refcpu.cpp (refitl) on the same lattice, one call, "
+                                                         "%.2f s" % dt}
+        leg["matches_oracle"] = bool((av_o == av).all() and (r_o.view("u8") == r.view("u8")).all())
+    return leg
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -259,6 +300,10 @@ def main():
     ap.add_argument("--batch-warmup", type=int, default=1)
     ap.add_argument("--no-fixed", action="store_true", help="batched leg: skip the fixed-event run")
     ap.add_argument("--no-error-stats", action="store_true", help="batched leg: skip the FP32 error sample")
+    ap.add_argument("--no-itl", action="store_true", help="skip the dumbbell-rate leg")
+    ap.add_argument("--itl-edge", type=int, default=256)
+    ap.add_argument("--itl-fraction", type=float, default=1e-5)
+    ap.add_argument("--itl-calls", type=int, default=5)
     ap.add_argument("--jobs", type=int, default=4000)
     ap.add_argument("--serve-scale", type=float, default=0.01,
                     help="scale of the job targets (1 = the full 1e5..1e7 events)")
@@ -322,6 +367,9 @@ def main():
     if not args.no_served:
         out["served"] = served_leg(args, batched["value"] if batched else 5e7)
         print("bench: served %.3g events/s" % out["served"]["value"], file=sys.stderr, flush=True)
+    if not args.no_itl:
+        out["itl"] = itl_leg(args)
+        print("bench: dumbbell rates %.3g /s" % out["itl"]["value"], file=sys.stderr, flush=True)
     out["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(args.cpu_events, args.cpu_seconds)
     print(json.dumps(out))
     return 0

@@ -179,12 +179,13 @@ int akmc_exp_device(int device, const double* x, double* y, int64_t n);
  * itl_list.cpp:9-29) and rates[8 n_itl] (Itl::updateRates with
  * ItlRatesSolver::rate as the callback, itl_list.cpp:31-50,
  * rate/itl_rates_solver.cpp:16-80, rate/bonds/energy_dumbbell.cpp:7-206):
- * every mask first, then every rate.  The reference has no caller of this
- * model (models/abvi/kmc.cpp:393-395), and its lattice no longer holds
- * dumbbell types; this is the model's rate step for callers that do. */
+ * every mask first, then every rate.  kernel_seconds (may be NULL): device
+ * time of the two kernels.  The reference has no caller of this model
+ * (models/abvi/kmc.cpp:393-395), and its lattice no longer holds dumbbell
+ * types; this is the model's rate step for callers that do. */
 int akmc_itl_rates(const int64_t box[3], const uint16_t* types, int64_t n_itl, const int64_t* sites,
                    const int8_t* orient, double attempt_freq, double temperature, int device, uint8_t* avail,
-                   double* rates);
+                   double* rates, double* kernel_seconds);
 
 /* Waits for all work on a device (benchmark bracketing). */
 int akmc_sync(int device);

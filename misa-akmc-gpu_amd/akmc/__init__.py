@@ -113,7 +113,7 @@ def lib():
     L.akmc_copy_bench.argtypes = [i32, i64, i32, P(dbl)]
     L.akmc_exp_host.argtypes = [vp, vp, i64]
     L.akmc_exp_device.argtypes = [i32, vp, vp, i64]
-    L.akmc_itl_rates.argtypes = [vp, vp, i64, vp, vp, dbl, dbl, i32, vp, vp]
+    L.akmc_itl_rates.argtypes = [vp, vp, i64, vp, vp, dbl, dbl, i32, vp, vp, P(dbl)]
     L.akmc_destroy.argtypes = [vp]
     L.akmc_destroy.restype = None
     _lib = L
@@ -647,10 +647,11 @@ def dumbbell(a, b):
     return (a << 8) | b
 
 
-def itl_rates(box, types, sites, orient, attempt_freq, temperature, device=0):
+def itl_rates(box, types, sites, orient, attempt_freq, temperature, device=0, timing=None):
     """Rates of the abvi dumbbell model (akmc_itl_rates): (avail[n],
     rates[n, 8]) for the dumbbells at `sites` (ascending) with orientations
-    `orient`, on a periodic lattice of LatticeTypes values `types`."""
+    `orient`, on a periodic lattice of LatticeTypes values `types`.  A dict
+    passed as `timing` receives the kernels' device seconds."""
     b = np.ascontiguousarray(box, dtype=np.int64)
     t = np.ascontiguousarray(types, dtype=np.uint16)
     s = np.ascontiguousarray(sites, dtype=np.int64)
@@ -661,7 +662,10 @@ def itl_rates(box, types, sites, orient, attempt_freq, temperature, device=0):
         raise AkmcError(AKMC_ERR_INVALID, "one orientation per dumbbell")
     av = np.zeros(s.size, dtype=np.uint8)
     r = np.zeros((s.size, 8), dtype=np.float64)
+    ks = ctypes.c_double()
     _check(lib().akmc_itl_rates(b.ctypes.data, t.ctypes.data, s.size, s.ctypes.data, o.ctypes.data,
                                 float(attempt_freq), float(temperature), int(device), av.ctypes.data,
-                                r.ctypes.data))
+                                r.ctypes.data, ctypes.byref(ks)))
+    if timing is not None:
+        timing["kernel_seconds"] = ks.value
     return av, r

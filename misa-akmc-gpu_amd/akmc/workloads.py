@@ -97,3 +97,28 @@ def jobs(n=4000, base=0x5EED0002, lo=32, hi=320, ev_lo=1e5, ev_hi=1e7):
         target = int(round(math.exp(st.uniform(math.log(ev_lo), math.log(ev_hi)))))
         out.append((_cfg(L, alloy, va, T, st), target))
     return out
+
+
+DUMBBELL_TYPES = (0x0101, 0x0102, 0x0104, 0x0108, 0x0202, 0x0204, 0x0208, 0x0404, 0x0408, 0x0808)
+
+
+def typed_lattice(box, seed, f_itl, solutes=(0.0134, 0.0070, 0.0130), f_v=2e-4):
+    """A typed lattice for the dumbbell model: Fe with Cu, Ni and Mn atoms
+    (fractions `solutes`), vacancies (f_v) and dumbbells of random kind and
+    orientation at a fraction f_itl of the sites.  Returns (types, sites,
+    orient) in akmc_itl_rates' layout."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    n = 2 * box[0] * box[1] * box[2]
+    u = rng.random(n)
+    t = np.full(n, 1, dtype=np.uint16)
+    c = np.cumsum([solutes[0], solutes[1], solutes[2], f_v])
+    t[u < c[3]] = 0
+    t[u < c[2]] = 8
+    t[u < c[1]] = 4
+    t[u < c[0]] = 2
+    k = max(1, int(round(f_itl * n)))
+    sites = np.sort(rng.choice(n, size=k, replace=False)).astype(np.int64)
+    t[sites] = rng.choice(np.array(DUMBBELL_TYPES, dtype=np.uint16), size=k)
+    orient = rng.choice(np.array([-6, -5, -4, -3, -2, -1, 1, 2, 3, 4, 5, 6], dtype=np.int8), size=k)
+    return t, sites, orient

@@ -225,8 +225,14 @@ __device__ __forceinline__ bool near(const Dims& d, int x, int y, int z, int cx,
   return dx >= -2 && dx <= 2 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1;
 }
 
-// Rates: eight lanes per dumbbell, lane b for direction b.
-__global__ void itl_rate_kernel(Args a) {
+// Rates: eight lanes per dumbbell, lane b for direction b.  The list's terms
+// and coordinates are staged through LDS in chunks, so each lane's serial
+// fold reads LDS (one broadcast address per step) instead of global memory.
+constexpr int RATE_THREADS = 256;
+constexpr int LIST_CHUNK = 1024;
+__global__ __launch_bounds__(RATE_THREADS) void itl_rate_kernel(Args a) {
+  __shared__ double s_term[LIST_CHUNK];
+  __shared__ int4 s_info[LIST_CHUNK];
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t k = g >> 3;
   const int b = (int)(g & 7);
@@ -235,38 +241,60 @@ __global__ void itl_rate_kernel(Args a) {
   const int type = me.w & 0xFFFF;
   const uint8_t av = (uint8_t)((me.w >> 16) & 0xFF);
   const int o = (me.w >> 24) - 6;
+  const bool work = live && ((av >> b) & 1);
   double rate = 0.0;
-  if (live && ((av >> b) & 1)) {
-    const int64_t src = site_of(a.d, me.x, me.y, me.z);
+  int ghost = 0, src_after = 0, dst_after = 0, dx = 0, dy = 0, dz = 0;
+  int64_t src = -1, dst = -1;
+  double e_src_b = 0, e_des_b = 0, e_src_a = 0, e_des_a = 0;
+  if (work) {
+    src = site_of(a.d, me.x, me.y, me.z);
     const int lo = (me.x & 1) ? 0 : -1;
     const int tx = me.x + ((b & 4) ? 1 : -1), ty = me.y + lo + ((b >> 1) & 1), tz = me.z + lo + (b & 1);
-    const int64_t dst = site_of(a.d, tx, ty, tz);
-    const int dx = wrap(tx, a.d.X), dy = wrap(ty, a.d.Y), dz = wrap(tz, a.d.Z);
+    dst = site_of(a.d, tx, ty, tz);
+    dx = wrap(tx, a.d.X);
+    dy = wrap(ty, a.d.Y);
+    dz = wrap(tz, a.d.Z);
     // orientation::tranAtom (orientation.cc:238-253): X of XY in the high
     // end's directions, Y in the low end's.
-    const int ghost = ((c_high[o + 6] >> b) & 1) ? (type >> 8) : (type & 0xFF);
+    ghost = ((c_high[o + 6] >> b) & 1) ? (type >> 8) : (type & 0xFF);
     const int tdst = a.types[dst];
     // LatticeTypes::diff and combineToInter (lattice_types.cpp:21-48).
-    const int src_after = (type & 0xFF) == ghost ? (type >> 8) : (type >> 8) == ghost ? (type & 0xFF) : type;
-    const int dst_after = tdst <= ghost ? (tdst << 8) | ghost : (ghost << 8) | tdst;
+    src_after = (type & 0xFF) == ghost ? (type >> 8) : (type >> 8) == ghost ? (type & 0xFF) : type;
+    dst_after = tdst <= ghost ? (tdst << 8) | ghost : (ghost << 8) | tdst;
     const View L0{a.types, -1, -1, 0, 0};
     const View L1{a.types, src, dst, src_after, dst_after};
-    const double e_src_b = count(a.d, L0, me.x, me.y, me.z, ghost);
-    const double e_des_b = count(a.d, L0, dx, dy, dz, tdst);
-    const double e_src_a = count(a.d, L1, me.x, me.y, me.z, src_after);
-    const double e_des_a = count(a.d, L1, dx, dy, dz, ghost);
-    double sb = 0, sa = 0;
-    for (int64_t j = 0; j < a.n; j++) {
-      const double t = a.term[j];
-      const int4 q = a.info[j];
-      double ta = t;
-      if (near(a.d, q.x, q.y, q.z, me.x, me.y, me.z) || near(a.d, q.x, q.y, q.z, dx, dy, dz)) {
-        const int64_t sj = site_of(a.d, q.x, q.y, q.z);
-        ta = edumb_term(a.d, L1, q.x, q.y, q.z, L1.at(sj), (q.w >> 24) - 6, (uint8_t)((q.w >> 16) & 0xFF));
-      }
-      sb += t;
-      sa += ta;
+    e_src_b = count(a.d, L0, me.x, me.y, me.z, ghost);
+    e_des_b = count(a.d, L0, dx, dy, dz, tdst);
+    e_src_a = count(a.d, L1, me.x, me.y, me.z, src_after);
+    e_des_a = count(a.d, L1, dx, dy, dz, ghost);
+  }
+  // Edumb before and after, left folds in list order; every thread of the
+  // block takes part in the staging.
+  const View L1{a.types, src, dst, src_after, dst_after};
+  double sb = 0, sa = 0;
+  for (int64_t base = 0; base < a.n; base += LIST_CHUNK) {
+    const int cnt = (int)min<int64_t>(LIST_CHUNK, a.n - base);
+    __syncthreads();
+    for (int i = threadIdx.x; i < cnt; i += RATE_THREADS) {
+      s_term[i] = a.term[base + i];
+      s_info[i] = a.info[base + i];
     }
+    __syncthreads();
+    if (work) {
+      for (int i = 0; i < cnt; i++) {
+        const double t = s_term[i];
+        const int4 q = s_info[i];
+        double ta = t;
+        if (near(a.d, q.x, q.y, q.z, me.x, me.y, me.z) || near(a.d, q.x, q.y, q.z, dx, dy, dz)) {
+          const int64_t sj = site_of(a.d, q.x, q.y, q.z);
+          ta = edumb_term(a.d, L1, q.x, q.y, q.z, L1.at(sj), (q.w >> 24) - 6, (uint8_t)((q.w >> 16) & 0xFF));
+        }
+        sb += t;
+        sa += ta;
+      }
+    }
+  }
+  if (work) {
     const double e_before = e_src_b + e_des_b + sb;
     const double e_after = e_src_a + e_des_a + sa;
     const double e0 = ghost == T_FE ? 0.32 : ghost == T_CU ? 0.36 : ghost == T_NI ? 0.45 : ghost == T_MN ? 0.31 : 0.0;
@@ -310,7 +338,8 @@ bool valid_dumbbell(int t) {
 
 extern "C" int akmc_itl_rates(const int64_t box[3], const uint16_t* types, int64_t n_itl, const int64_t* sites,
                               const int8_t* orient, double attempt_freq, double temperature, int device,
-                              uint8_t* avail, double* rates) {
+                              uint8_t* avail, double* rates, double* kernel_seconds) {
+  if (kernel_seconds) *kernel_seconds = 0.0;
   if (!box || !types || n_itl < 0 || (n_itl > 0 && (!sites || !orient || !avail || !rates)))
     return fail(AKMC_ERR_INVALID, "null argument");
   for (int i = 0; i < 3; i++)
@@ -362,13 +391,31 @@ extern "C" int akmc_itl_rates(const int64_t box[3], const uint16_t* types, int64
   a.avail = d_avail.p;
   a.rates = d_rates.p;
   const int threads = 256;
+  hipEvent_t ev[2];
+  HIP_TRY(hipEventCreate(&ev[0]));
+  if (hipEventCreate(&ev[1]) != hipSuccess) {
+    (void)hipEventDestroy(ev[0]);
+    return fail(AKMC_ERR_HIP, "hipEventCreate failed");
+  }
+  struct EvGuard {
+    hipEvent_t* e;
+    ~EvGuard() {
+      (void)hipEventDestroy(e[0]);
+      (void)hipEventDestroy(e[1]);
+    }
+  } evg{ev};
+  HIP_TRY(hipEventRecord(ev[0], nullptr));
   hipLaunchKernelGGL(akmc::itl::itl_prep_kernel, dim3((unsigned)((n_itl + threads - 1) / threads)), dim3(threads), 0,
                      nullptr, a);
   HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(akmc::itl::itl_rate_kernel, dim3((unsigned)((8 * n_itl + threads - 1) / threads)), dim3(threads),
-                     0, nullptr, a);
+  constexpr int rt = akmc::itl::RATE_THREADS;
+  hipLaunchKernelGGL(akmc::itl::itl_rate_kernel, dim3((unsigned)((8 * n_itl + rt - 1) / rt)), dim3(rt), 0, nullptr, a);
   HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(ev[1], nullptr));
   HIP_TRY(hipDeviceSynchronize());
+  float ms = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
+  if (kernel_seconds) *kernel_seconds = ms * 1e-3;
   HIP_TRY(hipMemcpy(avail, d_avail.p, n_itl, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(rates, d_rates.p, 8 * n_itl * sizeof(double), hipMemcpyDeviceToHost));
   return AKMC_OK;

@@ -6,29 +6,13 @@ import numpy as np
 import pytest
 
 import akmc
+from akmc import workloads
 from oracle import refcpu
 
 pytestmark = pytest.mark.gpu
 
-DUMBBELLS = [0x0101, 0x0102, 0x0104, 0x0108, 0x0202, 0x0204, 0x0208, 0x0404, 0x0408, 0x0808]
-
-
 def typed_lattice(box, seed, f_itl, solutes=(0.01, 0.02, 0.01), f_v=2e-4):
-    """Fe with Cu, Ni, Mn atoms, vacancies and dumbbells at random sites."""
-    rng = np.random.default_rng(seed)
-    n = 2 * box[0] * box[1] * box[2]
-    u = rng.random(n)
-    t = np.full(n, 1, dtype=np.uint16)
-    c = np.cumsum([solutes[0], solutes[1], solutes[2], f_v])
-    t[u < c[3]] = 0
-    t[u < c[2]] = 8
-    t[u < c[1]] = 4
-    t[u < c[0]] = 2
-    k = max(1, int(round(f_itl * n)))
-    sites = np.sort(rng.choice(n, size=k, replace=False)).astype(np.int64)
-    t[sites] = rng.choice(np.array(DUMBBELLS, dtype=np.uint16), size=k)
-    orient = rng.choice(np.array([-6, -5, -4, -3, -2, -1, 1, 2, 3, 4, 5, 6], dtype=np.int8), size=k)
-    return t, sites, orient
+    return workloads.typed_lattice(box, seed, f_itl, solutes=solutes, f_v=f_v)
 
 
 def _same(a, b):
