@@ -191,28 +191,128 @@ struct Args {
   const int8_t* orient;
   int64_t n;
   double nu, kT;
-  int4* info;      // per dumbbell: x, y, z, type | avail << 16 | (o + 6) << 24
-  double* term;    // per dumbbell: its Edumb term on the unchanged lattice
-  uint8_t* avail;  // out
-  double* rates;   // out, 8 per dumbbell
+  int4* info;        // per dumbbell: x, y, z, type | avail << 16 | (o + 6) << 24
+  double* term;      // per dumbbell: its Edumb term on the unchanged lattice
+  double* prefix;    // prefix[j]: left fold of term[0 .. j]
+  int32_t* nbr;      // per dumbbell: list indices of the dumbbells near it, ascending
+  int32_t* nbr_cnt;
+  uint8_t* avail;    // out
+  double* rates;     // out, 8 per dumbbell
 };
 
-// Availability masks and Edumb terms of all dumbbells (one thread each).
-__global__ void itl_prep_kernel(Args a) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= a.n) return;
+// Reach of the neighbour lists: every dumbbell whose Edumb term can read the
+// dumbbell's site or one of its 1nn lies within |dx| <= 2 + 1 (doubled x),
+// |dy| <= 1 + 1, |dz| <= 1 + 1 of it.
+constexpr int NBR_RX = 3, NBR_RY = 2, NBR_RZ = 2;
+constexpr int NBR_CAP = (2 * NBR_RX + 1) * (2 * NBR_RY + 1) * (2 * NBR_RZ + 1);
+
+// The i-th smallest of the distinct values (c + d) mod n, d in [-R, R]
+// (there are min(2R + 1, n) of them).
+__device__ __forceinline__ int ring_value(int c, int R, int n, int i) {
+  if (n <= 2 * R + 1) return i;
+  const int lo = c - R, hi = c + R;
+  if (lo < 0) return i <= hi ? i : n + lo + (i - hi - 1);
+  if (hi >= n) {
+    const int w = hi - n + 1;
+    return i < w ? i : lo + (i - w);
+  }
+  return lo + i;
+}
+
+// List index of site s (sites ascending), -1 if it is not listed.
+__device__ int64_t find_site(const int64_t* sites, int64_t n, int64_t s) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (sites[mid] < s) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < n && sites[lo] == s) ? lo : -1;
+}
+
+// Per dumbbell, one wavefront each: availability mask and Edumb term (lane
+// 0), and the ascending list of dumbbells within reach: lane l takes the
+// l-th candidate site in ascending index order (z, then y, then x over the
+// ascending coordinate values), and a ballot keeps that order.
+constexpr int PREP_THREADS = 64;
+__global__ __launch_bounds__(PREP_THREADS) void itl_prep_kernel(Args a) {
+  const int64_t k = blockIdx.x;
+  const int l = threadIdx.x;
   const int64_t s = a.sites[k];
   const int x = (int)(s % a.d.X), y = (int)((s / a.d.X) % a.d.Y), z = (int)(s / ((int64_t)a.d.X * a.d.Y));
-  const int type = a.types[s];
-  const int o = a.orient[k];
-  const View L{a.types, -1, -1, 0, 0};
-  uint8_t atoms = 0;
-  for (int b = 0; b < N1; b++)
-    if (is_atom(L.at(nn1(a.d, x, y, z, b)))) atoms |= (uint8_t)(1u << b);
-  const uint8_t av = (uint8_t)((c_high[o + 6] | c_low[o + 6]) & atoms);
-  a.avail[k] = av;
-  a.term[k] = edumb_term(a.d, L, x, y, z, type, o, av);
-  a.info[k] = make_int4(x, y, z, type | ((int)av << 16) | ((o + 6) << 24));
+  if (l == 0) {
+    const int type = a.types[s];
+    const int o = a.orient[k];
+    const View L{a.types, -1, -1, 0, 0};
+    uint8_t atoms = 0;
+    for (int b = 0; b < N1; b++)
+      if (is_atom(L.at(nn1(a.d, x, y, z, b)))) atoms |= (uint8_t)(1u << b);
+    const uint8_t av = (uint8_t)((c_high[o + 6] | c_low[o + 6]) & atoms);
+    a.avail[k] = av;
+    a.term[k] = edumb_term(a.d, L, x, y, z, type, o, av);
+    a.info[k] = make_int4(x, y, z, type | ((int)av << 16) | ((o + 6) << 24));
+  }
+  const int mx = min(2 * NBR_RX + 1, a.d.X), my = min(2 * NBR_RY + 1, a.d.Y), mz = min(2 * NBR_RZ + 1, a.d.Z);
+  const int total = mx * my * mz;
+  int32_t* out = a.nbr + k * NBR_CAP;
+  int cnt = 0;
+  for (int c0 = 0; c0 < total; c0 += PREP_THREADS) {
+    const int ci = c0 + l;
+    int64_t j = -1;
+    if (ci < total) {
+      const int ix = ci % mx, iy = (ci / mx) % my, iz = ci / (mx * my);
+      const int64_t c = ring_value(x, NBR_RX, a.d.X, ix) +
+                        (int64_t)a.d.X * (ring_value(y, NBR_RY, a.d.Y, iy) +
+                                          (int64_t)a.d.Y * ring_value(z, NBR_RZ, a.d.Z, iz));
+      if (is_dumbbell(a.types[c])) j = find_site(a.sites, a.n, c);
+    }
+    const uint64_t m = __ballot(j >= 0);
+    if (j >= 0) out[cnt + __popcll(m & ((1ull << l) - 1))] = (int32_t)j;
+    cnt += __popcll(m);
+  }
+  if (l == 0) a.nbr_cnt[k] = cnt;
+}
+
+// Lane i's double, read by every lane (i uniform).
+__device__ __forceinline__ double read_lane(double v, int i) {
+  const uint64_t u = __double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, i);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), i);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// Prefix folds of the terms: Edumb of the unchanged lattice and every partial
+// sum of it, in list order.  One wavefront: 64 terms per load, folded in
+// order through lane reads (every lane holds the same running sum).
+__global__ __launch_bounds__(64) void itl_prefix_kernel(Args a) {
+  const int l = threadIdx.x;
+  double acc = 0;
+  double t_next = l < a.n ? a.term[l] : 0.0;
+  for (int64_t base = 0; base < a.n; base += 64) {
+    const int64_t j = base + l;
+    const double t = t_next;  // the next chunk loads while this one folds
+    t_next = j + 64 < a.n ? a.term[j + 64] : 0.0;
+    const int cnt = (int)min<int64_t>(64, a.n - base);
+    double mine = 0;
+    for (int i = 0; i < cnt; i++) {
+      acc += read_lane(t, i);
+      mine = l == i ? acc : mine;
+    }
+    if (j < a.n) a.prefix[j] = mine;
+  }
+}
+
+// sa folded with term[j .. end), eight loads ahead of the adds.
+__device__ __forceinline__ double fold_terms(double sa, const double* __restrict__ tm, int64_t j, int64_t end) {
+  for (; j + 8 <= end; j += 8) {
+    double t[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) t[i] = tm[j + i];
+#pragma unroll
+    for (int i = 0; i < 8; i++) sa += t[i];
+  }
+  for (; j < end; j++) sa += tm[j];
+  return sa;
 }
 
 // True if (x, y, z) is within the reach of one Edumb term of (cx, cy, cz):
@@ -225,14 +325,12 @@ __device__ __forceinline__ bool near(const Dims& d, int x, int y, int z, int cx,
   return dx >= -2 && dx <= 2 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1;
 }
 
-// Rates: eight lanes per dumbbell, lane b for direction b.  The list's terms
-// and coordinates are staged through LDS in chunks, so each lane's serial
-// fold reads LDS (one broadcast address per step) instead of global memory.
+// Rates: eight lanes per dumbbell, lane b for direction b.  Edumb after the
+// jump is the prefix fold up to the first term the jump changes, then the
+// fold continues term by term, recomputing the changed ones (found in the
+// dumbbell's neighbour list); Edumb before is the full prefix fold.
 constexpr int RATE_THREADS = 256;
-constexpr int LIST_CHUNK = 1024;
 __global__ __launch_bounds__(RATE_THREADS) void itl_rate_kernel(Args a) {
-  __shared__ double s_term[LIST_CHUNK];
-  __shared__ int4 s_info[LIST_CHUNK];
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t k = g >> 3;
   const int b = (int)(g & 7);
@@ -241,60 +339,45 @@ __global__ __launch_bounds__(RATE_THREADS) void itl_rate_kernel(Args a) {
   const int type = me.w & 0xFFFF;
   const uint8_t av = (uint8_t)((me.w >> 16) & 0xFF);
   const int o = (me.w >> 24) - 6;
-  const bool work = live && ((av >> b) & 1);
   double rate = 0.0;
-  int ghost = 0, src_after = 0, dst_after = 0, dx = 0, dy = 0, dz = 0;
-  int64_t src = -1, dst = -1;
-  double e_src_b = 0, e_des_b = 0, e_src_a = 0, e_des_a = 0;
-  if (work) {
-    src = site_of(a.d, me.x, me.y, me.z);
+  if (live && ((av >> b) & 1)) {
+    const int64_t src = site_of(a.d, me.x, me.y, me.z);
     const int lo = (me.x & 1) ? 0 : -1;
     const int tx = me.x + ((b & 4) ? 1 : -1), ty = me.y + lo + ((b >> 1) & 1), tz = me.z + lo + (b & 1);
-    dst = site_of(a.d, tx, ty, tz);
-    dx = wrap(tx, a.d.X);
-    dy = wrap(ty, a.d.Y);
-    dz = wrap(tz, a.d.Z);
+    const int64_t dst = site_of(a.d, tx, ty, tz);
+    const int dx = wrap(tx, a.d.X), dy = wrap(ty, a.d.Y), dz = wrap(tz, a.d.Z);
     // orientation::tranAtom (orientation.cc:238-253): X of XY in the high
     // end's directions, Y in the low end's.
-    ghost = ((c_high[o + 6] >> b) & 1) ? (type >> 8) : (type & 0xFF);
+    const int ghost = ((c_high[o + 6] >> b) & 1) ? (type >> 8) : (type & 0xFF);
     const int tdst = a.types[dst];
     // LatticeTypes::diff and combineToInter (lattice_types.cpp:21-48).
-    src_after = (type & 0xFF) == ghost ? (type >> 8) : (type >> 8) == ghost ? (type & 0xFF) : type;
-    dst_after = tdst <= ghost ? (tdst << 8) | ghost : (ghost << 8) | tdst;
+    const int src_after = (type & 0xFF) == ghost ? (type >> 8) : (type >> 8) == ghost ? (type & 0xFF) : type;
+    const int dst_after = tdst <= ghost ? (tdst << 8) | ghost : (ghost << 8) | tdst;
     const View L0{a.types, -1, -1, 0, 0};
     const View L1{a.types, src, dst, src_after, dst_after};
-    e_src_b = count(a.d, L0, me.x, me.y, me.z, ghost);
-    e_des_b = count(a.d, L0, dx, dy, dz, tdst);
-    e_src_a = count(a.d, L1, me.x, me.y, me.z, src_after);
-    e_des_a = count(a.d, L1, dx, dy, dz, ghost);
-  }
-  // Edumb before and after, left folds in list order; every thread of the
-  // block takes part in the staging.
-  const View L1{a.types, src, dst, src_after, dst_after};
-  double sb = 0, sa = 0;
-  for (int64_t base = 0; base < a.n; base += LIST_CHUNK) {
-    const int cnt = (int)min<int64_t>(LIST_CHUNK, a.n - base);
-    __syncthreads();
-    for (int i = threadIdx.x; i < cnt; i += RATE_THREADS) {
-      s_term[i] = a.term[base + i];
-      s_info[i] = a.info[base + i];
-    }
-    __syncthreads();
-    if (work) {
-      for (int i = 0; i < cnt; i++) {
-        const double t = s_term[i];
-        const int4 q = s_info[i];
-        double ta = t;
-        if (near(a.d, q.x, q.y, q.z, me.x, me.y, me.z) || near(a.d, q.x, q.y, q.z, dx, dy, dz)) {
-          const int64_t sj = site_of(a.d, q.x, q.y, q.z);
-          ta = edumb_term(a.d, L1, q.x, q.y, q.z, L1.at(sj), (q.w >> 24) - 6, (uint8_t)((q.w >> 16) & 0xFF));
-        }
-        sb += t;
-        sa += ta;
+    const double e_src_b = count(a.d, L0, me.x, me.y, me.z, ghost);
+    const double e_des_b = count(a.d, L0, dx, dy, dz, tdst);
+    const double e_src_a = count(a.d, L1, me.x, me.y, me.z, src_after);
+    const double e_des_a = count(a.d, L1, dx, dy, dz, ghost);
+    const double sb = a.prefix[a.n - 1];
+    const int m = a.nbr_cnt[k];
+    const int32_t* nb = a.nbr + k * NBR_CAP;
+    double sa = sb;
+    int64_t j = -1;  // next list index to fold, once the first changed term is found
+    for (int q = 0; q < m; q++) {
+      const int32_t ja = nb[q];
+      const int4 qj = a.info[ja];
+      if (!near(a.d, qj.x, qj.y, qj.z, me.x, me.y, me.z) && !near(a.d, qj.x, qj.y, qj.z, dx, dy, dz)) continue;
+      if (j < 0) {
+        sa = ja > 0 ? a.prefix[ja - 1] : 0.0;
+        j = ja;
       }
+      sa = fold_terms(sa, a.term, j, ja);
+      const int64_t sj = site_of(a.d, qj.x, qj.y, qj.z);
+      sa += edumb_term(a.d, L1, qj.x, qj.y, qj.z, L1.at(sj), (qj.w >> 24) - 6, (uint8_t)((qj.w >> 16) & 0xFF));
+      j = ja + 1;
     }
-  }
-  if (work) {
+    if (j >= 0) sa = fold_terms(sa, a.term, j, a.n);
     const double e_before = e_src_b + e_des_b + sb;
     const double e_after = e_src_a + e_des_a + sa;
     const double e0 = ghost == T_FE ? 0.32 : ghost == T_CU ? 0.36 : ghost == T_NI ? 0.45 : ghost == T_MN ? 0.31 : 0.0;
@@ -342,6 +425,7 @@ extern "C" int akmc_itl_rates(const int64_t box[3], const uint16_t* types, int64
   if (kernel_seconds) *kernel_seconds = 0.0;
   if (!box || !types || n_itl < 0 || (n_itl > 0 && (!sites || !orient || !avail || !rates)))
     return fail(AKMC_ERR_INVALID, "null argument");
+  if (n_itl >= (int64_t)1 << 31) return fail(AKMC_ERR_INVALID, "at most 2^31 - 1 dumbbells");
   for (int i = 0; i < 3; i++)
     if (box[i] < 3 || box[i] > (1 << 20)) return fail(AKMC_ERR_INVALID, "box dimension %d is %lld (3 .. 2^20)", i, (long long)box[i]);
   if (!(temperature > 0) || !(attempt_freq >= 0) || !std::isfinite(attempt_freq))
@@ -366,13 +450,17 @@ extern "C" int akmc_itl_rates(const int64_t box[3], const uint16_t* types, int64
   DevBuf<int64_t> d_sites;
   DevBuf<int8_t> d_orient;
   DevBuf<int4> d_info;
-  DevBuf<double> d_term, d_rates;
+  DevBuf<double> d_term, d_prefix, d_rates;
+  DevBuf<int32_t> d_nbr, d_nbr_cnt;
   DevBuf<uint8_t> d_avail;
   HIP_TRY(d_types.alloc(nsites));
   HIP_TRY(d_sites.alloc(n_itl));
   HIP_TRY(d_orient.alloc(n_itl));
   HIP_TRY(d_info.alloc(n_itl));
   HIP_TRY(d_term.alloc(n_itl));
+  HIP_TRY(d_prefix.alloc(n_itl));
+  HIP_TRY(d_nbr.alloc((size_t)n_itl * akmc::itl::NBR_CAP));
+  HIP_TRY(d_nbr_cnt.alloc(n_itl));
   HIP_TRY(d_avail.alloc(n_itl));
   HIP_TRY(d_rates.alloc(8 * n_itl));
   HIP_TRY(hipMemcpy(d_types.p, types, nsites * sizeof(uint16_t), hipMemcpyHostToDevice));
@@ -388,9 +476,11 @@ extern "C" int akmc_itl_rates(const int64_t box[3], const uint16_t* types, int64
   a.kT = AKMC_KB * temperature;  // RatesSolver::arrhenius: kB * T first
   a.info = d_info.p;
   a.term = d_term.p;
+  a.prefix = d_prefix.p;
+  a.nbr = d_nbr.p;
+  a.nbr_cnt = d_nbr_cnt.p;
   a.avail = d_avail.p;
   a.rates = d_rates.p;
-  const int threads = 256;
   hipEvent_t ev[2];
   HIP_TRY(hipEventCreate(&ev[0]));
   if (hipEventCreate(&ev[1]) != hipSuccess) {
@@ -405,8 +495,9 @@ extern "C" int akmc_itl_rates(const int64_t box[3], const uint16_t* types, int64
     }
   } evg{ev};
   HIP_TRY(hipEventRecord(ev[0], nullptr));
-  hipLaunchKernelGGL(akmc::itl::itl_prep_kernel, dim3((unsigned)((n_itl + threads - 1) / threads)), dim3(threads), 0,
-                     nullptr, a);
+  hipLaunchKernelGGL(akmc::itl::itl_prep_kernel, dim3((unsigned)n_itl), dim3(akmc::itl::PREP_THREADS), 0, nullptr, a);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(akmc::itl::itl_prefix_kernel, dim3(1), dim3(64), 0, nullptr, a);
   HIP_TRY(hipGetLastError());
   constexpr int rt = akmc::itl::RATE_THREADS;
   hipLaunchKernelGGL(akmc::itl::itl_rate_kernel, dim3((unsigned)((8 * n_itl + rt - 1) / rt)), dim3(rt), 0, nullptr, a);
