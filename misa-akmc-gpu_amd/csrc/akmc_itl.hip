@@ -463,9 +463,17 @@ extern "C" int akmc_itl_rates(const int64_t box[3], const uint16_t* types, int64
   HIP_TRY(d_nbr_cnt.alloc(n_itl));
   HIP_TRY(d_avail.alloc(n_itl));
   HIP_TRY(d_rates.alloc(8 * n_itl));
-  HIP_TRY(hipMemcpy(d_types.p, types, nsites * sizeof(uint16_t), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(d_sites.p, sites, n_itl * sizeof(int64_t), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(d_orient.p, orient, n_itl * sizeof(int8_t), hipMemcpyHostToDevice));
+  // A stream of its own: the call waits for its own work only, not for an
+  // engine or server running on the same device.
+  hipStream_t st = nullptr;
+  HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() { (void)hipStreamDestroy(s); }
+  } sg{st};
+  HIP_TRY(hipMemcpyAsync(d_types.p, types, nsites * sizeof(uint16_t), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(d_sites.p, sites, n_itl * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(d_orient.p, orient, n_itl * sizeof(int8_t), hipMemcpyHostToDevice, st));
   akmc::itl::Args a;
   a.d = akmc::itl::Dims{(int32_t)X, (int32_t)box[1], (int32_t)box[2]};
   a.types = d_types.p;
@@ -494,20 +502,20 @@ extern "C" int akmc_itl_rates(const int64_t box[3], const uint16_t* types, int64
       (void)hipEventDestroy(e[1]);
     }
   } evg{ev};
-  HIP_TRY(hipEventRecord(ev[0], nullptr));
-  hipLaunchKernelGGL(akmc::itl::itl_prep_kernel, dim3((unsigned)n_itl), dim3(akmc::itl::PREP_THREADS), 0, nullptr, a);
+  HIP_TRY(hipEventRecord(ev[0], st));
+  hipLaunchKernelGGL(akmc::itl::itl_prep_kernel, dim3((unsigned)n_itl), dim3(akmc::itl::PREP_THREADS), 0, st, a);
   HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(akmc::itl::itl_prefix_kernel, dim3(1), dim3(64), 0, nullptr, a);
+  hipLaunchKernelGGL(akmc::itl::itl_prefix_kernel, dim3(1), dim3(64), 0, st, a);
   HIP_TRY(hipGetLastError());
   constexpr int rt = akmc::itl::RATE_THREADS;
-  hipLaunchKernelGGL(akmc::itl::itl_rate_kernel, dim3((unsigned)((8 * n_itl + rt - 1) / rt)), dim3(rt), 0, nullptr, a);
+  hipLaunchKernelGGL(akmc::itl::itl_rate_kernel, dim3((unsigned)((8 * n_itl + rt - 1) / rt)), dim3(rt), 0, st, a);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipEventRecord(ev[1], nullptr));
-  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipEventRecord(ev[1], st));
+  HIP_TRY(hipMemcpyAsync(avail, d_avail.p, n_itl, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(rates, d_rates.p, 8 * n_itl * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
   float ms = 0.0f;
   HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
   if (kernel_seconds) *kernel_seconds = ms * 1e-3;
-  HIP_TRY(hipMemcpy(avail, d_avail.p, n_itl, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(rates, d_rates.p, 8 * n_itl * sizeof(double), hipMemcpyDeviceToHost));
   return AKMC_OK;
 }

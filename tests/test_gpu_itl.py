@@ -58,3 +58,23 @@ def test_many_dumbbells():
     t, sites, orient = typed_lattice(box, 12, 0.02)
     assert len(sites) > 150
     _check(box, t, sites, orient)
+
+
+def test_runs_beside_an_engine():
+    # The call uses a stream of its own: an engine on the same device keeps
+    # its trajectory when dumbbell rates are computed between its runs.
+    from conftest import make_cfg
+    cfg = make_cfg(n=16, va=12)
+    a = akmc.Engine(cfg)
+    a.run(3000)
+    box = (8, 8, 8)
+    t, sites, orient = typed_lattice(box, 21, 0.05)
+    av1, r1 = akmc.itl_rates(box, t, sites, orient, 6e12, 573.0)
+    a.run(3000)
+    b = akmc.Engine(cfg)
+    b.run(6000)
+    assert np.array_equal(a.download(), b.download())
+    av2, r2 = akmc.itl_rates(box, t, sites, orient, 6e12, 573.0)
+    assert np.array_equal(av1, av2) and _same(r1, r2)
+    a.close()
+    b.close()
