@@ -168,6 +168,21 @@ int akmc_rng_streams(int64_t seed_time_inc, int64_t first, int64_t n, double* u,
 int akmc_exp_host(const double* x, double* y, int64_t n);
 int akmc_exp_device(int device, const double* x, double* y, int64_t n);
 
+/* Initial occupancy of pipe creation (creation::createFromPile,
+ * src/creation.cpp:184-230; selected by create.pipe in config.yaml,
+ * profiles/config/config_parsing.cpp:171-188): the atoms of the random
+ * creation's draw (seeds.create_types; p->va_count is ignored), then a
+ * vacancy at every "x y z" triple of pipe_file (x doubled, real coordinates;
+ * triples outside the box are skipped).  The site becomes a vacancy with no
+ * other tag, as the reference writes its type.  A file that cannot be opened
+ * is an error (the reference reads it silently as empty).  n_vacancies (may
+ * be NULL) receives the vacancy count.  Upload the result with akmc_upload. */
+int akmc_pipe_sites(const akmc_params* p, const char* pipe_file, uint8_t* sites, int64_t n, int64_t* n_vacancies);
+
+/* create.pipe.input_box of config.yaml when pipe creation is its create
+ * option (profiles/config/config_parsing.cpp:150-188), else AKMC_ERR_CONFIG. */
+int akmc_pipe_path_from_yaml(const char* yaml_path, char* out, int64_t cap);
+
 /* Self-interstitial (dumbbell) rates of the abvi model on a typed lattice.
  * types: one LatticeTypes value per real site (uint16: V 0, Fe 1, Cu 2, Ni 4,
  * Mn 8, dumbbell XY = X << 8 | Y with X <= Y; src/lattice/lattice_types.h:

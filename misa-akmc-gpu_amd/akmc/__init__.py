@@ -114,6 +114,8 @@ def lib():
     L.akmc_exp_host.argtypes = [vp, vp, i64]
     L.akmc_exp_device.argtypes = [i32, vp, vp, i64]
     L.akmc_itl_rates.argtypes = [vp, vp, i64, vp, vp, dbl, dbl, i32, vp, vp, P(dbl)]
+    L.akmc_pipe_sites.argtypes = [P(Params), ctypes.c_char_p, vp, i64, P(i64)]
+    L.akmc_pipe_path_from_yaml.argtypes = [ctypes.c_char_p, ctypes.c_char_p, i64]
     L.akmc_destroy.argtypes = [vp]
     L.akmc_destroy.restype = None
     _lib = L
@@ -199,6 +201,40 @@ def random_sites(params):
     out = np.zeros(n, dtype=np.uint8)
     _check(lib().akmc_random_sites(ctypes.byref(params), out.ctypes.data, n))
     return out
+
+
+def pipe_sites(params, pipe_file):
+    """Initial occupancy of pipe creation (akmc_pipe_sites): the random
+    creation's atoms and a vacancy at each "x y z" triple of pipe_file."""
+    if isinstance(params, dict):
+        params = params_from_config(params)
+    n = 2 * params.box[0] * params.box[1] * params.box[2]
+    out = np.zeros(n, dtype=np.uint8)
+    nv = ctypes.c_int64()
+    _check(lib().akmc_pipe_sites(ctypes.byref(params), str(pipe_file).encode(), out.ctypes.data, n,
+                                 ctypes.byref(nv)))
+    return out
+
+
+def pipe_path_from_yaml(path):
+    """create.pipe.input_box when pipe creation is the config's create option."""
+    buf = ctypes.create_string_buffer(4096)
+    _check(lib().akmc_pipe_path_from_yaml(str(path).encode(), buf, len(buf)))
+    return buf.value.decode()
+
+
+def engine_from_yaml(path, device=0):
+    """An engine set up as the reference's PKMC::onCreate would set it up from
+    config.yaml: random creation, or pipe creation from create.pipe.input_box
+    (profiles/pkmc.cpp:165-186)."""
+    p = params_from_yaml(str(path))
+    try:
+        pipe = pipe_path_from_yaml(path)
+    except AkmcError:
+        return Engine(p, device=device)
+    eng = Engine(p, device=device, init_random=False)
+    eng.upload(pipe_sites(p, pipe))
+    return eng
 
 
 def params_from_yaml(path):

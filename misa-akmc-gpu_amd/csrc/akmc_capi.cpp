@@ -296,12 +296,18 @@ int akmc_params_from_yaml(const char* path, akmc_params* p) {
   p->physics_time = sim->get_double("physics_time", 0.0);
   p->attempt_freq = sim->get_double("attempt_freq", 0.0);
   p->sector_time = sim->get_double("sector_time", 3.0e-7);  // extension; the reference hard-codes 3e-7
-  // parseCreate (:154-206): random creation only
+  // parseCreate (:150-206): random creation, else pipe creation, whose alloy
+  // is still read from create.random.alloy (:175-177); a pipe places no
+  // random vacancies (its file is read by akmc_pipe_sites).
   const akmc::yaml::Node* create = root.get("create");
   const akmc::yaml::Node* rnd = create ? create->get("random") : nullptr;
-  if (!rnd || !rnd->get_bool("enable", false))
-    return fail(AKMC_ERR_CONFIG, "no create option in config (only create.random is supported)");
-  p->va_count = rnd->get_int("va_count", 0);
+  const akmc::yaml::Node* pipe = create ? create->get("pipe") : nullptr;
+  const bool random_on = rnd && rnd->get_bool("enable", false);
+  const bool pipe_on = !random_on && pipe && pipe->get_bool("enable", false);
+  if (!random_on && !pipe_on)
+    return fail(AKMC_ERR_CONFIG, "no create option in config (create.random or create.pipe)");
+  if (!rnd) return fail(AKMC_ERR_CONFIG, "alloy must be a map in config.");
+  p->va_count = random_on ? rnd->get_int("va_count", 0) : 0;
   const akmc::yaml::Node* alloy = rnd->get("alloy");
   if (!alloy || !alloy->is_map()) return fail(AKMC_ERR_CONFIG, "alloy must be a map in config.");
   p->n_alloy = 0;

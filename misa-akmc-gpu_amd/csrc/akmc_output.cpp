@@ -192,4 +192,52 @@ int akmc_read_dump(const char* path, int64_t box[3], int64_t* ghost, uint8_t* si
   return AKMC_OK;
 }
 
+int akmc_pipe_path_from_yaml(const char* path, char* out, int64_t cap) {
+  if (!path || !out || cap <= 0) return fail(AKMC_ERR_INVALID, "bad arguments");
+  akmc::yaml::Node root;
+  std::string err;
+  if (!akmc::yaml::parse_file(path, &root, &err)) return fail(AKMC_ERR_CONFIG, "%s: %s", path, err.c_str());
+  // ConfigParsing::parseCreate (profiles/config/config_parsing.cpp:150-188):
+  // create.random wins when enabled; otherwise create.pipe.input_box.
+  const akmc::yaml::Node* create = root.get("create");
+  const akmc::yaml::Node* rnd = create ? create->get("random") : nullptr;
+  const akmc::yaml::Node* pipe = create ? create->get("pipe") : nullptr;
+  if ((rnd && rnd->get_bool("enable", false)) || !pipe || !pipe->get_bool("enable", false))
+    return fail(AKMC_ERR_CONFIG, "create.pipe is not the create option of %s", path);
+  const akmc::yaml::Node* f = pipe->get("input_box");
+  const std::string v = f ? f->scalar : std::string();
+  if ((int64_t)v.size() >= cap) return fail(AKMC_ERR_INVALID, "path longer than %lld bytes", (long long)cap - 1);
+  memcpy(out, v.c_str(), v.size() + 1);
+  return AKMC_OK;
+}
+
+int akmc_pipe_sites(const akmc_params* p, const char* pipe_file, uint8_t* sites, int64_t n, int64_t* n_vacancies) {
+  if (!p || !pipe_file || !sites) return fail(AKMC_ERR_INVALID, "null argument");
+  // creation::createFromPile (src/creation.cpp:184-230): the atoms of
+  // createAtomsRandom (the random creation's draw without its vacancies) ...
+  akmc_params q = *p;
+  q.va_count = 0;
+  const int rc = akmc_random_sites(&q, sites, n);
+  if (rc != AKMC_OK) return rc;
+  // ... then one vacancy per "x y z" triple of the file (x doubled, real
+  // coordinates), read until the first that does not parse; triples outside
+  // the box are skipped (the local sub-box test of :216, one rank).  The
+  // reference writes the site's type as V, so the atom is replaced.
+  std::ifstream in(pipe_file);
+  if (!in.good()) return fail(AKMC_ERR_INVALID, "cannot open pipe file %s", pipe_file);
+  const int64_t X = 2 * p->box[0], Y = p->box[1], Z = p->box[2];
+  long long x, y, z;
+  while (in >> x >> y >> z) {
+    if (x < 0 || x >= X || y < 0 || y >= Y || z < 0 || z >= Z) continue;
+    uint8_t& f = sites[x + X * (y + Y * z)];
+    if (!(f & akmc::BIT_V)) f = akmc::BIT_V;
+  }
+  if (n_vacancies) {
+    int64_t c = 0;
+    for (int64_t i = 0; i < n; i++) c += (sites[i] & akmc::BIT_V) ? 1 : 0;
+    *n_vacancies = c;
+  }
+  return AKMC_OK;
+}
+
 }  // extern "C"

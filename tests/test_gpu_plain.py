@@ -341,3 +341,27 @@ def test_config0_full_length_hashes():
     assert np.array_equal(_bits(st["evolution"]), _bits(z["evolution"]))
     assert st["sector_time"] == float(z["sector_time"]) and st["step"] == int(z["step"])
     eng.close()
+
+
+def test_pipe_created_lattice_bit_exact(tmp_path):
+    # Pipe creation: the random creation's atoms and vacancies from a file;
+    # the engine and the oracle start from the same sites and must agree.
+    cfg = make_cfg(n=16, va=0)
+    rng = np.random.default_rng(9)
+    f = tmp_path / "vac.txt"
+    f.write_text("".join("%d %d %d\n" % (rng.integers(32), rng.integers(16), rng.integers(16)) for _ in range(14)))
+    sites = akmc.pipe_sites(cfg, f)
+    nv = int((sites & akmc.BIT_V).sum())
+    say("pipe case: 16^3, %d vacancies" % nv)
+    assert 10 <= nv <= 14
+    eng = akmc.Engine(cfg, init_random=False)
+    eng.upload(sites)
+    ref = refcpu.RefCpu(cfg, create=False)
+    ref.set_sites(sites)
+    st = eng.run(4000)
+    ref.run(4000)
+    rs = ref.state()
+    assert st["events"] == rs["events"] == 4000
+    assert np.array_equal(eng.download(), ref.sites())
+    assert np.array_equal(_bits(st["evolution"]), _bits(rs["evolution"]))
+    eng.close()
