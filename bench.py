@@ -232,7 +232,7 @@ def served_leg(args, batched_value):
             "p50_latency_s": q(0.50), "p99_latency_s": q(0.99), "max_latency_s": lat[-1],
             "makespan_s": span, "gpu_busy_frac": st["busy_seconds"] / span, "launches": st["launches"],
             "hbm_peak_bytes": st["bytes_peak"], "failed": st["failed"],
-            "data": "synthetic jobs; vacancy-only (self-interstitial dumbbells are not modelled)"}
+            "data": "synthetic jobs, vacancy hops only (the reference defines no dumbbell event loop; dumbbell rates are the itl leg)"}
 
 
 def itl_leg(args):

@@ -83,7 +83,7 @@ def test_read_dump_errors(tmp_path):
         akmc.read_dump(p)          # count mismatch
     p.write_text("2\n\nFeCu\t2\t2\t2\nV\t2.5\t2.5\t2.5\n")
     with pytest.raises(akmc.AkmcError):
-        akmc.read_dump(p)          # dumbbells are not modelled
+        akmc.read_dump(p)          # dumbbell types have no place in the engine's site bytes
     with pytest.raises(akmc.AkmcError):
         akmc.read_dump(tmp_path / "missing.out")
 
