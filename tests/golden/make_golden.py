@@ -4,6 +4,7 @@ events' (from, to) site indices, a SHA-256 of the final site array, the
 sector clocks and the partial sector time.  Run from the repository root:
     python3 tests/golden/make_golden.py        # traj_*.npz
     python3 tests/golden/make_golden.py cfg0   # hashes_cfg0_40cube.npz (1e5 events, a few minutes)
+    python3 tests/golden/make_golden.py itl    # itl_dense_6x4x9.npz (dumbbell rates)
 """
 import hashlib
 import json
@@ -48,9 +49,36 @@ def cfg0_hashes():
     print("cfg0 hashes", st["events"], "events")
 
 
+def itl_rates():
+    """Dumbbell rates of a dense typed lattice (6 x 4 x 9 cells, 10 % dumbbell
+    sites of every kind and orientation, Cu/Ni/Mn atoms and vacancies) from
+    the oracle's restatement of the reference's dumbbell model."""
+    rng = np.random.default_rng(2024)
+    box = (6, 4, 9)
+    n = 2 * box[0] * box[1] * box[2]
+    u = rng.random(n)
+    t = np.full(n, 1, dtype=np.uint16)
+    t[u < 0.17] = 0
+    t[u < 0.15] = 8
+    t[u < 0.10] = 4
+    t[u < 0.05] = 2
+    sites = np.sort(rng.choice(n, size=n // 10, replace=False)).astype(np.int64)
+    kinds = np.array([0x0101, 0x0102, 0x0104, 0x0108, 0x0202, 0x0204, 0x0208, 0x0404, 0x0408, 0x0808], np.uint16)
+    t[sites] = kinds[np.arange(sites.size) % kinds.size]
+    orient = np.array([-6, -5, -4, -3, -2, -1, 1, 2, 3, 4, 5, 6], np.int8)[np.arange(sites.size) % 12]
+    nu, T = 6e12, 563.0
+    av, r = refcpu.Itl().rates(box, t, sites, orient, nu, T)
+    np.savez_compressed(os.path.join(HERE, "itl_dense_6x4x9.npz"), box=np.array(box, np.int64), types=t,
+                        sites=sites, orient=orient, nu=np.float64(nu), T=np.float64(T), avail=av, rates=r)
+    print("itl", sites.size, "dumbbells", int((r > 0).sum()), "non-zero rate slots")
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "cfg0":
         cfg0_hashes()
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "itl":
+        itl_rates()
         return
     for name, (cfg, n) in CASES.items():
         r = refcpu.RefCpu(cfg)

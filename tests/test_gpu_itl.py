@@ -78,3 +78,13 @@ def test_runs_beside_an_engine():
     assert np.array_equal(av1, av2) and _same(r1, r2)
     a.close()
     b.close()
+
+
+def test_golden_fixture():
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "itl_dense_6x4x9.npz"))
+    box = tuple(int(x) for x in g["box"])
+    print("itl case: golden fixture", box, len(g["sites"]), flush=True)
+    av, r = akmc.itl_rates(box, g["types"], g["sites"], g["orient"], float(g["nu"]), float(g["T"]))
+    assert np.array_equal(av, g["avail"])
+    assert _same(r, g["rates"])

@@ -288,3 +288,14 @@ def test_fecu_rates_in_pure_iron(itl):
         k_fe = nu * math.exp(-(0.32 + de_fe / 2) / (KB * T))
         assert r[0][lo] == r[0][lo + 1] == k_fe / 2
     assert r[0][4] == r[0][5] == r[0][6] == r[0][7] == k_cu / 2
+
+
+def test_golden_fixture(itl):
+    # The committed fixture (tests/golden/make_golden.py itl) still comes out
+    # of the oracle bit for bit; the GPU test checks the kernels against it.
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "itl_dense_6x4x9.npz"))
+    av, r = itl.rates(tuple(g["box"]), g["types"], g["sites"], g["orient"], float(g["nu"]), float(g["T"]))
+    assert np.array_equal(av, g["avail"])
+    assert np.array_equal(r.view(np.uint64), g["rates"].view(np.uint64))
+    assert (g["rates"] > 0).sum() > 50
