@@ -310,8 +310,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=0.0, help="offered jobs/s (0: 70%% of batched capacity)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (one lattice per GPU; replica and job "
-              "sharding across GPUs is described in DESIGN.md); --gpus %d is not supported" % args.gpus,
+        print("bench.py: runs on one GPU only (one lattice, batch or server per GPU; sharding replicas, "
+              "jobs or sub-boxes across GPUs is not built); --gpus %d is not supported" % args.gpus,
               file=sys.stderr)
         return 2
 
