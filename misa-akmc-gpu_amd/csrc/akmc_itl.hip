@@ -294,9 +294,17 @@ __global__ __launch_bounds__(64) void itl_prefix_kernel(Args a) {
     t_next = j + 64 < a.n ? a.term[j + 64] : 0.0;
     const int cnt = (int)min<int64_t>(64, a.n - base);
     double mine = 0;
-    for (int i = 0; i < cnt; i++) {
-      acc += read_lane(t, i);
-      mine = l == i ? acc : mine;
+    if (cnt == 64) {
+#pragma unroll
+      for (int i = 0; i < 64; i++) {
+        acc += read_lane(t, i);
+        mine = l == i ? acc : mine;
+      }
+    } else {
+      for (int i = 0; i < cnt; i++) {
+        acc += read_lane(t, i);
+        mine = l == i ? acc : mine;
+      }
     }
     if (j < a.n) a.prefix[j] = mine;
   }
@@ -331,6 +339,7 @@ __device__ __forceinline__ bool near(const Dims& d, int x, int y, int z, int cx,
 // dumbbell's neighbour list); Edumb before is the full prefix fold.
 constexpr int RATE_THREADS = 256;
 __global__ __launch_bounds__(RATE_THREADS) void itl_rate_kernel(Args a) {
+  const double* tm = a.term;
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t k = g >> 3;
   const int b = (int)(g & 7);
@@ -372,12 +381,12 @@ __global__ __launch_bounds__(RATE_THREADS) void itl_rate_kernel(Args a) {
         sa = ja > 0 ? a.prefix[ja - 1] : 0.0;
         j = ja;
       }
-      sa = fold_terms(sa, a.term, j, ja);
+      sa = fold_terms(sa, tm, j, ja);
       const int64_t sj = site_of(a.d, qj.x, qj.y, qj.z);
       sa += edumb_term(a.d, L1, qj.x, qj.y, qj.z, L1.at(sj), (qj.w >> 24) - 6, (uint8_t)((qj.w >> 16) & 0xFF));
       j = ja + 1;
     }
-    if (j >= 0) sa = fold_terms(sa, a.term, j, a.n);
+    if (j >= 0) sa = fold_terms(sa, tm, j, a.n);
     const double e_before = e_src_b + e_des_b + sb;
     const double e_after = e_src_a + e_des_a + sa;
     const double e0 = ghost == T_FE ? 0.32 : ghost == T_CU ? 0.36 : ghost == T_NI ? 0.45 : ghost == T_MN ? 0.31 : 0.0;
