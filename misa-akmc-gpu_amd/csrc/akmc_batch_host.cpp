@@ -543,7 +543,7 @@ int run_slices(akmc_batch* b, int64_t launches, int64_t* events, double* ksec) {
 
 int set_targets(akmc_batch* b, int64_t events) {
   for (auto& s : b->h_state)
-    if (s.status == AKMC_REP_RUNNING) s.target = s.events + events;
+    if (s.status == AKMC_REP_RUNNING) s.target = events > INT64_MAX - s.events ? INT64_MAX : s.events + events;
   return write_states(b);
 }
 
@@ -558,7 +558,7 @@ int advance(akmc_batch* b, int64_t events, int64_t* total, double* ksec, int64_t
   *total = 0;
   *ksec = 0;
   *launches = 0;
-  int64_t need = (events + b->R.tun.slice - 1) / b->R.tun.slice;
+  int64_t need = events / b->R.tun.slice + (events % b->R.tun.slice != 0 ? 1 : 0);
   while (need > 0) {
     int64_t ev = 0;
     double ks = 0;

@@ -534,7 +534,8 @@ int akmc_run(akmc_engine* e, int64_t max_events, akmc_stats* stats) {
   if (max_events < 0) return fail(AKMC_ERR_INVALID, "max_events < 0");
   HIP_TRY(hipSetDevice(e->device));
   const auto t0 = std::chrono::steady_clock::now();
-  const int64_t cap = e->h_state.events + max_events;
+  // Saturating: any max_events up to INT64_MAX means "no cap" beyond it.
+  const int64_t cap = max_events > INT64_MAX - e->h_state.events ? INT64_MAX : e->h_state.events + max_events;
   double ksec = 0.0;
   int64_t launches = 0;
   int stop = 0;

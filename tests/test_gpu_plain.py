@@ -365,3 +365,17 @@ def test_pipe_created_lattice_bit_exact(tmp_path):
     assert np.array_equal(eng.download(), ref.sites())
     assert np.array_equal(_bits(st["evolution"]), _bits(rs["evolution"]))
     eng.close()
+
+
+def test_run_with_int64_max_event_cap():
+    # INT64_MAX as max_events means "no cap": the run ends with the time loop.
+    say("test_run_with_int64_max_event_cap")
+    cfg = make_cfg(n=16, va=12, T=600.0, sector_T=3e-7, physics=5 * 3e-7)
+    eng = akmc.Engine(cfg)
+    st = eng.run((1 << 63) - 1)
+    ref = refcpu.RefCpu(cfg)
+    ref.run(10 ** 9)
+    rs = ref.state()
+    assert st["stop"] == akmc.AKMC_STOP_TIME and st["events"] == rs["events"] > 0
+    assert np.array_equal(eng.download(), ref.sites())
+    eng.close()
