@@ -59,6 +59,15 @@ inline ConfigValues parseConfig(const std::string& path) {
   return c;
 }
 
+/* dev_Vacancy (gpu/DeviceVacRatesSolver.h:7-14): one vacancy's 8 direction
+ * rates; id is the real-site index x + 2 nx (y + ny z) (the reference keeps
+ * its ghost-extended local id). */
+struct dev_Vacancy {
+  int64_t id;
+  double rates[8];
+  int valid;
+};
+
 /* Called after every finished time step (step counted from 0) and once when
  * the time loop has ended. */
 class EventHooks {
@@ -107,6 +116,24 @@ class ABVIModel {
       }
       cap = n;
     }
+  }
+
+  /* ABVIModel::calcRatesGPU's result for one sector (models/abvi/kmc.cpp:
+   * 118-179, first_calculate_GPU / calculate_GPU of gpu/gpu_simulate.h:33-35)
+   * as the reference's device records:
+   * every vacancy of the sector with its 8 rates; *total the sector total. */
+  std::vector<dev_Vacancy> calcRatesGPU(int sector, double* total = nullptr) {
+    std::vector<int64_t> s;
+    std::vector<double> r;
+    const double t = calcRates(sector, &s, &r);
+    if (total) *total = t;
+    std::vector<dev_Vacancy> v(s.size());
+    for (size_t i = 0; i < s.size(); i++) {
+      v[i].id = s[i];
+      for (int b = 0; b < 8; b++) v[i].rates[b] = r[8 * i + b];
+      v[i].valid = 1;
+    }
+    return v;
   }
 
   /* ModelAdapter::defectSize: the number of vacancies. */

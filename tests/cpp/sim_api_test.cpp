@@ -64,6 +64,14 @@ int simulate(const char* path, long hook_steps, const char* out) {
   std::vector<double> rates;
   const double total = model.calcRates(0, &sites, &rates);
   printf("sector0_vacancies=%zu\nsector0_total=%.17g\n", sites.size(), total);
+  double total2 = 0.0;
+  const std::vector<akmc_sim::dev_Vacancy> dv = model.calcRatesGPU(0, &total2);
+  bool same = dv.size() == sites.size() && total2 == total;
+  for (size_t i = 0; same && i < dv.size(); i++) {
+    same = dv[i].id == sites[i] && dv[i].valid == 1;
+    for (int b = 0; b < 8; b++) same = same && dv[i].rates[b] == rates[8 * i + b];
+  }
+  printf("dev_vacancy_check=%s\n", same ? "ok" : "differs");
   return 0;
 }
 

@@ -76,7 +76,7 @@ def test_simulate_random_creation(tmp_path, hook_steps):
     steps = math.ceil(7.5)
     assert out["step"] == str(steps) and out["all_done"] == "1" and out["seed_check"] == "ok"
     assert out["hook_steps"] == ",".join(str(s) for s in range(steps) if s % hook_steps == 0)
-    assert out["defects"] == "30"
+    assert out["defects"] == "30" and out["dev_vacancy_check"] == "ok"
     sites = np.fromfile(tmp_path / "sites.bin", dtype=np.uint8)
     ref_sites, rs = _oracle_final(cfg)
     assert int(out["events"]) == rs["events"] and rs["step"] == steps
