@@ -127,3 +127,16 @@ def test_invalid_arguments_return_status():
     out = np.zeros(10, dtype=np.uint8)
     rc = akmc.lib().akmc_random_sites(akmc.ctypes.byref(prm), out.ctypes.data, out.size)
     assert rc == -1 and b"site count" in akmc.lib().akmc_last_error()
+
+
+def test_missing_library_fails_loudly(tmp_path, monkeypatch):
+    # No fallback: without libakmc.so every entry point raises.
+    import akmc
+    monkeypatch.setattr(akmc, "_lib", None)
+    monkeypatch.setattr(akmc, "LIB_PATH", str(tmp_path / "libakmc.so"))
+    with pytest.raises(ImportError):
+        akmc.lib()
+    with pytest.raises(ImportError):
+        akmc.Engine(make_cfg(n=8, va=2))
+    with pytest.raises(ImportError):
+        akmc.itl_rates((4, 4, 4), [1] * 128, [], [], 6e12, 573.0)
