@@ -235,6 +235,51 @@ def served_leg(args, batched_value):
             "data": "synthetic jobs, vacancy hops only (the reference defines no dumbbell event loop; dumbbell rates are the itl leg)"}
 
 
+CONFIG0 = dict(box=(40, 40, 40), alloy=[("Fe", 9666), ("Cu", 134), ("Mn", 130), ("Ni", 70)], va_count=26,
+               seeds=dict(create_types=100565, create_vacancy=16650, event_selection=563264, time_inc=594689),
+               temperature=573.0, attempt_freq=6e12, sector_T=3e-7, physics_time=1e3)
+CONFIG0_EVENTS = 100000
+
+
+def config0_line(args):
+    """The reference's CPU-runnable case (40^3, 26 vacancies, sector T 3e-7 s,
+    1e5 events from creation) on the plain engine, and the oracle over the
+    same 1e5 events as the CPU baseline, with the final lattices compared."""
+    import hashlib
+    import akmc
+    eng = akmc.Engine(CONFIG0, device=0)
+    per = CONFIG0_EVENTS // args.steps
+    akmc.sync(0)
+    t0 = time.perf_counter()
+    ksec = 0.0
+    for _ in range(args.steps):
+        st = eng.run(per)
+        ksec += st["kernel_seconds"]
+    akmc.sync(0)
+    dt = time.perf_counter() - t0
+    events = st["events"]
+    gpu_hash = hashlib.sha256(eng.download().tobytes()).hexdigest()
+    eng.close()
+    out = {"metric": METRIC, "value": events / dt, "unit": "events/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": 0, "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "plain FP64 AKMC, the reference's CPU case: BCC 40^3 cells (128,000 sites), "
+                                  "Fe-1.34Cu-1.30Mn-0.70Ni at%, 26 vacancies, T=573 K, T_sector=3e-7 s, "
+                                  "1e5 events from creation", "events_timed": events},
+           "kernel_us_per_event": ksec / max(1, events) * 1e6}
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import refcpu
+        ref = refcpu.RefCpu(CONFIG0)
+        t0 = time.perf_counter()
+        ref.run(events)
+        cdt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": events / cdt, "unit": "events/s", "cores": 1, "kind": "port",
+                               "sample": "the same %d events, oracle/refcpu.cpp, 1 thread, %.1f s" % (events, cdt)}
+        out["matches_oracle"] = hashlib.sha256(ref.sites().tobytes()).hexdigest() == gpu_hash
+    return out
+
+
 def itl_leg(args):
     """Dumbbell (self-interstitial) rates on one serving job's lattice: the
     rate step of the abvi dumbbell model (akmc_itl_rates), device time of its
@@ -300,6 +345,8 @@ def main():
     ap.add_argument("--batch-warmup", type=int, default=1)
     ap.add_argument("--no-fixed", action="store_true", help="batched leg: skip the fixed-event run")
     ap.add_argument("--no-error-stats", action="store_true", help="batched leg: skip the FP32 error sample")
+    ap.add_argument("--config0", action="store_true",
+                    help="instead: the plain engine and the CPU baseline on the reference's 40^3 case, 1e5 events")
     ap.add_argument("--no-itl", action="store_true", help="skip the dumbbell-rate leg")
     ap.add_argument("--itl-edge", type=int, default=256)
     ap.add_argument("--itl-fraction", type=float, default=1e-5)
@@ -315,6 +362,9 @@ def main():
               file=sys.stderr)
         return 2
 
+    if args.config0:
+        print(json.dumps(config0_line(args)))
+        return 0
     import akmc
     out = {"metric": METRIC, "value": None, "unit": "events/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
