@@ -48,6 +48,20 @@ WORKLOAD = dict(box=(256, 256, 256), alloy=[("Fe", 9666), ("Cu", 134), ("Mn", 13
                 temperature=573.0, attempt_freq=6e12, sector_T=1e-6, physics_time=1.0)
 
 
+def host_cpu():
+    """The host CPU model and its logical CPU count, for the CPU baselines."""
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return "%s, %d logical CPUs" % (model, os.cpu_count() or 0)
+
+
 def algorithmic_bytes(d_events, d_eval, d_fold, d_shift):
     """Per-event algorithmic traffic of the plain path:
     B = n_eval*(55 + 8*8) + 8*K + 2 + 16 per event, with
@@ -73,7 +87,7 @@ def cpu_baseline(events, seconds_budget):
         ref.run(10)
         done = ref.state()["events"]
     dt = time.perf_counter() - t0
-    return {"value": done / dt, "unit": "events/s", "cores": 1, "kind": "port",
+    return {"value": done / dt, "unit": "events/s", "cores": 1, "kind": "port", "host": host_cpu(),
             "sample": "first %d events of the same 256^3 FP64 workload, oracle/refcpu.cpp "
                       "(reference CPU algorithm: hash occupancy, full-sector sweep per event), "
                       "1 thread, initial occupancy excluded, %.1f s" % (done, dt)}
@@ -99,7 +113,7 @@ def cpu_batched_baseline(cfgs, seconds):
     with ProcessPoolExecutor(max_workers=cores, mp_context=mp.get_context("spawn")) as ex:
         res = list(ex.map(_oracle_replica, cfgs[:cores], [seconds] * cores))
     value = sum(e / t for e, t in res)
-    return {"value": value, "unit": "events/s", "cores": cores, "kind": "port",
+    return {"value": value, "unit": "events/s", "cores": cores, "kind": "port", "host": host_cpu(),
             "sample": "oracle/refcpu.cpp on the first %d replicas of the same batch, one process per core, "
                       "%.0f s each, %d events in total" % (cores, seconds, sum(e for e, _ in res))}
 
@@ -275,7 +289,7 @@ def config0_line(args):
         ref.run(events)
         cdt = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": events / cdt, "unit": "events/s", "cores": 1, "kind": "port",
-                               "sample": "the same %d events, oracle/refcpu.cpp, 1 thread, %.1f s" % (events, cdt)}
+                               "host": host_cpu(), "sample": "the same %d events, oracle/refcpu.cpp, 1 thread, %.1f s" % (events, cdt)}
         out["matches_oracle"] = hashlib.sha256(ref.sites().tobytes()).hexdigest() == gpu_hash
     return out
 
@@ -312,7 +326,8 @@ def itl_leg(args):
         av_o, r_o = o.rates(box, types, sites, orient, 6e12, 573.0)
         dt = time.perf_counter() - t0
         leg["cpu_baseline"] = {"value": nrates // max(1, calls) / dt, "unit": "dumbbell rates/s", "cores": 1,
-                               "kind": "port", "sample": "oracle/refcpu.cpp (refitl) on the same lattice, one call, "
+                               "kind": "port", "host": host_cpu(),
+                               "sample": "oracle/refcpu.cpp (refitl) on the same lattice, one call, "
                                                          "%.2f s" % dt}
         leg["matches_oracle"] = bool((av_o == av).all() and (r_o.view("u8") == r.view("u8")).all())
     return leg
