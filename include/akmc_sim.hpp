@@ -165,6 +165,42 @@ class ABVIModel {
   akmc_engine* e_ = nullptr;
 };
 
+/* Itl (models/abvi/defect/itl_list.h): a dumbbell of the interstitial list,
+ * with its orientation (orientation.h:12-30, -6..6 without 0), the directions
+ * it can jump to and its 8 rate slots (two per orientation direction). */
+struct Itl {
+  int8_t orient = 0;
+  uint8_t avail_trans_dir = 0;
+  double rates[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+/* Itl::beforeRatesUpdate + Itl::updateRates with ItlRatesSolver::rate as the
+ * callback (itl_list.cpp:25-50, rate/itl_rates_solver.cpp:16-80) for every
+ * dumbbell of `list` (site index -> Itl, ascending sites as ItlList::mp).
+ * types: one LatticeTypes value per real site of the periodic box. */
+template <class Map>
+inline void updateItlRates(const int64_t box[3], const std::vector<uint16_t>& types, Map& list, double attempt_freq,
+                           double temperature, int device) {
+  std::vector<int64_t> sites;
+  std::vector<int8_t> ori;
+  sites.reserve(list.size());
+  ori.reserve(list.size());
+  for (const auto& kv : list) {
+    sites.push_back((int64_t)kv.first);
+    ori.push_back(kv.second.orient);
+  }
+  std::vector<uint8_t> avail(sites.size());
+  std::vector<double> rates(8 * sites.size());
+  check(akmc_itl_rates(box, types.data(), (int64_t)sites.size(), sites.data(), ori.data(), attempt_freq, temperature,
+                       device, avail.data(), rates.data(), nullptr));
+  size_t k = 0;
+  for (auto& kv : list) {
+    kv.second.avail_trans_dir = avail[k];
+    for (int b = 0; b < 8; b++) kv.second.rates[b] = rates[8 * k + b];
+    k++;
+  }
+}
+
 namespace simulation {
 
 /* simulation::simulate (src/simulation.h:61-85): runs the sublattice time

@@ -4,8 +4,11 @@
 // against the oracle.
 //   sim_api_test config <config.yaml>
 //   sim_api_test simulate <config.yaml> <hook_steps> <sites_out>
+//   sim_api_test itl
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -75,11 +78,38 @@ int simulate(const char* path, long hook_steps, const char* out) {
   return 0;
 }
 
+// Dumbbell rates through the C++ layer: FeFe dumbbells in pure Fe, one per
+// orientation, far apart.  Each jump leaves the bond sums unchanged (the same
+// two sums in the other order) and the dumbbell energy zero, so every rate
+// slot is nu exp(-0.32 / kB T) / 2.
+int itl() {
+  const int64_t box[3] = {24, 6, 6};
+  const int64_t X = 2 * box[0];
+  std::vector<uint16_t> types((size_t)(X * box[1] * box[2]), 1);
+  std::map<int64_t, akmc_sim::Itl> list;
+  const int8_t oris[6] = {1, -2, 3, -4, 5, -6};
+  for (int i = 0; i < 6; i++) {
+    const int64_t s = 8 * i + X * (2 + box[1] * 3);
+    types[(size_t)s] = 0x0101;
+    list[s].orient = oris[i];
+  }
+  akmc_sim::updateItlRates(box, types, list, 6e12, 573.0, 0);
+  const double want = 6e12 * std::exp(-0.32 / (8.617330350E-5 * 573.0)) / 2;
+  int bad = 0;
+  for (const auto& kv : list) {
+    bad += __builtin_popcount(kv.second.avail_trans_dir) != 4;
+    for (int b = 0; b < 8; b++) bad += kv.second.rates[b] != want;
+  }
+  printf("itl_dumbbells=%zu\nitl_check=%s\n", list.size(), bad ? "differs" : "ok");
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
   try {
     if (argc == 3 && std::string(argv[1]) == "config") return config(argv[2]);
+    if (argc == 2 && std::string(argv[1]) == "itl") return itl();
     if (argc == 5 && std::string(argv[1]) == "simulate") return simulate(argv[2], atol(argv[3]), argv[4]);
   } catch (const akmc_sim::Error& e) {
     printf("error=%d %s\n", e.code, e.what());

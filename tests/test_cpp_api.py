@@ -99,3 +99,9 @@ def test_simulate_pipe_creation(tmp_path):
     ref_sites, rs = _oracle_final(cfg, start)
     assert int(out["events"]) == rs["events"] and out["hook_steps"] == "0,1,2,3"
     assert np.array_equal(sites, ref_sites)
+
+
+@pytest.mark.gpu
+def test_dumbbell_rates_through_the_cpp_layer():
+    out = _run("itl")
+    assert out["itl_dumbbells"] == "6" and out["itl_check"] == "ok"
