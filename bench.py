@@ -40,7 +40,7 @@ HBM_PEAK_GBS = 8000.0
 # same workloads (FETCH_SIZE scaled by 2.0: it reads 524,306 KiB for the
 # 1 GiB read of the library's copy kernel; WRITE_SIZE reads it exactly):
 # profiles/plain_fp64_256cube_hbm_counters.md, profiles/replica_fp32_1024_hbm_counters.md.
-PLAIN_TRAFFIC_BYTES_PER_EVENT = 75.1
+PLAIN_TRAFFIC_BYTES_PER_EVENT = 75.0
 REPLICA_TRAFFIC_BYTES_PER_EVENT = 248.9
 
 WORKLOAD = dict(box=(256, 256, 256), alloy=[("Fe", 9666), ("Cu", 134), ("Mn", 130), ("Ni", 70)], va_count=6711,
