@@ -60,6 +60,15 @@ def test_many_dumbbells():
     _check(box, t, sites, orient)
 
 
+def test_dense_dumbbells_many_chunks():
+    # ~1,400 dumbbells (5 % of the sites): 22 chunks of the prefix fold and
+    # long neighbour lists.
+    box = (24, 24, 24)
+    t, sites, orient = typed_lattice(box, 13, 0.05, solutes=(0.05, 0.05, 0.05), f_v=0.01)
+    assert len(sites) > 1300
+    _check(box, t, sites, orient, T=540.0)
+
+
 def test_runs_beside_an_engine():
     # The call uses a stream of its own: an engine on the same device keeps
     # its trajectory when dumbbell rates are computed between its runs.
