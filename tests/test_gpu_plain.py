@@ -379,3 +379,27 @@ def test_run_with_int64_max_event_cap():
     assert st["stop"] == akmc.AKMC_STOP_TIME and st["events"] == rs["events"] > 0
     assert np.array_equal(eng.download(), ref.sites())
     eng.close()
+
+
+def test_engines_in_host_threads_are_independent():
+    # One engine per host thread, each on its own stream: the same results as
+    # the engines run one after another, and as the oracle.
+    from concurrent.futures import ThreadPoolExecutor
+    say("test_engines_in_host_threads_are_independent")
+    cfgs = [make_cfg(n=16, va=10 + k, T=560.0 + 10 * k, seeds=dict(create_types=40 + k, create_vacancy=50 + k,
+                                                                     time_inc=60 + k)) for k in range(6)]
+
+    def run(c):
+        e = akmc.Engine(c)
+        st = e.run(3000)
+        s = e.download()
+        e.close()
+        return st["events"], s
+
+    with ThreadPoolExecutor(6) as ex:
+        par = list(ex.map(run, cfgs))
+    for c, (n, s) in zip(cfgs, par):
+        ref = refcpu.RefCpu(c)
+        ref.run(3000)
+        assert n == ref.state()["events"] == 3000
+        assert np.array_equal(s, ref.sites())
