@@ -1,7 +1,7 @@
 """Precision study of the FP32 rate form (512^3 workload, 16 seeds).
 
     python3 misa-akmc-gpu_amd/akmc/studies.py precision --L 512 --seeds 16 --events 10000000 --out result.json
-    python3 misa-akmc-gpu_amd/akmc/studies.py plain --L 64 --seeds 16 --events 1000000 --out result.json
+    python3 misa-akmc-gpu_amd/akmc/studies.py plain --L 512 --seeds 16 --events 10000000 --threads 8 --out result.json
 
 `precision`: the same seeds (initial lattices from creation::createRandom)
 advanced by the replica engine with FP32 rates and with FP64 rates (the
@@ -90,7 +90,7 @@ def precision(L, seeds, events, hist_cap, chunk, log):
     return res
 
 
-def plain(L, seeds, events, hist_cap, log):
+def plain(L, seeds, events, hist_cap, log, threads=1):
     cfgs = []
     for k in range(seeds):
         c = workloads.precision_512(k, L)
@@ -99,18 +99,33 @@ def plain(L, seeds, events, hist_cap, log):
     box = cfgs[0]["box"]
     res = {"workload": "BCC %d^3, %d vacancies, T=563 K, %d seeds, %d events each" % (L, cfgs[0]["va_count"], seeds,
                                                                                        events)}
-    per_plain = []
     t0 = time.perf_counter()
-    for k, c in enumerate(cfgs):
-        e = akmc.Engine(c)
-        e.run(events)
-        per_plain.append(_stats(e.download(), box, hist_cap))
+
+    def one(k):
+        # One engine per host thread: each runs on its own stream.
+        e = akmc.Engine(cfgs[k])
+        done = 0
+        while done < events:
+            n = min(1_000_000, events - done)
+            e.run(n)
+            done += n
+            log("plain: seed %d, %d / %d events, %.1f s" % (k, done, events, time.perf_counter() - t0))
+        s = e.download()
         e.close()
-        log("plain: seed %d done, %.1f s" % (k, time.perf_counter() - t0))
+        return _stats(s, box, hist_cap)
+
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max(1, threads)) as ex:
+        per_plain = list(ex.map(one, range(len(cfgs))))
     res["plain_fp64"] = {"seconds": time.perf_counter() - t0, "summary": _summary(per_plain), "per_seed": per_plain}
     t0 = time.perf_counter()
     b = akmc.Batch(cfgs, precision=akmc.AKMC_PREC_FP32, init=akmc.AKMC_INIT_REFERENCE)
-    b.run(events)
+    done = 0
+    while done < events:
+        n = min(1_000_000, events - done)
+        b.run(n)
+        done += n
+        log("replica fp32: %d / %d events per seed, %.1f s" % (done, events, time.perf_counter() - t0))
     per_b = [_stats(b.download(i), box, hist_cap) for i in range(seeds)]
     b.close()
     log("replica fp32 done, %.1f s" % (time.perf_counter() - t0))
@@ -128,12 +143,13 @@ def main():
     ap.add_argument("--chunk", type=int, default=1_000_000)
     ap.add_argument("--hist-cap", type=int, default=64)
     ap.add_argument("--out", default="")
+    ap.add_argument("--threads", type=int, default=1, help="plain study: engines run at once, one per host thread")
     a = ap.parse_args()
     log = lambda m: print(m, file=sys.stderr, flush=True)
     if a.study == "precision":
         res = precision(a.L, a.seeds, a.events, a.hist_cap, a.chunk, log)
     else:
-        res = plain(a.L, a.seeds, a.events, a.hist_cap, log)
+        res = plain(a.L, a.seeds, a.events, a.hist_cap, log, a.threads)
     text = json.dumps(res)
     if a.out:
         with open(a.out, "w") as f:
