@@ -23,3 +23,14 @@ def test_precision_study_small():
     assert res["identical_final_lattices"] >= 4
     for k, t in res["welch_t"].items():
         assert abs(t) < 4.0, (k, t)
+
+
+def test_plain_study_in_threads_small():
+    # The plain-vs-FP32 study with its exact engines in host threads: the
+    # same per-seed results as with one engine at a time.
+    print("test_plain_study_in_threads_small", flush=True)
+    a = studies.plain(L=24, seeds=4, events=4000, hist_cap=32, log=lambda m: None, threads=1)
+    b = studies.plain(L=24, seeds=4, events=4000, hist_cap=32, log=lambda m: None, threads=4)
+    assert a["plain_fp64"]["per_seed"] == b["plain_fp64"]["per_seed"]
+    for k, t in b["welch_t"].items():
+        assert abs(t) < 6.0, (k, t)
