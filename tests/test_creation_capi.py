@@ -140,3 +140,19 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
         akmc.Engine(make_cfg(n=8, va=2))
     with pytest.raises(ImportError):
         akmc.itl_rates((4, 4, 4), [1] * 128, [], [], 6e12, 573.0)
+
+
+def test_header_is_plain_c(tmp_path):
+    # include/akmc.h is a C header: a C99 program compiles against it, links
+    # libakmc.so and calls an entry point that needs no device.
+    import subprocess
+    src = tmp_path / "c_abi.c"
+    src.write_text('#include "akmc.h"\n#include <stdio.h>\n'
+                   'int main(void) { akmc_params p; int rc = akmc_params_default(&p);\n'
+                   '  printf("%d %lld\\n", rc, (long long)p.box[0]); return rc; }\n')
+    lib = os.path.join(ROOT, "misa-akmc-gpu_amd", "lib")
+    exe = tmp_path / "c_abi"
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"), str(src),
+                    "-L", lib, "-lakmc", "-Wl,-rpath," + lib, "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and r.stdout.split()[0] == "0" and int(r.stdout.split()[1]) > 0
