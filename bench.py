@@ -9,9 +9,9 @@ akmc_run of --events-per-step events; the initial occupancy (host) and the
 upload are outside the timed region, so the lattice is resident in HBM when
 timing starts.  value = events performed in the timed steps / seconds.
 
-Beside the plain line, the same JSON line carries two more legs of the same
-metric (disable with --no-batched / --no-served) and the dumbbell-rate step
-(--no-itl):
+Beside the plain line, the same JSON line carries three more legs of the same
+metric (disable with --no-batched / --no-served / --plain-engines 0) and the
+dumbbell-rate step (--no-itl):
   batched  the replica engine on 1,024 independent replicas (edge U{16..96},
            random Fe-Cu-Ni-Mn compositions, T U[500,700] K, 2e-4 vacancies),
            FP32 rates with FP64 sums, autotuned, hipGraph replays; a "step"
@@ -20,6 +20,9 @@ metric (disable with --no-batched / --no-served) and the dumbbell-rate step
            vacancies U[1e-4,5e-4], targets log-uniform in [1e5,1e7] events scaled
            by --serve-scale), admitted by a 250 GB HBM budget; reports
            throughput and p50/p99 completion latency.
+  plain_engines  four plain FP64 engines at once (the 256^3 workload, seeds
+           differing), one host thread and stream each: aggregate events/s of
+           bit-exact trajectories.
   itl      the abvi dumbbell model's rate step (akmc_itl_rates) on a 256^3
            typed lattice with 1e-5 dumbbells: rates per second of kernel time.
 
@@ -294,6 +297,37 @@ def config0_line(args):
     return out
 
 
+def plain_engines_leg(args):
+    """Several plain FP64 engines on the 256^3 workload (seeds differ), one
+    per host thread, each on its own stream: the aggregate events/s of
+    bit-exact trajectories run at once."""
+    import akmc
+    from concurrent.futures import ThreadPoolExecutor
+    n = args.plain_engines
+    cfgs = []
+    for k in range(n):
+        c = dict(WORKLOAD)
+        c["seeds"] = {key: v + k for key, v in WORKLOAD["seeds"].items()}
+        cfgs.append(c)
+    with ThreadPoolExecutor(n) as ex:
+        engs = list(ex.map(lambda c: akmc.Engine(c, device=0), cfgs))
+    E = args.plain_engines_events
+    with ThreadPoolExecutor(n) as ex:
+        list(ex.map(lambda e: e.run(E // 10), engs))  # warmup
+    akmc.sync(0)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(n) as ex:
+        sts = list(ex.map(lambda e: e.run(E), engs))
+    akmc.sync(0)
+    dt = time.perf_counter() - t0
+    for e in engs:
+        e.close()
+    events = sum(E for _ in sts)
+    return {"value": events / dt, "unit": "events/s", "engines": n, "events_per_engine": E, "seconds": dt,
+            "kernel_us_per_event_per_engine": sum(s["kernel_seconds"] for s in sts) / events * 1e6,
+            "dtype": "f64", "note": "bit-exact plain engines, one host thread and stream each"}
+
+
 def itl_leg(args):
     """Dumbbell (self-interstitial) rates on one serving job's lattice: the
     rate step of the abvi dumbbell model (akmc_itl_rates), device time of its
@@ -363,6 +397,8 @@ def main():
     ap.add_argument("--config0", action="store_true",
                     help="instead: the plain engine and the CPU baseline on the reference's 40^3 case, 1e5 events")
     ap.add_argument("--no-itl", action="store_true", help="skip the dumbbell-rate leg")
+    ap.add_argument("--plain-engines", type=int, default=4, help="plain engines run at once (0: skip that leg)")
+    ap.add_argument("--plain-engines-events", type=int, default=500_000)
     ap.add_argument("--itl-edge", type=int, default=256)
     ap.add_argument("--itl-fraction", type=float, default=1e-5)
     ap.add_argument("--itl-calls", type=int, default=5)
@@ -377,6 +413,11 @@ def main():
               file=sys.stderr)
         return 2
 
+    # Streams of this process map onto HIP's hardware queues (4 by default,
+    # some taken by the runtime's own streams); the plain-engines leg runs one
+    # stream per engine, so ask for at least 8 before HIP starts.
+    if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
+        os.environ["GPU_MAX_HW_QUEUES"] = "8"
     if args.config0:
         print(json.dumps(config0_line(args)))
         return 0
@@ -432,6 +473,10 @@ def main():
     if not args.no_served:
         out["served"] = served_leg(args, batched["value"] if batched else 5e7)
         print("bench: served %.3g events/s" % out["served"]["value"], file=sys.stderr, flush=True)
+    if args.plain_engines > 0 and not args.no_plain:
+        out["plain_engines"] = plain_engines_leg(args)
+        print("bench: %d plain engines %.3g events/s" % (args.plain_engines, out["plain_engines"]["value"]),
+              file=sys.stderr, flush=True)
     if not args.no_itl:
         out["itl"] = itl_leg(args)
         print("bench: dumbbell rates %.3g /s" % out["itl"]["value"], file=sys.stderr, flush=True)
