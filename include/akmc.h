@@ -137,6 +137,15 @@ int akmc_get_stats(akmc_engine* e, akmc_stats* stats);
 int akmc_sector_rates(akmc_engine* e, int sector, int64_t cap, int64_t* sites, double* rates, double* total,
                       int64_t* n);
 
+/* Sector id (0..7: bit0 x high, bit1 y high, bit2 z high) of each of n real
+ * site indices (x + 2 box[0] (y + box[1] z)) of a box of box[0] x box[1] x
+ * box[2] cells, as the event loops assign sites to sectors: the low half of
+ * an axis of n cells holds floor(n/2) cells.  Host only.  Replaces the
+ * lookup of local_sector_region in the colored domain
+ * (src/algorithms/sl/sublattice.inl:195; split of src/simulation.cpp:91-110,
+ * per-sector bounds of add_ghost, gpu/gpu_simulate.hip:2093-3086). */
+int akmc_site_sectors(const int64_t box[3], const int64_t* sites, int64_t n, int8_t* sectors);
+
 /* Event log: when enabled, akmc_run records every event's (from, to) site
  * indices, for trajectory comparison.  capacity 0 disables it. */
 int akmc_set_event_log(akmc_engine* e, int64_t capacity);

@@ -97,6 +97,7 @@ def lib():
     L.akmc_run.argtypes = [vp, i64, P(Stats)]
     L.akmc_get_stats.argtypes = [vp, P(Stats)]
     L.akmc_sector_rates.argtypes = [vp, i32, i64, vp, vp, P(dbl), P(i64)]
+    L.akmc_site_sectors.argtypes = [P(i64), vp, i64, vp]
     L.akmc_set_event_log.argtypes = [vp, i64]
     L.akmc_get_event_log.argtypes = [vp, vp, vp, i64, P(i64)]
     L.akmc_set_rng_chunk.argtypes = [vp, i64]
@@ -128,6 +129,15 @@ def rng_streams(seed_time_inc, first, n):
     l = np.zeros(n, dtype=np.float64)
     _check(lib().akmc_rng_streams(int(seed_time_inc), int(first), int(n), u.ctypes.data, l.ctypes.data))
     return u, l
+
+
+def site_sectors(box, sites):
+    """Sector id of each real site index, as the event loops assign them."""
+    sites = np.ascontiguousarray(sites, dtype=np.int64)
+    out = np.zeros(sites.size, dtype=np.int8)
+    b = (ctypes.c_int64 * 3)(*[int(v) for v in box])
+    _check(lib().akmc_site_sectors(b, sites.ctypes.data, sites.size, out.ctypes.data))
+    return out
 
 
 def sync(device=0):

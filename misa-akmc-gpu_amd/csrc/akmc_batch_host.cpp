@@ -143,12 +143,7 @@ int setup_rep(HostRep* r, const akmc_params& p, bool fp32) {
   if (!(p.sector_time > 0) || !(p.temperature > 0) || !(p.attempt_freq > 0))
     return fail(AKMC_ERR_INVALID, "sector_time, temperature and attempt_freq must be positive");
   r->p = p;
-  r->d.X = (int32_t)(2 * p.box[0]);
-  r->d.Y = (int32_t)p.box[1];
-  r->d.Z = (int32_t)p.box[2];
-  r->d.hx = (int32_t)(2 * (p.box[0] / 2));
-  r->d.hy = (int32_t)(p.box[1] / 2);
-  r->d.hz = (int32_t)(p.box[2] / 2);
+  r->d = akmc::box_dims(p.box);
   r->nsites = (int64_t)r->d.X * r->d.Y * r->d.Z;
   r->cap = (int32_t)std::max<int64_t>(1, p.va_count);
   r->kT = AKMC_KB * p.temperature;
@@ -208,16 +203,8 @@ akmc::ReplicaState initial_state(const HostRep& r) {
   return st;
 }
 
-uint32_t key_of(const akmc::BoxDims& d, int64_t i) {
-  const uint32_t x = (uint32_t)(i % d.X), y = (uint32_t)((i / d.X) % d.Y), z = (uint32_t)(i / ((int64_t)d.X * d.Y));
-  return x | (y << 11) | (z << 22);
-}
-
-// Sector id of a key: bit0 x high, bit1 y high, bit2 z high.
-int sector_of(const akmc::BoxDims& d, uint32_t k) {
-  return ((int)(k & 0x7FF) >= d.hx ? 1 : 0) | ((int)((k >> 11) & 0x7FF) >= d.hy ? 2 : 0) |
-         ((int)(k >> 22) >= d.hz ? 4 : 0);
-}
+uint32_t key_of(const akmc::BoxDims& d, int64_t i) { return akmc::key_of_index(d, i); }
+int sector_of(const akmc::BoxDims& d, uint32_t k) { return akmc::key_sector(d, k); }
 
 void count_sectors(HostRep* r, const std::vector<uint32_t>& keys) {
   for (int q = 0; q < 8; q++) r->sec_count[q] = 0;
@@ -348,11 +335,7 @@ struct Runner {
     HIP_TRY(hipMalloc(&d_items, sizeof(int32_t) * std::max(1, nslots)));
     HIP_TRY(hipMalloc(&d_queue, 4 * sizeof(uint64_t)));
     HIP_TRY(hipMemset(d_queue, 0, 4 * sizeof(uint64_t)));
-    static akmc::StencilTables tables;
-    static std::once_flag once;
-    std::call_once(once, [] { akmc_host::build_stencil(&tables); });
-    if (akmc_batch_set_stencil(&tables)) return fail(AKMC_ERR_HIP, "stencil upload failed");
-    return AKMC_OK;
+    return akmc_host::upload_stencil(device);
   }
 
   void destroy() {

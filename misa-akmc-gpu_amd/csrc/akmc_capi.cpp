@@ -19,6 +19,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -26,6 +27,7 @@
 #include "../../include/akmc.h"
 #include "akmc_exp.h"
 #include "akmc_exp_table.h"
+#include "akmc_batch.h"
 #include "akmc_host.h"
 #include "akmc_model.h"
 #include "akmc_plain.h"
@@ -91,6 +93,32 @@ void build_stencil(akmc::StencilTables* S) {
       S->reach[par][sites[i][2] + 2][sites[i][1] + 2] |= (uint8_t)(1u << (sites[i][0] + 3));
     }
   }
+}
+
+}  // namespace akmc_host
+
+namespace akmc_host {
+
+const akmc::StencilTables& stencil_tables() {
+  static akmc::StencilTables tables;
+  static std::once_flag once;
+  std::call_once(once, [] { build_stencil(&tables); });
+  return tables;
+}
+
+int upload_stencil(int device) {
+  static std::mutex m;
+  static std::vector<bool> done;
+  const akmc::StencilTables& t = stencil_tables();
+  if (t.n_sites > akmc::STENCIL_MAX) return fail(AKMC_ERR_INVALID, "stencil too large");
+  std::lock_guard<std::mutex> g(m);
+  if (device < 0) return fail(AKMC_ERR_INVALID, "bad device");
+  if ((size_t)device >= done.size()) done.resize((size_t)device + 1, false);
+  if (done[(size_t)device]) return AKMC_OK;
+  if (akmc_plain_set_stencil(&t) != 0 || akmc_batch_set_stencil(&t) != 0)
+    return fail(AKMC_ERR_HIP, "copying stencil tables failed");
+  done[(size_t)device] = true;
+  return AKMC_OK;
 }
 
 }  // namespace akmc_host
@@ -346,12 +374,7 @@ int akmc_create(const akmc_params* p, int device, akmc_engine** out) {
   std::unique_ptr<akmc_engine> e(new akmc_engine());
   e->p = *p;
   e->device = device;
-  e->dims.X = (int32_t)(2 * p->box[0]);
-  e->dims.Y = (int32_t)p->box[1];
-  e->dims.Z = (int32_t)p->box[2];
-  e->dims.hx = (int32_t)(2 * (p->box[0] / 2));
-  e->dims.hy = (int32_t)(p->box[1] / 2);
-  e->dims.hz = (int32_t)(p->box[2] / 2);
+  e->dims = akmc::box_dims(p->box);
   e->nsites = (int64_t)e->dims.X * e->dims.Y * e->dims.Z;
   // SubLattice::startTimeLoop: time_steps = ceil(time_limit / T) (sublattice.inl:26)
   e->time_steps = (int64_t)ceil(p->physics_time / p->sector_time);
@@ -360,11 +383,10 @@ int akmc_create(const akmc_params* p, int device, akmc_engine** out) {
   // seeds: SubLattice seeds time_inc_rng with the (double -> int64) config seed
   // (simulation.h:70-76, sublattice.cpp:17); mt19937::seed takes it mod 2^32.
   e->time_rng.seed((std::mt19937::result_type)(int64_t)(double)p->seed_time_inc);
-  static akmc::StencilTables tables;
-  static bool tables_built = false;
-  if (!tables_built) { build_stencil(&tables); tables_built = true; }
-  if (tables.n_sites > akmc::STENCIL_MAX) return fail(AKMC_ERR_INVALID, "stencil too large");
-  if (akmc_plain_set_stencil(&tables) != 0) return fail(AKMC_ERR_HIP, "copying stencil tables failed");
+  {
+    const int rc = akmc_host::upload_stencil(device);
+    if (rc) return rc;
+  }
   HIP_TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
   HIP_TRY(hipEventCreate(&e->ev0));
   HIP_TRY(hipEventCreate(&e->ev1));
@@ -405,6 +427,21 @@ void akmc_destroy(akmc_engine* e) {
 }
 
 int64_t akmc_num_sites(const akmc_engine* e) { return e ? e->nsites : 0; }
+
+int akmc_site_sectors(const int64_t box[3], const int64_t* sites, int64_t n, int8_t* sectors) {
+  if (!box || (n > 0 && (!sites || !sectors)) || n < 0) return fail(AKMC_ERR_INVALID, "null argument");
+  for (int d = 0; d < 3; d++)
+    if (box[d] < 1) return fail(AKMC_ERR_INVALID, "box edge %lld < 1", (long long)box[d]);
+  if (2 * box[0] > 2048 || box[1] > 2047 || box[2] > 1023)
+    return fail(AKMC_ERR_INVALID, "box exceeds 1024 x 2047 x 1023 cells");
+  const akmc::BoxDims d = akmc::box_dims(box);
+  const int64_t ns = (int64_t)d.X * d.Y * d.Z;
+  for (int64_t i = 0; i < n; i++) {
+    if (sites[i] < 0 || sites[i] >= ns) return fail(AKMC_ERR_INVALID, "site %lld outside the box", (long long)sites[i]);
+    sectors[i] = (int8_t)akmc::key_sector(d, akmc::key_of_index(d, sites[i]));
+  }
+  return AKMC_OK;
+}
 
 int akmc_random_sites(const akmc_params* pp, uint8_t* out, int64_t n) {
   if (!pp || !out) return fail(AKMC_ERR_INVALID, "null argument");

@@ -12,10 +12,12 @@ namespace akmc {
 // Lattice key: x (doubled, 11 bits) | y << 11 (11 bits) | z << 22 (10 bits).
 // Sorting keys sorts sites in the reference's z, y, x sweep order
 // (models/abvi/kmc.cpp:72-75).
-__device__ __forceinline__ uint32_t key_x(uint32_t k) { return k & 0x7FFu; }
-__device__ __forceinline__ uint32_t key_y(uint32_t k) { return (k >> 11) & 0x7FFu; }
-__device__ __forceinline__ uint32_t key_z(uint32_t k) { return k >> 22; }
-__device__ __forceinline__ uint32_t make_key(uint32_t x, uint32_t y, uint32_t z) { return x | (y << 11) | (z << 22); }
+__host__ __device__ __forceinline__ uint32_t key_x(uint32_t k) { return k & 0x7FFu; }
+__host__ __device__ __forceinline__ uint32_t key_y(uint32_t k) { return (k >> 11) & 0x7FFu; }
+__host__ __device__ __forceinline__ uint32_t key_z(uint32_t k) { return k >> 22; }
+__host__ __device__ __forceinline__ uint32_t make_key(uint32_t x, uint32_t y, uint32_t z) {
+  return x | (y << 11) | (z << 22);
+}
 
 struct BoxDims {
   int32_t X;  // 2 * cells in x (doubled)
@@ -23,6 +25,27 @@ struct BoxDims {
   int32_t Z;
   int32_t hx, hy, hz;  // low-half sizes: 2*floor(nx/2), floor(ny/2), floor(nz/2)
 };
+
+// Dimensions of a box of box[0] x box[1] x box[2] cells.  The low half of
+// each axis has floor(n/2) cells: local_split_coord = sector_lattice_size[0]
+// + ghost with sector_lattice_size[0][d] = n/2 (src/simulation.cpp:91-110);
+// the sector bounds that add_ghost spells out per sector id
+// (gpu/gpu_simulate.hip:2093-3086) split every axis there.
+__host__ __device__ inline BoxDims box_dims(const int64_t box[3]) {
+  BoxDims d;
+  d.X = (int32_t)(2 * box[0]);
+  d.Y = (int32_t)box[1];
+  d.Z = (int32_t)box[2];
+  d.hx = (int32_t)(2 * (box[0] / 2));
+  d.hy = (int32_t)(box[1] / 2);
+  d.hz = (int32_t)(box[2] / 2);
+  return d;
+}
+
+// Key of site index i = x + X (y + Y z).
+__host__ __device__ __forceinline__ uint32_t key_of_index(const BoxDims& d, int64_t i) {
+  return make_key((uint32_t)(i % d.X), (uint32_t)((i / d.X) % d.Y), (uint32_t)(i / ((int64_t)d.X * d.Y)));
+}
 
 __device__ __forceinline__ uint64_t key_index(const BoxDims& d, uint32_t k) {
   return (uint64_t)key_x(k) + (uint64_t)d.X * ((uint64_t)key_y(k) + (uint64_t)d.Y * key_z(k));
@@ -35,10 +58,11 @@ __device__ __forceinline__ uint32_t key_add(const BoxDims& d, uint32_t k, int dx
                   (uint32_t)wrapc((int)key_z(k) + dz, d.Z));
 }
 
-// Sector of a site: bit0 x high, bit1 y high, bit2 z high (low half has
-// floor(n/2) cells; reference src/simulation.cpp:91-110 describes the
-// colored-domain sector split).
-__device__ __forceinline__ int key_sector(const BoxDims& d, uint32_t k) {
+// Sector of a site: bit0 x high, bit1 y high, bit2 z high (the low half has
+// floor(n/2) cells; the per-sector ghost bounds of add_ghost,
+// gpu/gpu_simulate.hip:2103-3086, place sector s on the high side of x, y, z
+// exactly where bits 0, 1, 2 of s are set).
+__host__ __device__ __forceinline__ int key_sector(const BoxDims& d, uint32_t k) {
   return ((int)key_x(k) >= d.hx ? 1 : 0) | ((int)key_y(k) >= d.hy ? 2 : 0) | ((int)key_z(k) >= d.hz ? 4 : 0);
 }
 

@@ -17,6 +17,14 @@ extern thread_local std::string g_err;
 // The 55-site rate stencil tables (akmc_model.h).
 void build_stencil(akmc::StencilTables* S);
 
+// The stencil tables, built once per process (thread-safe).
+const akmc::StencilTables& stencil_tables();
+
+// Copies the stencil tables into the __constant__ tables of both event-loop
+// modules on the current device, once per device per process (thread-safe).
+// Kernels of every engine on the device read these shared tables.
+int upload_stencil(int device);
+
 }  // namespace akmc_host
 
 #define HIP_TRY(expr)                                                                                    \

@@ -10,8 +10,11 @@
 // four hash containers (vac/cu/mn/ni), every event re-sweeps the whole sector
 // through those hashes, and the C library's exp/log are used.  Each function
 // cites the reference lines it follows.  Single-rank ghost sites are periodic
-// images of the box (ghost init + per-sector sync of src/pack/*); here that is
-// expressed as an index wrap on the ghost-extended coordinates, see wrap().
+// images of the box (ghost init + per-sector sync of src/pack/*); by default
+// that is expressed as an index wrap on the ghost-extended coordinates
+// (wrapId).  The ghost_images form keeps real ghost copies and the packers'
+// bookkeeping instead (sector_sync); tests/test_ghost_oracle.py shows that the
+// two give the same trajectories.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -111,8 +114,24 @@ class RefCpu {
     return r + g;
   }
   int64_t wrapId(int64_t x, int64_t y, int64_t z) const {
+    if (ghost_images) {
+      // Ghost-image form: ids address the ghost-extended grid directly, as
+      // LatListMeta::getIdBy1nnOffset does (lattice_list_meta.cpp:22-70).
+      if (x < 0 || x >= size_x || y < 0 || y >= size_y || z < 0 || z >= size_z) {
+        fprintf(stderr, "refcpu: stencil leaves the ghost-extended grid\n");
+        abort();
+      }
+      return getId(x, y, z);
+    }
     return getId(wrap1(x, ghost_x, box_x), wrap1(y, ghost_y, box_y), wrap1(z, ghost_z, box_z));
   }
+  // The real site a ghost-extended id is an image of.
+  int64_t imageId(int64_t id) const {
+    int64_t x, y, z;
+    coord(id, &x, &y, &z);
+    return getId(wrap1(x, ghost_x, box_x), wrap1(y, ghost_y, box_y), wrap1(z, ghost_z, box_z));
+  }
+  bool isGhost(int64_t id) const { return imageId(id) != id; }
 
   // PeriodLatticeList::getnnlattice readout order V > Cu > Mn > Ni > Fe
   // (src/lattice/period_lattice_list.cpp:307-324).
@@ -296,6 +315,73 @@ class RefCpu {
     if (ic != cu_hash.end()) { cu_hash.erase(ic); cu_hash.emplace(from); }
     else if (im != mn_hash.end()) { mn_hash.erase(im); mn_hash.emplace(from); }
     else if (in != ni_hash.end()) { ni_hash.erase(in); ni_hash.emplace(from); }
+    if (ghost_images) {
+      // addExchange_ghost / addExchange_surface (kmc.cpp:385-392): a hop into
+      // a ghost site is sent to its owner after the sector; changed real
+      // sites refresh their ghost copies.
+      if (isGhost(to)) exchange_ghost.push_back(to);
+      else changed.insert(to);
+      changed.insert(from);
+    }
+  }
+
+  // ---- Ghost-image form of one rank (src/pack/*).  Ghost sites hold copies
+  // of their periodic images; the copies a hop changed are pushed to the
+  // owner after the sector, and every ghost copy of a changed real site is
+  // refreshed before the next sector.
+
+  // Sets the flags of `id` to `type` the way the packers' receive handlers do
+  // (ghost_init_packer.cpp:281-357, sim_sync_packer.cpp:169-208,
+  // ghost_sync_packer.cpp:285-318): erase the first of V, Cu, Mn, Ni that is
+  // present, then insert `type` (nothing for Fe).
+  void receive_type(int64_t id, int type) {
+    auto iv = vac_hash.find(id);
+    if (iv != vac_hash.end()) vac_hash.erase(iv);
+    else if (cu_hash.erase(id)) {}
+    else if (mn_hash.erase(id)) {}
+    else ni_hash.erase(id);
+    if (type == V) vac_hash.emplace(id, VacancyHash{});
+    else if (auto* h = species_hash(type)) h->emplace(id);
+  }
+  // Every ghost copy of real site `id` receives its readout.
+  void refresh_copies(int64_t id) {
+    int64_t x, y, z;
+    coord(id, &x, &y, &z);
+    const int t = readout(id);
+    const int64_t c[3] = {x, y, z}, n[3] = {box_x, box_y, box_z}, s[3] = {size_x, size_y, size_z};
+    int64_t opts[3][3];
+    int cnt[3];
+    for (int d = 0; d < 3; d++) {
+      cnt[d] = 0;
+      for (int k = -1; k <= 1; k++) {
+        const int64_t v = c[d] + k * n[d];
+        if (v >= 0 && v < s[d]) opts[d][cnt[d]++] = v;
+      }
+    }
+    for (int i = 0; i < cnt[0]; i++)
+      for (int j = 0; j < cnt[1]; j++)
+        for (int k = 0; k < cnt[2]; k++) {
+          const int64_t g = getId(opts[0][i], opts[1][j], opts[2][k]);
+          if (g != id) receive_type(g, t);
+        }
+  }
+  // GhostInitPacker (ghost_init_packer.cpp:107-179, 266-403): every ghost
+  // site receives the readout of its image.
+  void ghost_init() {
+    for (int64_t id = 0; id < size_x * size_y * size_z; id++)
+      if (!isGhost(id)) refresh_copies(id);
+  }
+  // syncSimRegions then syncNextSectorGhostRegions (sublattice.inl:161-163).
+  void sector_sync() {
+    if (!ghost_sync) return;
+    for (int64_t g : exchange_ghost) {  // SimSyncPacker::onReceive2: the ghost's type is V
+      const int64_t img = imageId(g);
+      receive_type(img, readout(g));
+      changed.insert(img);
+    }
+    exchange_ghost.clear();
+    for (int64_t id : changed) refresh_copies(id);
+    changed.clear();
   }
 
   // LatticeTypes::randomAtomsType (src/lattice/lattice_types.cpp:9-19).
@@ -323,7 +409,7 @@ class RefCpu {
           const int t = randomAtomsType(hit);
           const bool real = x >= ghost_x && x < ghost_x + box_x && y >= ghost_y && y < ghost_y + box_y &&
                             z >= ghost_z && z < ghost_z + box_z;
-          if (!real) continue;
+          if (!real && !ghost_images) continue;  // wrap form: ghost draws are overwritten by images
           const int64_t id = getId(x, y, z);
           if (t == Cu) cu_hash.emplace(id);
           else if (t == Ni) ni_hash.emplace(id);
@@ -346,6 +432,7 @@ class RefCpu {
         va_i++;
       }
     }
+    if (ghost_images) ghost_init();
   }
 
   // SubLattice::startTimeLoop (src/algorithms/sl/sublattice.inl:25-190),
@@ -385,6 +472,7 @@ class RefCpu {
         }
         evolution[s] += sector_time;
         in_sector = false;
+        if (ghost_images) sector_sync();
       }
       sect = 0;
     }
@@ -408,6 +496,11 @@ class RefCpu {
   int64_t events = 0;
   std::vector<int64_t>* log_from = nullptr;
   std::vector<int64_t>* log_to = nullptr;
+  // Ghost-image form (set before createRandom): see sector_sync().
+  bool ghost_images = false;
+  bool ghost_sync = true;                // false only in a test that plants a missing sync
+  std::vector<int64_t> exchange_ghost;   // ghost sites hopped into during this sector
+  std::unordered_set<int64_t> changed;   // real sites changed since their copies were refreshed
 };
 
 }  // namespace refcpu
@@ -463,7 +556,7 @@ static int64_t real_to_local(refcpu::RefCpu* r, int64_t i) {
 }
 static int64_t local_to_real(refcpu::RefCpu* r, int64_t id) {
   int64_t x, y, z;
-  r->coord(id, &x, &y, &z);
+  r->coord(r->imageId(id), &x, &y, &z);  // a ghost id names its image
   return (x - r->ghost_x) + r->box_x * ((y - r->ghost_y) + r->box_y * (z - r->ghost_z));
 }
 
@@ -471,11 +564,42 @@ void ref_get_sites(void* h, uint8_t* out) {
   auto* r = (refcpu::RefCpu*)h;
   const int64_t n = ref_n_sites(h);
   memset(out, 0, (size_t)n);
-  for (auto& kv : r->vac_hash) out[local_to_real(r, kv.first)] |= 1;
-  for (auto id : r->cu_hash) out[local_to_real(r, id)] |= 2;
-  for (auto id : r->mn_hash) out[local_to_real(r, id)] |= 4;
-  for (auto id : r->ni_hash) out[local_to_real(r, id)] |= 8;
+  for (auto& kv : r->vac_hash) if (!r->isGhost(kv.first)) out[local_to_real(r, kv.first)] |= 1;
+  for (auto id : r->cu_hash) if (!r->isGhost(id)) out[local_to_real(r, id)] |= 2;
+  for (auto id : r->mn_hash) if (!r->isGhost(id)) out[local_to_real(r, id)] |= 4;
+  for (auto id : r->ni_hash) if (!r->isGhost(id)) out[local_to_real(r, id)] |= 8;
+  // Ghost-image form stopped inside a sector: the owner's sites as the
+  // sector-end push will leave them (erase the first flag present, set V).
+  for (int64_t g : r->exchange_ghost) {
+    uint8_t& f = out[local_to_real(r, g)];
+    const uint8_t first = (f & 1) ? 1 : (f & 2) ? 2 : (f & 4) ? 4 : (f & 8) ? 8 : 0;
+    f = (uint8_t)((f & ~first) | 1);
+  }
 }
+
+// Ghost-image form on or off (before ref_create_random / ref_set_sites).
+void ref_set_ghost_images(void* h, int on) { ((refcpu::RefCpu*)h)->ghost_images = on != 0; }
+// Sector-end push and refresh of the ghost form (on by default; a test
+// switches it off to show that the comparison detects a missing sync).
+void ref_set_ghost_sync(void* h, int on) { ((refcpu::RefCpu*)h)->ghost_sync = on != 0; }
+
+// Flags of every ghost-extended site, x + size_x*(y + size_y*z) (size_x =
+// 2(N + 2g)): real sites and, in the ghost-image form, the ghost copies.
+int64_t ref_get_ext_sites(void* h, uint8_t* out, int64_t cap) {
+  auto* r = (refcpu::RefCpu*)h;
+  const int64_t n = r->size_x * r->size_y * r->size_z;
+  if (cap < n) return -n;
+  memset(out, 0, (size_t)n);
+  for (auto& kv : r->vac_hash) out[kv.first] |= 1;
+  for (auto id : r->cu_hash) out[id] |= 2;
+  for (auto id : r->mn_hash) out[id] |= 4;
+  for (auto id : r->ni_hash) out[id] |= 8;
+  return n;
+}
+
+// local_sector_region[s] of the oracle (ghost-extended cell coordinates, x
+// not doubled).
+void ref_region(void* h, int s, int64_t lo[3], int64_t hi[3]) { ((refcpu::RefCpu*)h)->region(s, lo, hi); }
 
 void ref_set_sites(void* h, const uint8_t* in) {
   auto* r = (refcpu::RefCpu*)h;
@@ -489,6 +613,7 @@ void ref_set_sites(void* h, const uint8_t* in) {
     if (in[i] & 4) r->mn_hash.emplace(id);
     if (in[i] & 8) r->ni_hash.emplace(id);
   }
+  if (r->ghost_images) r->ghost_init();
 }
 
 // Runs up to max_events; logs (from, to) real-site indices when log arrays

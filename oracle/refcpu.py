@@ -81,6 +81,14 @@ def lib():
         L.ref_exp_array.restype = None
         L.ref_streams.argtypes = [ctypes.c_int64, ctypes.c_int64, vp, vp]
         L.ref_streams.restype = None
+        L.ref_set_ghost_images.argtypes = [vp, ctypes.c_int]
+        L.ref_set_ghost_images.restype = None
+        L.ref_set_ghost_sync.argtypes = [vp, ctypes.c_int]
+        L.ref_set_ghost_sync.restype = None
+        L.ref_get_ext_sites.argtypes = [vp, vp, ctypes.c_int64]
+        L.ref_get_ext_sites.restype = ctypes.c_int64
+        L.ref_region.argtypes = [vp, ctypes.c_int, vp, vp]
+        L.ref_region.restype = None
         _lib = L
     return _lib
 
@@ -111,13 +119,20 @@ def make_params(cfg):
 
 
 class RefCpu:
-    """The reference CPU build's event loop (hash containers, full sweeps)."""
+    """The reference CPU build's event loop (hash containers, full sweeps).
 
-    def __init__(self, cfg, create=True):
+    ghost_images=True runs the single-rank ghost-image form instead of the
+    index wrap: ghost sites hold copies of their periodic images, hops into
+    them reach the owner after the sector and changed sites refresh their
+    copies (src/pack/*, sublattice.inl:161-163)."""
+
+    def __init__(self, cfg, create=True, ghost_images=False):
         self.L = lib()
         self._p = make_params(cfg)
         self.h = self.L.ref_create(ctypes.byref(self._p))
         self.n_sites = self.L.ref_n_sites(self.h)
+        if ghost_images:
+            self.L.ref_set_ghost_images(self.h, 1)
         if create:
             self.L.ref_create_random(self.h)
 
@@ -130,6 +145,22 @@ class RefCpu:
         out = np.zeros(self.n_sites, dtype=np.uint8)
         self.L.ref_get_sites(self.h, out.ctypes.data)
         return out
+
+    def ext_sites(self):
+        """Flags of the ghost-extended grid, shape (size_z, size_y, size_x)."""
+        g = self._p.ghost
+        shape = (self._p.box[2] + 2 * g, self._p.box[1] + 2 * g, 2 * (self._p.box[0] + 2 * g))
+        out = np.zeros(int(np.prod(shape)), dtype=np.uint8)
+        n = self.L.ref_get_ext_sites(self.h, out.ctypes.data, out.size)
+        assert n == out.size
+        return out.reshape(shape)
+
+    def region(self, s):
+        """(lo, hi) cell bounds of sector s on the ghost-extended grid (x not doubled)."""
+        lo = np.zeros(3, dtype=np.int64)
+        hi = np.zeros(3, dtype=np.int64)
+        self.L.ref_region(self.h, int(s), lo.ctypes.data, hi.ctypes.data)
+        return lo, hi
 
     def set_sites(self, sites):
         sites = np.ascontiguousarray(sites, dtype=np.uint8)
