@@ -687,9 +687,25 @@ int akmc_checkpoint_load(const char* path, int device, akmc_engine** out) {
   bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, kCkptMagic, 8) == 0 && fread(sz, sizeof(sz), 1, f) == 1 &&
             sz[0] == sizeof(akmc_params) && sz[1] == sizeof(akmc::PlainState) && fread(&p, sizeof(p), 1, f) == 1 &&
             fread(&st, sizeof(st), 1, f) == 1 && fread(&n, sizeof(n), 1, f) == 1 && n > 0;
+  // The site count must be the one the stored box implies, and the file must
+  // hold exactly that many bytes after the header, before anything is
+  // allocated from it.
+  if (ok) {
+    for (int d = 0; d < 3; d++) ok = ok && p.box[d] >= 1 && p.box[d] <= 2048;
+    ok = ok && n == 2 * p.box[0] * p.box[1] * p.box[2];
+  }
+  if (ok) {
+    const long here = ftell(f);
+    ok = here >= 0 && fseek(f, 0, SEEK_END) == 0 && ftell(f) - here == (long)n && fseek(f, here, SEEK_SET) == 0;
+  }
   std::vector<uint8_t> sites;
   if (ok) {
-    sites.resize((size_t)n);
+    try {
+      sites.resize((size_t)n);
+    } catch (const std::exception&) {
+      fclose(f);
+      return fail(AKMC_ERR_INVALID, "%s: cannot hold %lld sites", path, (long long)n);
+    }
     ok = fread(sites.data(), 1, sites.size(), f) == sites.size();
   }
   fclose(f);

@@ -166,12 +166,20 @@ int akmc_read_dump(const char* path, int64_t box[3], int64_t* ghost, uint8_t* si
     ls >> xf >> yf >> zf;
     const int fl = flags_of(t);
     if (fl < 0) return fail(AKMC_ERR_INVALID, "lattice type \"%s\" is not supported (vacancy and atom types only)", t.c_str());
+    const double lim = 1e9;  // far beyond any box the engines take; keeps the integer conversions defined
+    if (!(std::fabs(xf) < lim && std::fabs(yf) < lim && std::fabs(zf) < lim))
+      return fail(AKMC_ERR_INVALID, "coordinates out of range in line \"%s\"", line.c_str());
     const P p{(int64_t)(2 * xf), (int64_t)yf, (int64_t)zf, fl};
     xmin = std::min(xmin, p.x); ymin = std::min(ymin, p.y); zmin = std::min(zmin, p.z);
     xmax = std::max(xmax, p.x); ymax = std::max(ymax, p.y); zmax = std::max(zmax, p.z);
     ps.push_back(p);
   }
   if (ps.size() != count || ps.empty()) return fail(AKMC_ERR_INVALID, "bad lattice count");
+  // Whole unit cells: doubled x must start at a corner site (even) and span
+  // corner/body pairs (an even count), as LatticeDump writes it.
+  if ((xmin & 1) != 0 || ((xmax - xmin + 1) & 1) != 0)
+    return fail(AKMC_ERR_INVALID, "doubled x runs from %lld to %lld: not whole unit cells", (long long)xmin,
+                (long long)xmax);
   box[0] = (xmax - xmin + 1) / 2;
   box[1] = ymax - ymin + 1;
   box[2] = zmax - zmin + 1;
@@ -184,6 +192,8 @@ int akmc_read_dump(const char* path, int64_t box[3], int64_t* ghost, uint8_t* si
   std::vector<uint8_t> seen(n, 0);
   for (const P& p : ps) {
     const int64_t i = (p.x - xmin) + X * ((p.y - ymin) + Y * (p.z - zmin));
+    if (i < 0 || i >= n) return fail(AKMC_ERR_INVALID, "site (%lld, %lld, %lld) lies outside the box", (long long)p.x,
+                                     (long long)p.y, (long long)p.z);
     if (seen[i]) return fail(AKMC_ERR_INVALID, "site (%lld, %lld, %lld) appears twice", (long long)p.x,
                              (long long)p.y, (long long)p.z);
     seen[i] = 1;

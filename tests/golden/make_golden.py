@@ -5,6 +5,7 @@ sector clocks and the partial sector time.  Run from the repository root:
     python3 tests/golden/make_golden.py        # traj_*.npz
     python3 tests/golden/make_golden.py cfg0   # hashes_cfg0_40cube.npz (1e5 events, a few minutes)
     python3 tests/golden/make_golden.py itl    # itl_dense_6x4x9.npz (dumbbell rates)
+    python3 tests/golden/make_golden.py cfg1   # hashes_cfg1_256cube.npz (1e5 events of 256^3, ~1 h)
 """
 import hashlib
 import json
@@ -49,6 +50,35 @@ def cfg0_hashes():
     print("cfg0 hashes", st["events"], "events")
 
 
+# Config 1 of the benchmark suite (256^3 cells, 6,711 vacancies, T = 573 K,
+# sector threshold 1e-6 s): the first 1e5 events.  Every 1000 events the
+# SHA-256 of that block's (from, to) site indices (int64 pairs) is stored,
+# every 10,000 events the SHA-256 of the whole site array, and the sector
+# clocks at the end.  About an hour of oracle time (~30 events/s).
+CFG1 = make_cfg(n=256, va=6711, T=573.0, sector_T=1e-6, physics=1.0)
+CFG1_EVENTS, CFG1_EVERY, CFG1_SITES_EVERY = 100000, 1000, 10000
+
+
+def cfg1_hashes():
+    r = refcpu.RefCpu(CFG1)
+    log_digests, site_digests = [], []
+    out = os.path.join(HERE, "hashes_cfg1_256cube.npz")
+    for k in range(CFG1_EVENTS // CFG1_EVERY):
+        rc, lf, lt = r.run(CFG1_EVERY, log=True)
+        assert rc == 1 and r.state()["events"] == (k + 1) * CFG1_EVERY, rc
+        pairs = np.stack([lf, lt], axis=1).astype(np.int64)
+        log_digests.append(np.frombuffer(hashlib.sha256(pairs.tobytes()).digest(), dtype=np.uint8))
+        if (k + 1) * CFG1_EVERY % CFG1_SITES_EVERY == 0:
+            site_digests.append(np.frombuffer(hashlib.sha256(r.sites().tobytes()).digest(), dtype=np.uint8))
+            st = r.state()
+            np.savez_compressed(out, cfg=json.dumps(CFG1), events=np.int64(st["events"]),
+                                every=np.int64(CFG1_EVERY), sites_every=np.int64(CFG1_SITES_EVERY),
+                                log_sha256=np.stack(log_digests), sites_sha256=np.stack(site_digests),
+                                evolution=st["evolution"], sector_time=np.float64(st["sector_time"]),
+                                step=np.int64(st["step"]), sect=np.int64(st["sect"]))
+            print("cfg1", st["events"], "events", flush=True)
+
+
 def itl_rates():
     """Dumbbell rates of a dense typed lattice (6 x 4 x 9 cells, 10 % dumbbell
     sites of every kind and orientation, Cu/Ni/Mn atoms and vacancies) from
@@ -76,6 +106,9 @@ def itl_rates():
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "cfg0":
         cfg0_hashes()
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "cfg1":
+        cfg1_hashes()
         return
     if len(sys.argv) > 1 and sys.argv[1] == "itl":
         itl_rates()

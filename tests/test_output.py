@@ -117,3 +117,37 @@ def test_checkpoint_load_rejects_other_files(tmp_path):
         akmc.Engine.load(p)
     with pytest.raises(akmc.AkmcError):
         akmc.Engine.load(tmp_path / "missing.ckpt")
+
+
+def test_read_dump_rejects_partial_unit_cells(tmp_path):
+    """Doubled x must start at a corner site and span whole cells: a dump
+    whose x span is odd, or that starts at a body site, is refused (it would
+    otherwise map a site past the end of the box)."""
+    p = tmp_path / "odd.out"
+    xs = [0, 0.5, 1, 1.5, 2, 2.5, 3, 4]  # doubled x 0..6 and 8: span 9
+    p.write_text("8\n\n" + "".join("Fe\t%g\t%g\t%g\n" % (x, x % 1, x % 1) for x in xs))
+    with pytest.raises(akmc.AkmcError):
+        akmc.read_dump(p)
+    q = tmp_path / "body_first.out"
+    q.write_text("2\n\nFe\t0.5\t0.5\t0.5\nFe\t1\t0\t0\n")  # doubled x 1..2
+    with pytest.raises(akmc.AkmcError):
+        akmc.read_dump(q)
+
+
+def test_checkpoint_load_checks_the_site_count(tmp_path):
+    """A checkpoint header with a valid magic and sizes but a site count the
+    box does not imply, or a short file, is an error status, not an
+    allocation failure through the C boundary."""
+    import ctypes
+    prm = akmc.params_from_config(dict(box=(8, 8, 8), alloy=[("Fe", 1)], va_count=1,
+                                       seeds=dict(create_types=1, create_vacancy=2, time_inc=3),
+                                       temperature=573.0, attempt_freq=6e12, sector_T=3e-7, physics_time=1.0))
+    state_size = 136  # sizeof(akmc::PlainState), akmc_plain.h: 12 x 8-byte fields, 4 x int32, 3 x int64
+    hdr = b"AKMCCKP1" + np.array([ctypes.sizeof(akmc.Params), state_size], dtype=np.uint32).tobytes()
+    body = bytes(prm) + bytes(state_size)
+    for n, tail in [(1 << 50, b""), (2 * 8 * 8 * 8, bytes(100)), (2 * 8 * 8 * 8 + 1, bytes(1025))]:
+        p = tmp_path / ("c%d.ckpt" % n)
+        p.write_bytes(hdr + body + np.int64(n).tobytes() + tail)
+        with pytest.raises(akmc.AkmcError) as ei:
+            akmc.Engine.load(p)
+        assert "checkpoint" in str(ei.value)

@@ -196,3 +196,28 @@ def test_a_wrong_split_fails_the_table():
         assert ok == (shift == 0 and (not swap or bits[0] == bits[1])), (s, shift, swap)
         good += ok
     assert good == 8 + 4
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("box", [(8, 8, 8), (17, 13, 11), (9, 10, 16)])
+def test_device_sectors_match_add_ghost(box):
+    """The kernels' own sector lists (the parity hooks of both event loops)
+    hold exactly the vacancies inside the table's region of each sector."""
+    X, Y, Z = 2 * box[0], box[1], box[2]
+    cfg = make_cfg(box=box, va=X * Y * Z // 20, seeds=dict(create_types=5, create_vacancy=6, time_inc=7))
+    eng = akmc.Engine(cfg, device=0)
+    sites = eng.download()
+    vac = np.nonzero(sites & 1)[0]
+    x, y, z = vac % X, (vac // X) % Y, vac // (X * Y)
+    bat = akmc.Batch([cfg], device=0, precision=akmc.AKMC_PREC_FP64, init=akmc.AKMC_INIT_REFERENCE)
+    assert np.array_equal(bat.download(0), sites)
+    for s in range(8):
+        lo, hi = _table_region(box, s)
+        inside = (x >= 2 * lo[0]) & (x < 2 * hi[0]) & (y >= lo[1]) & (y < hi[1]) & (z >= lo[2]) & (z < hi[2])
+        want = np.sort(vac[inside])
+        got, _, _ = eng.sector_rates(s, cap=1 << 14)
+        assert np.array_equal(np.sort(got), want), (box, s)
+        got_b, _, _ = bat.sector_rates(0, s, cap=1 << 14)
+        assert np.array_equal(np.sort(got_b), want), (box, s)
+    bat.close()
+    eng.close()
