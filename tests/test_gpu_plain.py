@@ -388,18 +388,25 @@ def test_engines_in_host_threads_are_independent():
     say("test_engines_in_host_threads_are_independent")
     cfgs = [make_cfg(n=16, va=10 + k, T=560.0 + 10 * k, seeds=dict(create_types=40 + k, create_vacancy=50 + k,
                                                                      time_inc=60 + k)) for k in range(6)]
+    # Two crowded lattices (~2,000 vacancies per sector) whose sector state
+    # lives in device memory: their engines allocate it during the run, while
+    # the other threads' kernels are in flight.
+    cfgs += [make_cfg(n=32, va=16000, T=600.0 + 10 * k, seeds=dict(create_types=70 + k, create_vacancy=80 + k,
+                                                                     time_inc=90 + k)) for k in range(2)]
+    events = [3000] * 6 + [300] * 2
 
-    def run(c):
+    def run(arg):
+        c, n = arg
         e = akmc.Engine(c)
-        st = e.run(3000)
+        st = e.run(n)
         s = e.download()
         e.close()
         return st["events"], s
 
-    with ThreadPoolExecutor(6) as ex:
-        par = list(ex.map(run, cfgs))
-    for c, (n, s) in zip(cfgs, par):
+    with ThreadPoolExecutor(len(cfgs)) as ex:
+        par = list(ex.map(run, zip(cfgs, events)))
+    for c, want, (n, s) in zip(cfgs, events, par):
         ref = refcpu.RefCpu(c)
-        ref.run(3000)
-        assert n == ref.state()["events"] == 3000
+        ref.run(want)
+        assert n == ref.state()["events"] == want
         assert np.array_equal(s, ref.sites())
