@@ -20,7 +20,9 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <fstream>
 #include <random>
+#include <sstream>
 #include <string>
 #include <unordered_map>
 #include <unordered_set>
@@ -595,6 +597,58 @@ int64_t ref_get_ext_sites(void* h, uint8_t* out, int64_t cap) {
   for (auto id : r->mn_hash) out[id] |= 4;
   for (auto id : r->ni_hash) out[id] |= 8;
   return n;
+}
+
+// Saves / restores the whole oracle state (sites with every tag, both random
+// engines and the time-stream distribution, the sector-loop position and
+// clocks), so that a long fixture run can resume.  Wrap form only.
+void ref_set_sites(void* h, const uint8_t* in);
+
+int ref_save(void* h, const char* path) {
+  auto* r = (refcpu::RefCpu*)h;
+  if (r->ghost_images) return -1;
+  std::ofstream f(path);
+  if (!f) return -1;
+  f.precision(17);
+  f << r->sel_rng << "\n" << r->time_inc_rng << "\n" << r->time_inc_dis << "\n";
+  f << r->step << " " << r->sect << " " << (r->in_sector ? 1 : 0) << " " << r->events << "\n";
+  f << std::hexfloat << r->thr << " " << r->sector_time;
+  for (int i = 0; i < 8; i++) f << " " << r->evolution[i];
+  f << std::defaultfloat << "\n";
+  const int64_t n = ref_n_sites(h);
+  std::vector<uint8_t> sites((size_t)n);
+  ref_get_sites(h, sites.data());
+  for (int64_t i = 0; i < n; i++)
+    if (sites[(size_t)i]) f << i << " " << (int)sites[(size_t)i] << "\n";
+  f << -1 << "\n";
+  return f.good() ? 0 : -1;
+}
+
+int ref_load(void* h, const char* path) {
+  auto* r = (refcpu::RefCpu*)h;
+  std::ifstream f(path);
+  if (!f) return -1;
+  int ins = 0;
+  f >> r->sel_rng >> r->time_inc_rng >> r->time_inc_dis;
+  f >> r->step >> r->sect >> ins >> r->events;
+  std::string t;
+  f >> t; r->thr = std::strtod(t.c_str(), nullptr);
+  f >> t; r->sector_time = std::strtod(t.c_str(), nullptr);
+  for (int i = 0; i < 8; i++) { f >> t; r->evolution[i] = std::strtod(t.c_str(), nullptr); }
+  r->in_sector = ins != 0;
+  const int64_t n = ref_n_sites(h);
+  std::vector<uint8_t> sites((size_t)n, 0);
+  for (;;) {
+    int64_t i = -1;
+    int v = 0;
+    if (!(f >> i) || i < 0) break;
+    f >> v;
+    if (i >= n) return -1;
+    sites[(size_t)i] = (uint8_t)v;
+  }
+  if (!f.good() && !f.eof()) return -1;
+  ref_set_sites(h, sites.data());
+  return 0;
 }
 
 // local_sector_region[s] of the oracle (ghost-extended cell coordinates, x

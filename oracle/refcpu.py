@@ -89,6 +89,10 @@ def lib():
         L.ref_get_ext_sites.restype = ctypes.c_int64
         L.ref_region.argtypes = [vp, ctypes.c_int, vp, vp]
         L.ref_region.restype = None
+        L.ref_save.argtypes = [vp, ctypes.c_char_p]
+        L.ref_save.restype = ctypes.c_int
+        L.ref_load.argtypes = [vp, ctypes.c_char_p]
+        L.ref_load.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -154,6 +158,13 @@ class RefCpu:
         n = self.L.ref_get_ext_sites(self.h, out.ctypes.data, out.size)
         assert n == out.size
         return out.reshape(shape)
+
+    def save(self, path):
+        """Writes the whole state (text), for resuming a long run."""
+        assert self.L.ref_save(self.h, str(path).encode()) == 0
+
+    def load(self, path):
+        assert self.L.ref_load(self.h, str(path).encode()) == 0
 
     def region(self, s):
         """(lo, hi) cell bounds of sector s on the ghost-extended grid (x not doubled)."""

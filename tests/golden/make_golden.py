@@ -5,7 +5,7 @@ sector clocks and the partial sector time.  Run from the repository root:
     python3 tests/golden/make_golden.py        # traj_*.npz
     python3 tests/golden/make_golden.py cfg0   # hashes_cfg0_40cube.npz (1e5 events, a few minutes)
     python3 tests/golden/make_golden.py itl    # itl_dense_6x4x9.npz (dumbbell rates)
-    python3 tests/golden/make_golden.py cfg1   # hashes_cfg1_256cube.npz (1e5 events of 256^3, ~1 h)
+    python3 tests/golden/make_golden.py cfg1 [resume_dir]   # hashes_cfg1_256cube.npz (1e5 events of 256^3, ~1 h)
 """
 import hashlib
 import json
@@ -59,11 +59,25 @@ CFG1 = make_cfg(n=256, va=6711, T=573.0, sector_T=1e-6, physics=1.0)
 CFG1_EVENTS, CFG1_EVERY, CFG1_SITES_EVERY = 100000, 1000, 10000
 
 
-def cfg1_hashes():
+def cfg1_hashes(resume_dir=None):
+    """resume_dir (optional): where the oracle state and the digests so far
+    are kept after every 10,000 events, so that an interrupted run resumes."""
     r = refcpu.RefCpu(CFG1)
     log_digests, site_digests = [], []
     out = os.path.join(HERE, "hashes_cfg1_256cube.npz")
-    for k in range(CFG1_EVENTS // CFG1_EVERY):
+    k0 = 0
+    if resume_dir:
+        os.makedirs(resume_dir, exist_ok=True)
+        st_path, dg_path = os.path.join(resume_dir, "state.txt"), os.path.join(resume_dir, "digests.npz")
+        if os.path.exists(dg_path) and os.path.exists(st_path):
+            z = np.load(dg_path, allow_pickle=False)
+            log_digests = list(z["log_sha256"])
+            site_digests = list(z["sites_sha256"])
+            r.load(st_path)
+            k0 = len(log_digests)
+            assert r.state()["events"] == k0 * CFG1_EVERY
+            print("cfg1 resumes at", k0 * CFG1_EVERY, "events", flush=True)
+    for k in range(k0, CFG1_EVENTS // CFG1_EVERY):
         rc, lf, lt = r.run(CFG1_EVERY, log=True)
         assert rc == 1 and r.state()["events"] == (k + 1) * CFG1_EVERY, rc
         pairs = np.stack([lf, lt], axis=1).astype(np.int64)
@@ -77,6 +91,11 @@ def cfg1_hashes():
                                 evolution=st["evolution"], sector_time=np.float64(st["sector_time"]),
                                 step=np.int64(st["step"]), sect=np.int64(st["sect"]))
             print("cfg1", st["events"], "events", flush=True)
+            if resume_dir:
+                r.save(st_path + ".tmp")
+                np.savez(dg_path + ".tmp.npz", log_sha256=np.stack(log_digests), sites_sha256=np.stack(site_digests))
+                os.replace(st_path + ".tmp", st_path)
+                os.replace(dg_path + ".tmp.npz", dg_path)
 
 
 def itl_rates():
@@ -108,7 +127,7 @@ def main():
         cfg0_hashes()
         return
     if len(sys.argv) > 1 and sys.argv[1] == "cfg1":
-        cfg1_hashes()
+        cfg1_hashes(sys.argv[2] if len(sys.argv) > 2 else None)
         return
     if len(sys.argv) > 1 and sys.argv[1] == "itl":
         itl_rates()
