@@ -211,6 +211,25 @@ int akmc_itl_rates(const int64_t box[3], const uint16_t* types, int64_t n_itl, c
                    const int8_t* orient, double attempt_freq, double temperature, int device, uint8_t* avail,
                    double* rates, double* kernel_seconds);
 
+/* akmc_itl_rates for callers that cannot allocate device memory (code
+ * between pre-allocated buffers in a serving loop, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Same inputs, outputs and
+ * results as akmc_itl_rates, but d_types, d_sites, d_orient, d_avail and
+ * d_rates are device pointers on `device`, the work runs on `stream` (a
+ * hipStream_t; NULL is the null stream) and the call waits for it.  The
+ * working arrays live in a static device array of the library, sized for at
+ * most AKMC_ITL_NOALLOC_MAX dumbbells (11.5 MiB per device, held until the
+ * process ends); more dumbbells return AKMC_ERR_CAPACITY.  Inputs are checked
+ * on the device; an invalid input returns AKMC_ERR_INVALID and writes no
+ * output.  The call allocates nothing.  Every call on a device shares that
+ * device's array and leaves its contents behind: calls that overlap on one
+ * device (from several threads or streams) race and must be serialised by
+ * the caller. */
+#define AKMC_ITL_NOALLOC_MAX 16384
+int akmc_itl_rates_noalloc(const int64_t box[3], const uint16_t* d_types, int64_t n_itl, const int64_t* d_sites,
+                           const int8_t* d_orient, double attempt_freq, double temperature, int device, void* stream,
+                           uint8_t* d_avail, double* d_rates);
+
 /* Waits for all work on a device (benchmark bracketing). */
 int akmc_sync(int device);
 

@@ -115,6 +115,7 @@ def lib():
     L.akmc_exp_host.argtypes = [vp, vp, i64]
     L.akmc_exp_device.argtypes = [i32, vp, vp, i64]
     L.akmc_itl_rates.argtypes = [vp, vp, i64, vp, vp, dbl, dbl, i32, vp, vp, P(dbl)]
+    L.akmc_itl_rates_noalloc.argtypes = [vp, vp, i64, vp, vp, dbl, dbl, i32, vp, vp, vp]
     L.akmc_pipe_sites.argtypes = [P(Params), ctypes.c_char_p, vp, i64, P(i64)]
     L.akmc_pipe_path_from_yaml.argtypes = [ctypes.c_char_p, ctypes.c_char_p, i64]
     L.akmc_destroy.argtypes = [vp]
@@ -715,3 +716,17 @@ def itl_rates(box, types, sites, orient, attempt_freq, temperature, device=0, ti
     if timing is not None:
         timing["kernel_seconds"] = ks.value
     return av, r
+
+
+ITL_NOALLOC_MAX = 16384
+
+
+def itl_rates_noalloc(box, d_types, n_itl, d_sites, d_orient, attempt_freq, temperature, d_avail, d_rates,
+                      device=0, stream=0):
+    """akmc_itl_rates_noalloc on caller-owned device buffers, given as device
+    addresses (ints); `stream` is a hipStream_t address (0: null stream)."""
+    b = np.ascontiguousarray(box, dtype=np.int64)
+    _check(lib().akmc_itl_rates_noalloc(b.ctypes.data, ctypes.c_void_p(d_types), int(n_itl), ctypes.c_void_p(d_sites),
+                                        ctypes.c_void_p(d_orient), float(attempt_freq), float(temperature),
+                                        int(device), ctypes.c_void_p(stream), ctypes.c_void_p(d_avail),
+                                        ctypes.c_void_p(d_rates)))

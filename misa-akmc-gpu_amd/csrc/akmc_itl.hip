@@ -31,6 +31,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <vector>
 
 #include "../../include/akmc.h"
 #include "akmc_exp.h"
@@ -64,6 +66,14 @@ __device__ __forceinline__ int species(int t) {  // single type -> bond-table in
 struct Dims {
   int32_t X, Y, Z;
 };
+
+// LatticeTypes values the model takes (lattice_types.h:20-36): V or a single
+// atom, or a dumbbell XY = X << 8 | Y of two atoms with X <= Y.
+__host__ __device__ inline bool valid_single(int t) { return t == 0 || t == 1 || t == 2 || t == 4 || t == 8; }
+__host__ __device__ inline bool valid_dumbbell(int t) {
+  const int hi = t >> 8, lo = t & 0xFF;
+  return hi != 0 && valid_single(hi) && valid_single(lo) && hi <= lo && (t >> 16) == 0;
+}
 
 __device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : v >= n ? v - n : v; }
 __device__ __forceinline__ int64_t site_of(const Dims& d, int x, int y, int z) {
@@ -198,6 +208,7 @@ struct Args {
   int32_t* nbr_cnt;
   uint8_t* avail;    // out
   double* rates;     // out, 8 per dumbbell
+  const int32_t* status;  // input check of the no-allocation entry point (non-zero: skip), or null
 };
 
 // Reach of the neighbour lists: every dumbbell whose Edumb term can read the
@@ -236,6 +247,7 @@ __device__ int64_t find_site(const int64_t* sites, int64_t n, int64_t s) {
 // ascending coordinate values), and a ballot keeps that order.
 constexpr int PREP_THREADS = 64;
 __global__ __launch_bounds__(PREP_THREADS) void itl_prep_kernel(Args a) {
+  if (a.status && *a.status) return;
   const int64_t k = blockIdx.x;
   const int l = threadIdx.x;
   const int64_t s = a.sites[k];
@@ -285,6 +297,7 @@ __device__ __forceinline__ double read_lane(double v, int i) {
 // sum of it, in list order.  One wavefront: 64 terms per load, folded in
 // order through lane reads (every lane holds the same running sum).
 __global__ __launch_bounds__(64) void itl_prefix_kernel(Args a) {
+  if (a.status && *a.status) return;
   const int l = threadIdx.x;
   double acc = 0;
   double t_next = l < a.n ? a.term[l] : 0.0;
@@ -339,6 +352,7 @@ __device__ __forceinline__ bool near(const Dims& d, int x, int y, int z, int cx,
 // dumbbell's neighbour list); Edumb before is the full prefix fold.
 constexpr int RATE_THREADS = 256;
 __global__ __launch_bounds__(RATE_THREADS) void itl_rate_kernel(Args a) {
+  if (a.status && *a.status) return;
   const double* tm = a.term;
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t k = g >> 3;
@@ -420,11 +434,8 @@ struct DevBuf {
   hipError_t alloc(size_t n) { return hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)); }
 };
 
-bool valid_single(int t) { return t == 0 || t == 1 || t == 2 || t == 4 || t == 8; }
-bool valid_dumbbell(int t) {
-  const int hi = t >> 8, lo = t & 0xFF;
-  return hi != 0 && valid_single(hi) && valid_single(lo) && hi <= lo && (t >> 16) == 0;
-}
+using akmc::itl::valid_dumbbell;
+using akmc::itl::valid_single;
 
 }  // namespace
 
@@ -498,6 +509,7 @@ extern "C" int akmc_itl_rates(const int64_t box[3], const uint16_t* types, int64
   a.nbr_cnt = d_nbr_cnt.p;
   a.avail = d_avail.p;
   a.rates = d_rates.p;
+  a.status = nullptr;  // inputs were checked on the host above
   hipEvent_t ev[2];
   HIP_TRY(hipEventCreate(&ev[0]));
   if (hipEventCreate(&ev[1]) != hipSuccess) {
@@ -526,5 +538,136 @@ extern "C" int akmc_itl_rates(const int64_t box[3], const uint16_t* types, int64
   float ms = 0.0f;
   HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[1]));
   if (kernel_seconds) *kernel_seconds = ms * 1e-3;
+  return AKMC_OK;
+}
+
+// ---------------------------------------------------------------- no allocation
+// akmc_itl_rates_noalloc: the same kernels with caller-supplied device buffers
+// and the working arrays (per dumbbell: info, term, prefix, neighbour list and
+// count) in one static device array of a fixed maximum size.  Each device
+// holds its own copy of the array, at its own address, from the library's
+// load until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device.
+namespace akmc {
+namespace itl {
+
+constexpr int64_t NOALLOC_MAX = AKMC_ITL_NOALLOC_MAX;
+constexpr size_t SCR_STATUS = 0;                                  // int32 input-check flag
+constexpr size_t SCR_INFO = 256;                                  // int4 x MAX
+constexpr size_t SCR_TERM = SCR_INFO + 16 * NOALLOC_MAX;          // double x MAX
+constexpr size_t SCR_PREFIX = SCR_TERM + 8 * NOALLOC_MAX;         // double x MAX
+constexpr size_t SCR_NBR = SCR_PREFIX + 8 * NOALLOC_MAX;          // int32 x MAX x NBR_CAP
+constexpr size_t SCR_NBR_CNT = SCR_NBR + 4 * NOALLOC_MAX * NBR_CAP;  // int32 x MAX
+constexpr size_t SCR_BYTES = SCR_NBR_CNT + 4 * NOALLOC_MAX;
+
+static __device__ __align__(256) unsigned char scratch[SCR_BYTES];
+
+// The flag starts at zero on every call (a kernel, not a memset).
+__global__ void itl_clear_kernel(int32_t* status) { *status = 0; }
+
+// Device-side input check: the conditions akmc_itl_rates checks on the host.
+// Any failure sets the flag, and the rate kernels then return at once, so an
+// invalid input reads nothing out of bounds and writes no output.
+__global__ __launch_bounds__(256) void itl_check_kernel(const uint16_t* types, int64_t nsites, const int64_t* sites,
+                                                        const int8_t* orient, int64_t n, int32_t* status) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  bool bad = false;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nsites; i += stride) {
+    const int t = types[i];
+    bad = bad || !(valid_single(t) || valid_dumbbell(t));
+  }
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+    const int64_t sk = sites[k];
+    const int o = orient[k];
+    if (sk < 0 || sk >= nsites) bad = true;
+    else bad = bad || !valid_dumbbell(types[sk]) || (k > 0 && !(sites[k - 1] < sk));
+    bad = bad || o == 0 || o < -6 || o > 6;
+  }
+  if (bad) *status = 1;  // every writer stores the same value
+}
+
+}  // namespace itl
+}  // namespace akmc
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<unsigned char*> g_scratch_ptr;  // per device, from hipGetSymbolAddress
+
+// The scratch array's device address on `device` (current), looked up once.
+int scratch_address(int device, unsigned char** out) {
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if ((size_t)device >= g_scratch_ptr.size()) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+  if (!g_scratch_ptr[(size_t)device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(akmc::itl::scratch));
+    if (e != hipSuccess || !p)
+      return fail(AKMC_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[(size_t)device] = (unsigned char*)p;
+  }
+  *out = g_scratch_ptr[(size_t)device];
+  return AKMC_OK;
+}
+}  // namespace
+
+extern "C" int akmc_itl_rates_noalloc(const int64_t box[3], const uint16_t* d_types, int64_t n_itl,
+                                      const int64_t* d_sites, const int8_t* d_orient, double attempt_freq,
+                                      double temperature, int device, void* stream, uint8_t* d_avail,
+                                      double* d_rates) {
+  using namespace akmc::itl;
+  if (!box || !d_types || n_itl < 0 || (n_itl > 0 && (!d_sites || !d_orient || !d_avail || !d_rates)))
+    return fail(AKMC_ERR_INVALID, "null argument");
+  if (n_itl > NOALLOC_MAX)
+    return fail(AKMC_ERR_CAPACITY, "%lld dumbbells: the no-allocation entry point takes at most %lld",
+                (long long)n_itl, (long long)NOALLOC_MAX);
+  for (int i = 0; i < 3; i++)
+    if (box[i] < 3 || box[i] > (1 << 20)) return fail(AKMC_ERR_INVALID, "box dimension %d is %lld (3 .. 2^20)", i, (long long)box[i]);
+  if (!(temperature > 0) || !(attempt_freq >= 0) || !std::isfinite(attempt_freq))
+    return fail(AKMC_ERR_INVALID, "bad temperature or attempt frequency");
+  if (device < 0) return fail(AKMC_ERR_INVALID, "bad device");
+  if (n_itl == 0) return AKMC_OK;
+  const int64_t X = 2 * box[0], nsites = X * box[1] * box[2];
+  HIP_TRY(hipSetDevice(device));
+  unsigned char* base = nullptr;
+  {
+    const int rc = scratch_address(device, &base);
+    if (rc) return rc;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  int32_t* status = reinterpret_cast<int32_t*>(base + SCR_STATUS);
+  Args a;
+  a.d = Dims{(int32_t)X, (int32_t)box[1], (int32_t)box[2]};
+  a.types = d_types;
+  a.sites = d_sites;
+  a.orient = d_orient;
+  a.n = n_itl;
+  a.nu = attempt_freq;
+  a.kT = AKMC_KB * temperature;  // RatesSolver::arrhenius: kB * T first
+  a.info = reinterpret_cast<int4*>(base + SCR_INFO);
+  a.term = reinterpret_cast<double*>(base + SCR_TERM);
+  a.prefix = reinterpret_cast<double*>(base + SCR_PREFIX);
+  a.nbr = reinterpret_cast<int32_t*>(base + SCR_NBR);
+  a.nbr_cnt = reinterpret_cast<int32_t*>(base + SCR_NBR_CNT);
+  a.avail = d_avail;
+  a.rates = d_rates;
+  a.status = status;
+  hipLaunchKernelGGL(itl_clear_kernel, dim3(1), dim3(1), 0, st, status);
+  HIP_TRY(hipGetLastError());
+  const int64_t work = std::max(nsites, n_itl);
+  const unsigned cblocks = (unsigned)std::min<int64_t>(4096, (work + 255) / 256);
+  hipLaunchKernelGGL(itl_check_kernel, dim3(cblocks), dim3(256), 0, st, d_types, nsites, d_sites, d_orient, n_itl,
+                     status);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(itl_prep_kernel, dim3((unsigned)n_itl), dim3(PREP_THREADS), 0, st, a);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(itl_prefix_kernel, dim3(1), dim3(64), 0, st, a);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(itl_rate_kernel, dim3((unsigned)((8 * n_itl + RATE_THREADS - 1) / RATE_THREADS)),
+                     dim3(RATE_THREADS), 0, st, a);
+  HIP_TRY(hipGetLastError());
+  int32_t flag = 0;
+  HIP_TRY(hipMemcpyAsync(&flag, status, sizeof(flag), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (flag)
+    return fail(AKMC_ERR_INVALID, "invalid input: a site type, a dumbbell site (range, order, type) or an orientation");
   return AKMC_OK;
 }
