@@ -84,16 +84,25 @@ def cpu_baseline(events, seconds_budget):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import refcpu
     ref = refcpu.RefCpu(WORKLOAD)
-    t0 = time.perf_counter()
-    done = 0
-    while done < events and time.perf_counter() - t0 < seconds_budget:
-        ref.run(10)
-        done = ref.state()["events"]
-    dt = time.perf_counter() - t0
+    # Pinned to one core for the timed loop (as taskset -c would): the oracle
+    # runs in this thread, so its affinity is set and restored afterwards.
+    allowed = os.sched_getaffinity(0)
+    core = min(allowed)
+    os.sched_setaffinity(0, {core})
+    try:
+        t0 = time.perf_counter()
+        done = 0
+        while done < events and time.perf_counter() - t0 < seconds_budget:
+            ref.run(10)
+            done = ref.state()["events"]
+        dt = time.perf_counter() - t0
+    finally:
+        os.sched_setaffinity(0, allowed)
     return {"value": done / dt, "unit": "events/s", "cores": 1, "kind": "port", "host": host_cpu(),
+            "pinned_cpu": core,
             "sample": "first %d events of the same 256^3 FP64 workload, oracle/refcpu.cpp "
                       "(reference CPU algorithm: hash occupancy, full-sector sweep per event), "
-                      "1 thread, initial occupancy excluded, %.1f s" % (done, dt)}
+                      "1 thread pinned to CPU %d, initial occupancy excluded, %.1f s" % (done, core, dt)}
 
 
 def _oracle_replica(cfg, seconds):
@@ -140,33 +149,40 @@ def batched_leg(args, plain_value):
         b.run(args.batch_events)
     akmc.sync(0)
     before = [b.replica_stats(i) for i in range(b.n)]
-    # Throughput: each step is one launch in which every replica runs until a
-    # deadline (no replica waits for another).
-    ev, ksec = 0, 0.0
+    # Throughput: each step advances every replica by --batch-events events
+    # through replays of the captured hipGraph (a step ends with its slowest
+    # replica).
+    ev, ksec, launches = 0, 0.0, 0
     t0 = time.perf_counter()
     for i in range(args.steps):
-        st = b.run_for(args.batch_seconds)
+        st = b.run(args.batch_events)
         ev += st["events"]
         ksec += st["kernel_seconds"]
+        launches += st["launches"]
     akmc.sync(0)
     dt = time.perf_counter() - t0
-    launches = args.steps
     after = [b.replica_stats(i) for i in range(b.n)]
     d_eval = sum(a["n_eval"] - x["n_eval"] for a, x in zip(after, before))
     # Algorithmic HBM bytes: 55 stencil bytes per evaluation, the 2-byte swap
     # and the 4-byte vacancy key per event (the sector state stays in LDS).
     nbytes = d_eval * 55 + ev * 6
     achieved = nbytes / max(1, launches) / (ksec / max(1, launches)) / 1e9
-    fixed = None
-    if not args.no_fixed:
-        # Time to advance every replica by the same number of events (bounded
-        # by the slowest replica).
+    # Beside it: deadline launches, in which every replica runs until a
+    # deadline so that no replica waits for another (a launch of its own,
+    # outside the graph).
+    deadline = None
+    if not args.no_deadline:
+        dev, dks = 0, 0.0
         t1 = time.perf_counter()
-        st_fixed = b.run(args.batch_events)
+        for _ in range(args.deadline_steps):
+            st = b.run_for(args.batch_seconds)
+            dev += st["events"]
+            dks += st["kernel_seconds"]
         akmc.sync(0)
-        dt_fixed = time.perf_counter() - t1
-        fixed = {"events_per_replica": args.batch_events, "seconds": dt_fixed,
-                 "events_per_s": st_fixed["events"] / dt_fixed}
+        ddt = time.perf_counter() - t1
+        deadline = {"events_per_s": dev / ddt, "launches": args.deadline_steps, "events": dev,
+                    "seconds": ddt, "step": "one launch, every replica runs until a %.0f ms deadline"
+                                            % (args.batch_seconds * 1e3)}
     nvac = sum(b.replica_stats(i)["nvac"] for i in range(b.n))
     sites = sum(b.num_sites(i) for i in range(b.n))
     b.close()
@@ -184,9 +200,9 @@ def batched_leg(args, plain_value):
                       "sample": "64 replicas x 2000 events, every rate of every evaluation"}
     value = ev / dt
     return {"value": value, "unit": "events/s", "replicas": len(cfgs), "sites": sites, "vacancies": nvac,
-            "step": "one launch, every replica runs until a %.0f ms deadline" % (args.batch_seconds * 1e3),
+            "step": "every replica advances by %d events through hipGraph replays" % args.batch_events,
             "events_timed": ev, "ms_per_step": dt / args.steps * 1e3,
-            "fixed_events": fixed,
+            "deadline": deadline,
             "kernel_seconds": ksec, "launches": launches, "dtype": "f32 rates, f64 sums/search/dt",
             "tuning": tun, "vs_plain": value / plain_value if plain_value else None,
             "evaluations_per_event": d_eval / max(1, ev),
@@ -377,7 +393,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--events-per-step", type=int, default=1_000_000)
+    ap.add_argument("--events-per-step", type=int, default=250_000)
     ap.add_argument("--cpu-events", type=int, default=400)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -397,7 +413,8 @@ def main():
     ap.add_argument("--batch-waves", type=int, default=1)
     ap.add_argument("--batch-order", type=int, default=0)
     ap.add_argument("--batch-warmup", type=int, default=1)
-    ap.add_argument("--no-fixed", action="store_true", help="batched leg: skip the fixed-event run")
+    ap.add_argument("--no-deadline", action="store_true", help="batched leg: skip the deadline launches")
+    ap.add_argument("--deadline-steps", type=int, default=3)
     ap.add_argument("--no-error-stats", action="store_true", help="batched leg: skip the FP32 error sample")
     ap.add_argument("--config0", action="store_true",
                     help="instead: the plain engine and the CPU baseline on the reference's 40^3 case, 1e5 events")
@@ -442,11 +459,27 @@ def main():
         t0 = time.perf_counter()
         ksec = 0.0
         launches = 0
+        per_step = []
+        prev, tp = s0, t0
         for i in range(args.steps):
             st = eng.run(E)
             ksec += st["kernel_seconds"]
             launches += st["launches"]
-            print("bench: step %d/%d events %d" % (i + 1, args.steps, st["events"]), file=sys.stderr, flush=True)
+            # Per-step figures (the host reads counters the run already
+            # returned; no extra device work): they show how the cost per
+            # event moves as the lattice evolves.
+            tn = time.perf_counter()
+            de = max(1, st["events"] - prev["events"])
+            rec = {"events": st["events"], "seconds": tn - tp, "us_per_event": (tn - tp) / de * 1e6,
+                   "kernel_us_per_event": st["kernel_seconds"] / de * 1e6,
+                   "eval_per_event": (st["n_eval"] - prev["n_eval"]) / de,
+                   "fold_per_event": (st["n_fold"] - prev["n_fold"]) / de,
+                   "shift_per_event": (st["n_shift"] - prev["n_shift"]) / de}
+            per_step.append(rec)
+            prev, tp = st, tn
+            print("bench: step %d/%d events %d  %.3f s  %.2f us/event  eval %.2f  fold %.0f  shift %.0f per event"
+                  % (i + 1, args.steps, st["events"], rec["seconds"], rec["us_per_event"], rec["eval_per_event"],
+                     rec["fold_per_event"], rec["shift_per_event"]), file=sys.stderr, flush=True)
         akmc.sync(0)
         dt = time.perf_counter() - t0
         s1 = eng.stats()
@@ -467,7 +500,15 @@ def main():
                          "frac": achieved / HBM_PEAK_GBS,
                          "traffic": PLAIN_TRAFFIC_BYTES_PER_EVENT * d_events / max(1, launches),
                          "algorithmic_bytes_per_event": nbytes / max(1, d_events),
-                         "kernel_us_per_event": ksec / max(1, d_events) * 1e6}})
+                         "kernel_us_per_event": ksec / max(1, d_events) * 1e6,
+                         # HBM bytes the counters measure per event (the algorithmic bytes are mostly
+                         # sector state kept in LDS): the fraction of peak the device traffic reaches.
+                         "hbm_counter_bytes_per_event": PLAIN_TRAFFIC_BYTES_PER_EVENT,
+                         "hbm_counter_frac": PLAIN_TRAFFIC_BYTES_PER_EVENT * d_events / max(ksec, 1e-12)
+                                             / 1e9 / HBM_PEAK_GBS,
+                         "note": "one wavefront runs the serial event chain: bound by dependent latency "
+                                 "(LDS, L2/HBM gathers, FP64 folds), not by HBM bandwidth"},
+            "per_step": per_step})
     if args.copy_calibration:
         akmc.copy_bench(1 << 30, iters=5)
     batched = None
