@@ -494,6 +494,112 @@ int akmc_server_get_stats(akmc_server* s, akmc_server_stats* st);
 double akmc_server_clock(akmc_server* s);
 void akmc_server_destroy(akmc_server* s);
 
+/* ------------------------------------------------------------------------
+ * Vacancies and self-interstitial dumbbells: the abvi model's event loop on a
+ * typed lattice (one LatticeTypes value per site, as akmc_itl_rates takes).
+ *
+ * The reference has the pieces but no wired loop: the dumbbell branch of
+ * ABVIModel::select is commented out (models/abvi/kmc.cpp:251-268), perform
+ * asserts on every event but a vacancy hop (:393-395), and the dumbbell
+ * transition, recombination and defect generation are described in comments
+ * (:310-355).  These entry points wire them as the oracle restates them
+ * (oracle/refcpu.cpp, refmix): the sublattice time loop and selection of
+ * the vacancy engine, with a sector's vacancies and dumbbells interleaved in
+ * sweep order (8 slots each: a vacancy's direction rates, a dumbbell's
+ * Itl::updateRates slots), dumbbell transitions by orientation::tranAtom and
+ * orientation::trans (orientation.cc:238-252), recombination of a vacancy
+ * and a dumbbell at 1nn or 2nn after every transition (rec::RecList,
+ * recombine.cpp, with an mt19937 of seed_recombine in place of std::rand),
+ * and defect generation at defect_gen_rate (env.h:23; 0 = none, as the
+ * reference configures it): a sector site becomes a vacancy and its atom
+ * joins one of the site's 12 third neighbours as a dumbbell.  Each handle holds any number of lattices; a run
+ * advances all of them in one launch, one wavefront per lattice.
+ * ---------------------------------------------------------------------- */
+typedef struct akmc_mix_params {
+  int64_t box[3];            /* cells; every edge >= 8 */
+  double temperature;        /* K */
+  double attempt_freq;       /* 1/s */
+  double sector_time;        /* sublattice threshold per sector cycle, s */
+  double physics_time;       /* s; the time loop runs ceil(physics_time / sector_time) steps */
+  double ef110;              /* <110> dumbbell formation energy in pure Fe, eV (env.h:22; default 3.53) */
+  double defect_gen_rate;    /* Frenkel-pair generation rate per sector, 1/s (env.h:23; default 0) */
+  int64_t seed_time_inc;     /* time stream, as akmc_params */
+  int64_t seed_recombine;    /* stand-in stream for std::rand() and the generation draws */
+  int32_t device_rng;        /* 0: the reference's mt19937 streams from the host (bit-exact with the oracle);
+                                1: xoshiro256** streams on the device (reproducible, no host work) */
+  int32_t reserved;
+  int64_t vac_capacity;      /* list capacities (0: the initial counts, + 4096 when generating) */
+  int64_t itl_capacity;
+} akmc_mix_params;
+
+typedef struct akmc_mix_stats {
+  int64_t events;
+  int64_t step;
+  int32_t sector_pos;
+  int32_t in_sector;
+  double evolution[8];
+  double sector_time;
+  int32_t stop;              /* AKMC_STOP_TIME or AKMC_STOP_EVENTS */
+  int32_t nvac;              /* vacancies now */
+  int32_t nitl;              /* dumbbells now */
+  int32_t reserved;
+  int64_t n_recombined;      /* vacancy-dumbbell pairs recombined */
+  int64_t n_generated;       /* Frenkel pairs generated */
+  int64_t n_itl_events;      /* dumbbell transitions */
+  int64_t n_eval;            /* rate records evaluated */
+  double seconds;            /* wall time of the handle's last akmc_mix_run */
+  double kernel_seconds;     /* device time of its launches */
+  int64_t launches;
+} akmc_mix_stats;
+
+typedef struct akmc_mix akmc_mix;
+
+int akmc_mix_params_default(akmc_mix_params* p);
+
+/* Creates n lattices on `device`: types[i] holds 2 box[0] box[1] box[2]
+ * LatticeTypes values; itl_sites[i] / itl_orient[i] list its n_itl[i]
+ * dumbbells in ascending site order with orientations in [-6, 6] \ {0}. */
+int akmc_mix_create(int32_t n, const akmc_mix_params* p, const uint16_t* const* types, const int64_t* n_itl,
+                    const int64_t* const* itl_sites, const int8_t* const* itl_orient, int device, akmc_mix** out);
+
+/* Advances every lattice by up to max_events events (fewer when its time loop
+ * ends); *total_events (may be NULL) receives the events of all lattices. */
+int akmc_mix_run(akmc_mix* m, int64_t max_events, int64_t* total_events);
+
+int akmc_mix_stats_of(akmc_mix* m, int32_t lattice, akmc_mix_stats* stats);
+
+/* The lattice's types (n_sites of them, may be NULL) and its dumbbell list
+ * (up to cap entries; *n_itl receives the count). */
+int akmc_mix_download(akmc_mix* m, int32_t lattice, uint16_t* types, int64_t n_sites, int64_t cap,
+                      int64_t* itl_sites, int8_t* itl_orient, int64_t* n_itl);
+
+/* Parity hook: calcRates of one sector: its vacancies (kind 0) and dumbbells
+ * (kind 1) in sweep order with their 8 slots, and the total (with
+ * defect_gen_rate). */
+int akmc_mix_sector_rates(akmc_mix* m, int32_t lattice, int sector, int64_t cap, int64_t* sites, int32_t* kinds,
+                          double* rates, double* total, int64_t* n);
+
+/* Event log of every lattice (capacity events each; 0 disables): from site,
+ * to site (-1 for a generation that changed nothing) and kind (0 vacancy
+ * hop, 1 dumbbell transition, 2 defect generation). */
+int akmc_mix_set_event_log(akmc_mix* m, int64_t capacity);
+int akmc_mix_get_event_log(akmc_mix* m, int32_t lattice, int64_t* from, int64_t* to, int32_t* kind, int64_t cap,
+                           int64_t* n);
+
+void akmc_mix_destroy(akmc_mix* m);
+
+/* Initial typed lattice: creation::createRandom's occupancy
+ * (akmc_random_sites; a vacancy's leftover Ni/Mn tag is dropped), then
+ * placeDumbbells of the reference's serial driver
+ * (profile_serial/main.cpp:137-161) on an mt19937 of `seed`: for each kind
+ * k, itl_counts[k] draws of a site (mt() % sites); a single atom there
+ * becomes the dumbbell of itself and an atom of itl_types[k] with
+ * orientation::random() (orientation.h:38-42).  The dumbbells come back in
+ * ascending site order (up to cap; *n_itl receives the count). */
+int akmc_mix_random_lattice(const akmc_params* p, const int32_t* itl_types, const int64_t* itl_counts,
+                            int32_t n_kinds, int64_t seed, uint16_t* types, int64_t n_sites, int64_t cap,
+                            int64_t* itl_sites, int8_t* itl_orient, int64_t* n_itl);
+
 const char* akmc_last_error(void);
 
 #ifdef __cplusplus

@@ -730,3 +730,159 @@ def itl_rates_noalloc(box, d_types, n_itl, d_sites, d_orient, attempt_freq, temp
                                         ctypes.c_void_p(d_orient), float(attempt_freq), float(temperature),
                                         int(device), ctypes.c_void_p(stream), ctypes.c_void_p(d_avail),
                                         ctypes.c_void_p(d_rates)))
+
+
+# ---------------------------------------------------------------- vacancies and dumbbells
+class MixParams(ctypes.Structure):
+    _fields_ = [("box", ctypes.c_int64 * 3), ("temperature", ctypes.c_double), ("attempt_freq", ctypes.c_double),
+                ("sector_time", ctypes.c_double), ("physics_time", ctypes.c_double), ("ef110", ctypes.c_double),
+                ("defect_gen_rate", ctypes.c_double), ("seed_time_inc", ctypes.c_int64),
+                ("seed_recombine", ctypes.c_int64), ("device_rng", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("vac_capacity", ctypes.c_int64), ("itl_capacity", ctypes.c_int64)]
+
+
+class MixStats(ctypes.Structure):
+    _fields_ = [("events", ctypes.c_int64), ("step", ctypes.c_int64), ("sector_pos", ctypes.c_int32),
+                ("in_sector", ctypes.c_int32), ("evolution", ctypes.c_double * 8), ("sector_time", ctypes.c_double),
+                ("stop", ctypes.c_int32), ("nvac", ctypes.c_int32), ("nitl", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("n_recombined", ctypes.c_int64), ("n_generated", ctypes.c_int64),
+                ("n_itl_events", ctypes.c_int64), ("n_eval", ctypes.c_int64), ("seconds", ctypes.c_double),
+                ("kernel_seconds", ctypes.c_double), ("launches", ctypes.c_int64)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("evolution", "reserved")}
+        d["evolution"] = np.array(self.evolution[:])
+        return d
+
+
+def _bind_mix(L):
+    if getattr(L, "_mix_bound", False):
+        return L
+    vp, i64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32
+    P = ctypes.POINTER
+    L.akmc_mix_params_default.argtypes = [P(MixParams)]
+    L.akmc_mix_create.argtypes = [i32, vp, vp, vp, vp, vp, ctypes.c_int, P(vp)]
+    L.akmc_mix_run.argtypes = [vp, i64, P(i64)]
+    L.akmc_mix_stats_of.argtypes = [vp, i32, P(MixStats)]
+    L.akmc_mix_download.argtypes = [vp, i32, vp, i64, i64, vp, vp, P(i64)]
+    L.akmc_mix_sector_rates.argtypes = [vp, i32, ctypes.c_int, i64, vp, vp, vp, P(ctypes.c_double), P(i64)]
+    L.akmc_mix_set_event_log.argtypes = [vp, i64]
+    L.akmc_mix_get_event_log.argtypes = [vp, i32, vp, vp, vp, i64, P(i64)]
+    L.akmc_mix_destroy.argtypes = [vp]
+    L.akmc_mix_destroy.restype = None
+    L.akmc_mix_random_lattice.argtypes = [P(Params), vp, vp, i32, i64, vp, i64, i64, vp, vp, P(i64)]
+    L._mix_bound = True
+    return L
+
+
+def mix_params(**kw):
+    p = MixParams()
+    _check(_bind_mix(lib()).akmc_mix_params_default(ctypes.byref(p)))
+    for k, v in kw.items():
+        if k == "box":
+            for d in range(3):
+                p.box[d] = int(v[d])
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def mix_random_lattice(config, itl=(("Fe", 0),), seed=1):
+    """creation::createRandom's lattice as LatticeTypes values plus dumbbells
+    placed as the reference's placeDumbbells does: (types, sites, orient)."""
+    L = _bind_mix(lib())
+    prm = config if isinstance(config, Params) else params_from_config(config)
+    n = 2 * prm.box[0] * prm.box[1] * prm.box[2]
+    kinds = np.array([LAT_TYPES[k] for k, _ in itl], dtype=np.int32)
+    counts = np.array([int(c) for _, c in itl], dtype=np.int64)
+    t = np.zeros(n, np.uint16)
+    cap = int(counts.sum()) + 1
+    s = np.zeros(cap, np.int64)
+    o = np.zeros(cap, np.int8)
+    m = ctypes.c_int64()
+    _check(L.akmc_mix_random_lattice(ctypes.byref(prm), kinds.ctypes.data, counts.ctypes.data, len(kinds), int(seed),
+                                     t.ctypes.data, n, cap, s.ctypes.data, o.ctypes.data, ctypes.byref(m)))
+    return t, s[:m.value], o[:m.value]
+
+
+class Mix:
+    """Lattices with vacancies and dumbbells (akmc_mix_*), one wavefront each."""
+
+    def __init__(self, lattices, device=0):
+        """lattices: list of (MixParams, types, sites, orient)."""
+        L = _bind_mix(lib())
+        self.n = len(lattices)
+        self._keep = []
+        prm = (MixParams * self.n)(*[x[0] for x in lattices])
+        tp = (ctypes.c_void_p * self.n)()
+        sp = (ctypes.c_void_p * self.n)()
+        op = (ctypes.c_void_p * self.n)()
+        ni = np.zeros(self.n, np.int64)
+        self.n_sites = []
+        for i, (p, t, s, o) in enumerate(lattices):
+            t = np.ascontiguousarray(t, np.uint16)
+            s = np.ascontiguousarray(s, np.int64)
+            o = np.ascontiguousarray(o, np.int8)
+            self._keep += [t, s, o]
+            tp[i], sp[i], op[i] = t.ctypes.data, s.ctypes.data, o.ctypes.data
+            ni[i] = s.size
+            self.n_sites.append(t.size)
+        h = ctypes.c_void_p()
+        _check(L.akmc_mix_create(self.n, ctypes.cast(prm, ctypes.c_void_p), ctypes.cast(tp, ctypes.c_void_p),
+                                 ni.ctypes.data, ctypes.cast(sp, ctypes.c_void_p), ctypes.cast(op, ctypes.c_void_p),
+                                 int(device), ctypes.byref(h)))
+        self.h = h
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            _bind_mix(lib()).akmc_mix_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def run(self, max_events):
+        tot = ctypes.c_int64()
+        _check(_bind_mix(lib()).akmc_mix_run(self.h, int(max_events), ctypes.byref(tot)))
+        return tot.value
+
+    def stats(self, i=0):
+        st = MixStats()
+        _check(_bind_mix(lib()).akmc_mix_stats_of(self.h, int(i), ctypes.byref(st)))
+        return st.as_dict()
+
+    def download(self, i=0):
+        L = _bind_mix(lib())
+        t = np.zeros(self.n_sites[i], np.uint16)
+        m = ctypes.c_int64()
+        _check(L.akmc_mix_download(self.h, int(i), t.ctypes.data, t.size, 0, None, None, ctypes.byref(m)))
+        s = np.zeros(max(m.value, 1), np.int64)
+        o = np.zeros(max(m.value, 1), np.int8)
+        _check(L.akmc_mix_download(self.h, int(i), None, 0, m.value, s.ctypes.data, o.ctypes.data, ctypes.byref(m)))
+        return t, s[:m.value], o[:m.value]
+
+    def sector_rates(self, i, sector, cap=1 << 16):
+        L = _bind_mix(lib())
+        s = np.zeros(cap, np.int64)
+        k = np.zeros(cap, np.int32)
+        r = np.zeros((cap, 8))
+        tot = ctypes.c_double()
+        n = ctypes.c_int64()
+        _check(L.akmc_mix_sector_rates(self.h, int(i), int(sector), cap, s.ctypes.data, k.ctypes.data, r.ctypes.data,
+                                       ctypes.byref(tot), ctypes.byref(n)))
+        m = min(n.value, cap)
+        return s[:m], k[:m], r[:m], tot.value
+
+    def set_event_log(self, capacity):
+        _check(_bind_mix(lib()).akmc_mix_set_event_log(self.h, int(capacity)))
+
+    def event_log(self, i, cap):
+        f = np.zeros(cap, np.int64)
+        t = np.zeros(cap, np.int64)
+        k = np.zeros(cap, np.int32)
+        n = ctypes.c_int64()
+        _check(_bind_mix(lib()).akmc_mix_get_event_log(self.h, int(i), f.ctypes.data, t.ctypes.data, k.ctypes.data,
+                                                       cap, ctypes.byref(n)))
+        m = min(n.value, cap)
+        return f[:m], t[:m], k[:m]

@@ -20,6 +20,9 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <map>
+#include <array>
+#include <algorithm>
 #include <fstream>
 #include <random>
 #include <sstream>
@@ -1128,6 +1131,464 @@ static void rates(Typed& L, std::vector<Itl>& list, double nu, double T) {
 }
 
 }  // namespace refitl
+
+// ---------------------------------------------------------------- refmix
+// The abvi model's event loop with vacancies and self-interstitial dumbbells
+// on a typed lattice.  The reference has the pieces but no wired loop: the
+// dumbbell branch of ABVIModel::select is commented out
+// (models/abvi/kmc.cpp:251-268), perform() handles vacancy hops only and
+// asserts otherwise (:358-396), and its comments (:310-355) describe the
+// dumbbell transition, recombination and defect generation in words.  This
+// restatement wires them as follows (each choice is stated where it is made):
+//  * calcRates over a sector sweeps z, y, x like ABVIModel::calcRates
+//    (kmc.cpp:68-115); a vacancy contributes its 8 direction rates
+//    (VacRatesSolver on the typed lattice: dumbbell neighbours carry no bond,
+//    bonds_counter.cpp:35-81; a direction is available iff its 1nn is a single
+//    atom, vac_hash.cpp:7-15), a dumbbell its 8 rate slots (Itl::updateRates,
+//    itl_list.cpp:31-50, ItlRatesSolver).  The total folds every slot in that
+//    order (zero slots add nothing), then adds defect_gen_rate
+//    (ABVIModel::defectGenRate, kmc.cpp:207).  Every dumbbell's availability
+//    mask is recomputed from the lattice before any rate (the masks
+//    bond::Edumb reads, energy_dumbbell.cpp:144).
+//  * select follows the commented branch (kmc.cpp:251-268): the first slot
+//    whose running sum exceeds u * total; a dumbbell slot i is the
+//    transition towards 1nn tag Itl::get1nnIdByRatesIndex(i, dirs)
+//    (itl_list.cpp:76-91) with rotate = i odd; past the last slot the
+//    default event is defect generation (kmc.cpp:245).
+//  * a dumbbell transition (kmc.cpp:324-333 in words): the jumping atom g =
+//    orientation::tranAtom (orientation.cc:238-253) leaves the dumbbell, whose
+//    other atom stays (LatticeTypes::diff), and forms a dumbbell with the atom
+//    at the target (combineToInter); the new orientation is
+//    orientation::trans(tag, g at the high end of the new dumbbell, rotate)
+//    (orientation.cc:245-252); the list entry moves to the target.
+//  * after every transition, recombination around the moved defect
+//    (rec::RecList::create / pickMinimum / Rec::recombine, recombine.cpp:
+//    11-140): every vacancy-dumbbell pair it forms at 1nn or (masked) 2nn
+//    is a candidate.  stageCombination compares the high end with itself
+//    (recombine.cpp:91), so only "high atom to the vacancy, low atom stays"
+//    is staged; pickMinimum keeps the candidates within 1e-6 of the lowest
+//    energy through abs(), which for a double argument resolves to the int
+//    overload with these includes under g++ 11 (a |difference| below 1 eV
+//    truncates to 0), and draws one with std::rand(); here a stable sort and
+//    an mt19937 seeded from seed_event_selection (parsed but unused by the
+//    reference) stand in.  Edumb inside a candidate uses the masks of the
+//    lattice before the candidate change, as a rate does.
+//  * defect generation (kmc.cpp:334-353 in words: a random lattice, a
+//    partner next to it, one becomes a vacancy and the other a dumbbell with
+//    a random orientation): a site of the sector and one of its 12 third
+//    neighbours (<110> type, the nearest shell outside the 1nn/2nn
+//    recombination range) are drawn from the same stand-in stream; if both
+//    hold single atoms the first becomes a vacancy and its atom joins the
+//    second as a dumbbell of orientation::random() (orientation.h:38-42);
+//    otherwise nothing changes.  Recombination follows around the new
+//    vacancy.
+namespace refmix {
+
+using refitl::Itl;
+using refitl::Typed;
+
+// orientation::TransHashTable (orientation.cc:23-140), indexed
+// [|o| - 1][1nn tag][!new_higher][!rotate]; 0 = unknown.
+static const int8_t kTrans[6][8][2][2] = {
+    {{{-3, -5}, {3, 5}}, {{-4, -6}, {4, 6}}, {{0, 0}, {0, 0}}, {{0, 0}, {0, 0}},
+     {{0, 0}, {0, 0}}, {{0, 0}, {0, 0}}, {{4, 6}, {-4, -6}}, {{3, 5}, {-3, -5}}},
+    {{{0, 0}, {0, 0}}, {{0, 0}, {0, 0}}, {{4, -5}, {-4, 5}}, {{3, -6}, {-3, 6}},
+     {{-3, 6}, {3, -6}}, {{-4, 5}, {4, -5}}, {{0, 0}, {0, 0}}, {{0, 0}, {0, 0}}},
+    {{{-1, -5}, {1, 5}}, {{0, 0}, {0, 0}}, {{0, 0}, {0, 0}}, {{-6, -2}, {6, 2}},
+     {{6, 2}, {-6, -2}}, {{0, 0}, {0, 0}}, {{0, 0}, {0, 0}}, {{1, 5}, {-1, -5}}},
+    {{{0, 0}, {0, 0}}, {{-1, -6}, {1, 6}}, {{-2, -5}, {2, 5}}, {{0, 0}, {0, 0}},
+     {{0, 0}, {0, 0}}, {{2, 5}, {-2, -5}}, {{1, 6}, {-1, -6}}, {{0, 0}, {0, 0}}},
+    {{{-3, -1}, {3, 1}}, {{0, 0}, {0, 0}}, {{-4, 2}, {4, -2}}, {{0, 0}, {0, 0}},
+     {{0, 0}, {0, 0}}, {{4, -2}, {-4, 2}}, {{0, 0}, {0, 0}}, {{3, 1}, {-3, -1}}},
+    {{{0, 0}, {0, 0}}, {{-4, -1}, {4, 1}}, {{0, 0}, {0, 0}}, {{-3, 2}, {3, -2}},
+     {{3, -2}, {-3, 2}}, {{0, 0}, {0, 0}}, {{4, 1}, {-4, -1}}, {{0, 0}, {0, 0}}},
+};
+// Third neighbours (doubled x), in the order generation draws them.
+static const int kThird[12][3] = {{-2, -1, 0}, {-2, 1, 0}, {-2, 0, -1}, {-2, 0, 1}, {2, -1, 0}, {2, 1, 0},
+                                  {2, 0, -1},  {2, 0, 1},  {0, -1, -1}, {0, -1, 1}, {0, 1, -1}, {0, 1, 1}};
+
+static int trans(int o, int tag, bool new_higher, bool rotate) {
+  if (o == 0) return 0;
+  const int h = o < 0 ? -o - 1 : o - 1;  // orientation::tranHash
+  return kTrans[h][tag][new_higher ? 0 : 1][rotate ? 0 : 1];
+}
+
+enum Kind : int { K_VAC = 0, K_ITL = 1, K_GEN = 2, K_NONE = -1 };
+
+struct MixParams {
+  int64_t box[3];
+  double nu, T, sector_T, physics_time, ef110, gen_rate;
+  int64_t seed_time_inc, seed_rec;
+};
+
+struct Rec {
+  int64_t v, d;
+  int tv, td;
+  double de;
+};
+
+class RefMix {
+ public:
+  RefMix(const MixParams& p, const uint16_t* types, int64_t n_itl, const int64_t* sites, const int8_t* ori)
+      : P(p), t(types, types + 2 * p.box[0] * p.box[1] * p.box[2]) {
+    L = Typed{2 * p.box[0], p.box[1], p.box[2], t.data()};
+    for (int64_t k = 0; k < n_itl; k++) itl[sites[k]] = Itl{sites[k], ori[k], 0, {0}};
+    for (int64_t s = 0; s < (int64_t)t.size(); s++)
+      if (t[s] == refcpu::V) vac[s] = {};
+    // SubLattice seeds its time stream with the (double -> int64) config seed
+    // (simulation.h:70-76, sublattice.cpp:17).
+    time_inc_rng.seed((std::mt19937::result_type)(int64_t)(double)p.seed_time_inc);
+    rec_rng.seed((std::mt19937::result_type)p.seed_rec);
+  }
+
+  // Real-site sector region (cells; x not doubled): low half floor(n/2).
+  void region(int s, int64_t lo[3], int64_t hi[3]) const {
+    for (int d = 0; d < 3; d++) {
+      const int64_t half = P.box[d] / 2;
+      const bool high = (s >> d) & 1;
+      lo[d] = high ? half : 0;
+      hi[d] = high ? P.box[d] : half;
+    }
+  }
+
+  void refresh_masks() {
+    for (auto& kv : itl) {
+      int nb[8];
+      for (int b = 0; b < 8; b++) nb[b] = t[L.nn1(kv.first, b)];
+      kv.second.avail = refitl::itl_avail(kv.second.o, refitl::kStatus1, nb);
+    }
+  }
+  std::vector<Itl> list() const {
+    std::vector<Itl> v;
+    v.reserve(itl.size());
+    for (auto& kv : itl) v.push_back(kv.second);
+    return v;
+  }
+
+  // VacRatesSolver::deltaE / RatesSolver::rate on the typed lattice.
+  double vac_rate(int64_t v, int b) {
+    const int64_t tb = L.nn1(v, b);
+    const int X = t[tb];
+    if (!refitl::is_atom(X)) return 0.0;
+    const double e_before = refitl::count(L, v, refcpu::V) + refitl::count(L, tb, X);
+    t[tb] = refcpu::V;
+    t[v] = (uint16_t)X;
+    const double e_after = refitl::count(L, v, X) + refitl::count(L, tb, refcpu::V);
+    t[tb] = (uint16_t)X;
+    t[v] = refcpu::V;
+    const double ea = refcpu::e0(X) + (e_after - e_before) / 2;
+    return P.nu * exp(-ea / (refcpu::kBoltzmann * P.T));
+  }
+  // Itl::updateRates of one dumbbell (masks already current).
+  void itl_rates(Itl& d, const std::vector<Itl>& lst) {
+    for (double& r : d.rates) r = 0;
+    const uint8_t dirs = refitl::avail_all(d.o);
+    for (int b = 0; b < 8; b++) {
+      if (!((d.avail >> b) & 1)) continue;
+      const int64_t dst = L.nn1(d.site, b);
+      const int ghost = refitl::tran_atom(d.o, t[d.site], b);
+      const double ea = refitl::e0_itl(ghost) + refitl::delta_e(L, lst, d.site, dst, ghost) / 2;
+      const double rate = P.nu * exp(-ea / (refcpu::kBoltzmann * P.T));
+      d.rates[refitl::rates_index(b, dirs, false)] = rate / 2;
+      d.rates[refitl::rates_index(b, dirs, true)] = rate / 2;
+    }
+  }
+
+  template <class F>
+  void sweep(int s, F f) {
+    int64_t lo[3], hi[3];
+    region(s, lo, hi);
+    const int64_t X = L.X, Y = L.Y;
+    for (int64_t z = lo[2]; z < hi[2]; z++)
+      for (int64_t y = lo[1]; y < hi[1]; y++)
+        for (int64_t x = 2 * lo[0]; x < 2 * hi[0]; x++)
+          if (!f(x + X * (y + Y * z))) return;
+  }
+
+  double calcRates(int s) {
+    refresh_masks();
+    const std::vector<Itl> lst = list();
+    double sum = 0;
+    sweep(s, [&](int64_t site) {
+      auto iv = vac.find(site);
+      if (iv != vac.end()) {
+        for (int b = 0; b < 8; b++) iv->second[b] = vac_rate(site, b);
+        for (int b = 0; b < 8; b++) sum += iv->second[b];
+      } else {
+        auto ii = itl.find(site);
+        if (ii != itl.end()) {
+          itl_rates(ii->second, lst);
+          for (int b = 0; b < 8; b++) sum += ii->second.rates[b];
+        }
+      }
+      return true;
+    });
+    sum += P.gen_rate;
+    return sum;
+  }
+
+  // First slot of the sweep whose running sum exceeds target.
+  int select(int s, double target, int64_t* from, int* slot) {
+    double acc = 0;
+    int kind = K_GEN;
+    sweep(s, [&](int64_t site) {
+      const double* r = nullptr;
+      int k = K_NONE;
+      auto iv = vac.find(site);
+      if (iv != vac.end()) { r = iv->second.data(); k = K_VAC; }
+      else {
+        auto ii = itl.find(site);
+        if (ii != itl.end()) { r = ii->second.rates; k = K_ITL; }
+      }
+      if (!r) return true;
+      for (int b = 0; b < 8; b++) {
+        acc += r[b];
+        if (acc > target) { *from = site; *slot = b; kind = k; return false; }
+      }
+      return true;
+    });
+    return kind;
+  }
+
+  // rec::RecList::create + pickMinimum + Rec::recombine around site c.
+  void recombine(int64_t c) {
+    const bool is_vac = t[c] == refcpu::V;
+    if (!is_vac && !refitl::is_dumbbell(t[c])) return;
+    std::vector<std::pair<int64_t, int64_t>> pairs;
+    for (int b = 0; b < 8; b++) {
+      const int64_t n = L.nn1(c, b);
+      if (is_vac ? refitl::is_dumbbell(t[n]) : t[n] == refcpu::V) pairs.push_back(is_vac ? std::make_pair(c, n) : std::make_pair(n, c));
+    }
+    for (int k = 0; k < 6; k++) {
+      if (!((refitl::kStatus2 >> k) & 1)) continue;
+      const int64_t n = L.nn2(c, k);
+      if (is_vac ? refitl::is_dumbbell(t[n]) : t[n] == refcpu::V) pairs.push_back(is_vac ? std::make_pair(c, n) : std::make_pair(n, c));
+    }
+    if (pairs.empty()) return;
+    refresh_masks();
+    const std::vector<Itl> lst = list();
+    std::vector<Rec> recs;
+    for (auto& pr : pairs) {
+      const int64_t v = pr.first, d = pr.second;
+      const int td = t[d], hi = refitl::high_end(td), lo = refitl::low_end(td);
+      const double e_before = refitl::count(L, v, refcpu::V) + refitl::count(L, d, hi) + refitl::edumb(L, lst);
+      t[v] = (uint16_t)hi;
+      t[d] = (uint16_t)lo;
+      const double e_after = refitl::count(L, v, hi) + refitl::count(L, d, lo) + refitl::edumb(L, lst);
+      t[v] = refcpu::V;
+      t[d] = (uint16_t)td;
+      recs.push_back(Rec{v, d, hi, lo, (-1 * P.ef110) + e_after - e_before});
+    }
+    std::stable_sort(recs.begin(), recs.end(), [](const Rec& a, const Rec& b) { return a.de < b.de; });
+    std::vector<Rec> cand;
+    for (const Rec& r : recs)
+      if (std::trunc(r.de - recs[0].de) == 0.0) cand.push_back(r);  // int abs(): |d| < 1
+    const Rec& r = cand.size() == 1 ? cand[0] : cand[rec_rng() % cand.size()];
+    t[r.v] = (uint16_t)r.tv;
+    t[r.d] = (uint16_t)r.td;
+    vac.erase(r.v);
+    itl.erase(r.d);
+    n_recombined++;
+  }
+
+  // Applies the selected event; for a generation `from` receives the drawn
+  // site.  *to_out: the moved defect's new site (-1: nothing moved).
+  void perform(int kind, int s, int64_t& from, int slot, int64_t* to_out) {
+    int64_t to = -1;
+    if (kind == K_VAC) {
+      to = L.nn1(from, slot);
+      t[from] = t[to];
+      t[to] = refcpu::V;
+      vac.erase(from);
+      vac[to] = {};
+    } else if (kind == K_ITL) {
+      const Itl d = itl.at(from);
+      const uint8_t dirs = refitl::avail_all(d.o);
+      const int tag = refitl::first_nn_by_rates_index(slot, dirs);
+      to = L.nn1(from, tag);
+      const int g = refitl::tran_atom(d.o, t[from], tag);
+      t[from] = (uint16_t)refitl::diff(t[from], g);
+      t[to] = (uint16_t)refitl::combine_to_inter(t[to], g);
+      const bool new_higher = refitl::high_end(t[to]) == g;
+      itl.erase(from);
+      itl[to] = Itl{to, trans(d.o, tag, new_higher, (slot % 2) != 0), 0, {0}};
+    } else {  // K_GEN
+      std::vector<int64_t> sec;
+      sweep(s, [&](int64_t site) { sec.push_back(site); return true; });
+      const int64_t a = sec[rec_rng() % sec.size()];
+      const int b = (int)(rec_rng() % 12u);
+      const int64_t c = L.wrap_site(a, kThird[b][0], kThird[b][1], kThird[b][2]);
+      if (refitl::is_atom(t[a]) && refitl::is_atom(t[c])) {
+        const int ta = t[a];
+        t[a] = refcpu::V;
+        t[c] = (uint16_t)refitl::combine_to_inter(t[c], ta);
+        vac[a] = {};
+        itl[c] = Itl{c, refitl::orientation_from_draw(rec_rng()), 0, {0}};
+        to = a;
+      }
+      from = a;
+    }
+    *to_out = to;
+    if (to >= 0) recombine(to);
+  }
+
+  // SubLattice::startTimeLoop (sublattice.inl:61-143), as RefCpu::run.
+  int run(int64_t max_events, std::vector<int64_t>* lf, std::vector<int64_t>* lt, std::vector<int>* lk) {
+    static const int ring[8] = {0, 7, 2, 5, 3, 4, 1, 6};
+    const int64_t time_steps = (int64_t)ceil(P.physics_time / P.sector_T);
+    for (; step < time_steps; step++) {
+      for (; sect < 8; sect++) {
+        const int s = ring[sect];
+        if (!in_sector) {
+          thr = static_cast<double>(step + 1) * P.sector_T - evolution[s];
+          sector_time = 0.0;
+          in_sector = true;
+        }
+        while (sector_time < thr) {
+          if (events >= max_events) return 1;
+          const double total = calcRates(s);
+          if (total == 0.0 || std::abs(total) < std::numeric_limits<double>::epsilon()) {
+            sector_time = thr;
+          } else {
+            const double u = sel_rng() * (1.0 / std::mt19937::max());
+            int64_t from = -1, to = -1;
+            int slot = -1;
+            const int kind = select(s, u * total, &from, &slot);
+            if (kind == K_GEN && !(P.gen_rate > 0)) return -2;  // DefectGen with no generation rate
+            perform(kind, s, from, slot, &to);
+            if (lf) { lf->push_back(from); lt->push_back(to); lk->push_back(kind); }
+            const double rnd = time_inc_dis(time_inc_rng);
+            sector_time += -std::log(rnd) / total;
+            events++;
+          }
+        }
+        evolution[s] += sector_time;
+        in_sector = false;
+      }
+      sect = 0;
+    }
+    return 0;
+  }
+
+  MixParams P;
+  std::vector<uint16_t> t;
+  Typed L;
+  std::map<int64_t, Itl> itl;
+  std::map<int64_t, std::array<double, 8>> vac;
+  std::mt19937 sel_rng;       // r::rng: default-seeded (rand_generators.cpp:8)
+  std::mt19937 time_inc_rng;  // SubLattice's time stream
+  std::mt19937 rec_rng;       // stand-in for std::rand() (recombination) and generation draws
+  std::uniform_real_distribution<double> time_inc_dis{0, 1};
+  double evolution[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int64_t step = 0, events = 0, n_recombined = 0;
+  int sect = 0;
+  bool in_sector = false;
+  double thr = 0, sector_time = 0;
+};
+
+}  // namespace refmix
+
+extern "C" {
+
+// ---- refmix (see the namespace comment)
+void* refmix_create(const int64_t box[3], double nu, double T, double sector_T, double physics_time, double ef110,
+                    double gen_rate, int64_t seed_time_inc, int64_t seed_rec, const uint16_t* types, int64_t n_itl,
+                    const int64_t* sites, const int8_t* ori) {
+  refmix::MixParams p{{box[0], box[1], box[2]}, nu, T, sector_T, physics_time, ef110, gen_rate, seed_time_inc,
+                      seed_rec};
+  return new refmix::RefMix(p, types, n_itl, sites, ori);
+}
+void refmix_destroy(void* h) { delete (refmix::RefMix*)h; }
+// Runs up to max_events more events; logs (from, to, kind) per event when the
+// arrays are given (kind 0 vacancy hop, 1 dumbbell transition, 2 defect
+// generation; to = -1 when a generation changed nothing).
+int refmix_run(void* h, int64_t max_events, int64_t* lf, int64_t* lt, int32_t* lk, int64_t* n_logged) {
+  auto* r = (refmix::RefMix*)h;
+  std::vector<int64_t> f, t;
+  std::vector<int> k;
+  const bool log = lf != nullptr;
+  const int rc = r->run(r->events + max_events, log ? &f : nullptr, log ? &t : nullptr, log ? &k : nullptr);
+  if (log) {
+    for (size_t i = 0; i < f.size(); i++) { lf[i] = f[i]; lt[i] = t[i]; lk[i] = k[i]; }
+    *n_logged = (int64_t)f.size();
+  }
+  return rc;
+}
+void refmix_state(void* h, double evolution[8], int64_t* ints, double* dbls) {
+  auto* r = (refmix::RefMix*)h;
+  for (int i = 0; i < 8; i++) evolution[i] = r->evolution[i];
+  ints[0] = r->step; ints[1] = r->sect; ints[2] = r->in_sector ? 1 : 0; ints[3] = r->events;
+  ints[4] = (int64_t)r->itl.size(); ints[5] = (int64_t)r->vac.size(); ints[6] = r->n_recombined;
+  dbls[0] = r->thr; dbls[1] = r->sector_time;
+}
+void refmix_types(void* h, uint16_t* out) {
+  auto* r = (refmix::RefMix*)h;
+  memcpy(out, r->t.data(), r->t.size() * sizeof(uint16_t));
+}
+// Dumbbell list in site order: sites and orientations (cap entries).
+int64_t refmix_itl(void* h, int64_t cap, int64_t* sites, int8_t* ori) {
+  auto* r = (refmix::RefMix*)h;
+  int64_t n = 0;
+  for (auto& kv : r->itl) {
+    if (n < cap) { sites[n] = kv.first; ori[n] = (int8_t)kv.second.o; }
+    n++;
+  }
+  return n;
+}
+// calcRates of sector s: the defects in sweep order (site, kind) and their 8
+// slots; returns the count and writes the total (with defect_gen_rate).
+int64_t refmix_sector_rates(void* h, int s, int64_t cap, int64_t* sites, int32_t* kinds, double* rates,
+                            double* total) {
+  auto* r = (refmix::RefMix*)h;
+  *total = r->calcRates(s);
+  int64_t n = 0;
+  r->sweep(s, [&](int64_t site) {
+    const double* q = nullptr;
+    int k = -1;
+    auto iv = r->vac.find(site);
+    if (iv != r->vac.end()) { q = iv->second.data(); k = 0; }
+    else {
+      auto ii = r->itl.find(site);
+      if (ii != r->itl.end()) { q = ii->second.rates; k = 1; }
+    }
+    if (q) {
+      if (n < cap) {
+        sites[n] = site;
+        kinds[n] = k;
+        for (int b = 0; b < 8; b++) rates[8 * n + b] = q[b];
+      }
+      n++;
+    }
+    return true;
+  });
+  return n;
+}
+int refmix_trans(int o, int tag, int new_higher, int rotate) { return refmix::trans(o, tag, new_higher != 0, rotate != 0); }
+
+// placeDumbbells of the reference's serial driver (profile_serial/main.cpp:
+// 137-161) on a typed lattice of n sites: for each kind, counts[k] draws
+// rand32(0, n) = mt() % n (random.cpp:27-32); a single atom there becomes
+// combineToInter(atom, types[k]) with orientation::random(), drawn next.
+// Writes the placed sites and orientations in placement order; returns the
+// count.
+int64_t refmix_place(uint16_t* t, int64_t n, const int32_t* kinds, const int64_t* counts, int32_t nk, int64_t seed,
+                     int64_t* sites, int8_t* ori) {
+  std::mt19937 rng((std::mt19937::result_type)seed);
+  int64_t m = 0;
+  for (int32_t k = 0; k < nk; k++)
+    for (int64_t c = 0; c < counts[k]; c++) {
+      const uint32_t id = (uint32_t)(rng() % (uint32_t)n);
+      if (refitl::is_atom(t[id])) {
+        t[id] = (uint16_t)refitl::combine_to_inter(t[id], kinds[k]);
+        sites[m] = id;
+        ori[m] = (int8_t)refitl::orientation_from_draw((uint32_t)rng());
+        m++;
+      }
+    }
+  return m;
+}
+
+}  // extern "C"
 
 extern "C" {
 

@@ -368,3 +368,106 @@ class Itl:
         self.L.ref_itl_rates(b.ctypes.data, t.ctypes.data, s.size, s.ctypes.data, o.ctypes.data,
                              float(nu), float(T), av.ctypes.data, r.ctypes.data)
         return av, r
+
+
+class RefMix:
+    """Vacancies and dumbbells on a typed lattice: the abvi model's event loop
+    as oracle/refcpu.cpp's refmix restates it (see the comment there)."""
+
+    def __init__(self, box, types, sites, orient, nu=6e12, T=573.0, sector_T=3e-7, physics_time=1.0, ef110=3.53,
+                 gen_rate=0.0, seed_time_inc=594689, seed_rec=563264):
+        L = lib()
+        if not hasattr(L, "_mix_bound"):
+            vp, i64, dbl = ctypes.c_void_p, ctypes.c_int64, ctypes.c_double
+            L.refmix_create.restype = vp
+            L.refmix_create.argtypes = [vp, dbl, dbl, dbl, dbl, dbl, dbl, i64, i64, vp, i64, vp, vp]
+            L.refmix_destroy.argtypes = [vp]
+            L.refmix_run.restype = ctypes.c_int
+            L.refmix_run.argtypes = [vp, i64, vp, vp, vp, vp]
+            L.refmix_state.argtypes = [vp, vp, vp, vp]
+            L.refmix_types.argtypes = [vp, vp]
+            L.refmix_itl.restype = i64
+            L.refmix_itl.argtypes = [vp, i64, vp, vp]
+            L.refmix_sector_rates.restype = i64
+            L.refmix_sector_rates.argtypes = [vp, ctypes.c_int, i64, vp, vp, vp, vp]
+            L.refmix_trans.restype = ctypes.c_int
+            L.refmix_trans.argtypes = [ctypes.c_int] * 4
+            L._mix_bound = True
+        self.L = L
+        self.box = tuple(int(v) for v in box)
+        self.n_sites = 2 * self.box[0] * self.box[1] * self.box[2]
+        b = np.array(self.box, dtype=np.int64)
+        t = np.ascontiguousarray(types, dtype=np.uint16)
+        s = np.ascontiguousarray(sites, dtype=np.int64)
+        o = np.ascontiguousarray(orient, dtype=np.int8)
+        assert t.size == self.n_sites and s.size == o.size
+        self.h = L.refmix_create(b.ctypes.data, nu, T, sector_T, physics_time, ef110, gen_rate, int(seed_time_inc),
+                                 int(seed_rec), t.ctypes.data, s.size, s.ctypes.data, o.ctypes.data)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.refmix_destroy(self.h)
+            self.h = None
+
+    def run(self, n, log=False):
+        if not log:
+            return self.L.refmix_run(self.h, int(n), None, None, None, None), None
+        f = np.zeros(n, np.int64)
+        t = np.zeros(n, np.int64)
+        k = np.zeros(n, np.int32)
+        m = ctypes.c_int64()
+        rc = self.L.refmix_run(self.h, int(n), f.ctypes.data, t.ctypes.data, k.ctypes.data, ctypes.byref(m))
+        return rc, (f[:m.value], t[:m.value], k[:m.value])
+
+    def state(self):
+        evo = np.zeros(8)
+        ints = np.zeros(7, np.int64)
+        dbls = np.zeros(2)
+        self.L.refmix_state(self.h, evo.ctypes.data, ints.ctypes.data, dbls.ctypes.data)
+        return {"evolution": evo, "step": int(ints[0]), "sect": int(ints[1]), "in_sector": int(ints[2]),
+                "events": int(ints[3]), "n_itl": int(ints[4]), "n_vac": int(ints[5]), "recombined": int(ints[6]),
+                "thr": float(dbls[0]), "sector_time": float(dbls[1])}
+
+    def types(self):
+        out = np.zeros(self.n_sites, np.uint16)
+        self.L.refmix_types(self.h, out.ctypes.data)
+        return out
+
+    def itl(self):
+        n = self.L.refmix_itl(self.h, 0, None, None)
+        s = np.zeros(max(n, 1), np.int64)
+        o = np.zeros(max(n, 1), np.int8)
+        self.L.refmix_itl(self.h, n, s.ctypes.data, o.ctypes.data)
+        return s[:n], o[:n]
+
+    def sector_rates(self, sector, cap=1 << 16):
+        s = np.zeros(cap, np.int64)
+        k = np.zeros(cap, np.int32)
+        r = np.zeros((cap, 8))
+        tot = ctypes.c_double()
+        n = self.L.refmix_sector_rates(self.h, int(sector), cap, s.ctypes.data, k.ctypes.data, r.ctypes.data,
+                                       ctypes.byref(tot))
+        assert n <= cap
+        return s[:n], k[:n], r[:n], tot.value
+
+
+def mix_trans(o, tag, new_higher, rotate):
+    """orientation::trans as the oracle restates it."""
+    return lib().refmix_trans(int(o), int(tag), int(bool(new_higher)), int(bool(rotate)))
+
+
+def mix_place(types, kinds, counts, seed):
+    """placeDumbbells restated (oracle): the new types and the placed (site,
+    orientation) pairs in placement order."""
+    L = lib()
+    L.refmix_place.restype = ctypes.c_int64
+    L.refmix_place.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                               ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
+    t = np.array(types, dtype=np.uint16, copy=True)
+    k = np.array([LAT_TYPES[x] if isinstance(x, str) else int(x) for x in kinds], dtype=np.int32)
+    c = np.array(counts, dtype=np.int64)
+    s = np.zeros(int(c.sum()) + 1, np.int64)
+    o = np.zeros(int(c.sum()) + 1, np.int8)
+    m = L.refmix_place(t.ctypes.data, t.size, k.ctypes.data, c.ctypes.data, len(k), int(seed), s.ctypes.data,
+                       o.ctypes.data)
+    return t, s[:m], o[:m]
