@@ -485,9 +485,23 @@ int akmc_server_create(int device, const akmc_server_options* o, akmc_server** o
  * time loop).  Fails with AKMC_ERR_BUDGET if the job alone exceeds the
  * budget.  Thread-safe. */
 int akmc_server_submit(akmc_server* s, const akmc_params* p, int64_t target_events, int64_t* job);
+/* Queues a job that also carries self-interstitial dumbbells (the abvi
+ * model's vacancy + dumbbell event loop, akmc_mix_*, FP64 rates, device
+ * streams): the lattice of p drawn on the device as for akmc_server_submit
+ * (alloy, va_count vacancies), plus n_itl dumbbells at distinct atom sites,
+ * each of the site's atom and an atom of type itl_atom (AKMC_TYPE_FE for
+ * self-interstitials), with random orientations; ef110 as akmc_mix_params.
+ * p->seed_event_selection seeds the recombination stand-in draws.  The job
+ * shares the request loop, the HBM budget and the arrival order with the
+ * vacancy-only jobs; its steps are batched with the other dumbbell jobs in
+ * one launch, beside the vacancy jobs' launch. */
+int akmc_server_submit_itl(akmc_server* s, const akmc_params* p, int64_t n_itl, int32_t itl_atom, double ef110,
+                           int64_t target_events, int64_t* job);
 int akmc_server_job_info(akmc_server* s, int64_t job, akmc_job_info* info);
 /* The finished lattice of a job (keep_results). */
 int akmc_server_result(akmc_server* s, int64_t job, uint8_t* sites, int64_t n);
+/* The finished typed lattice of a dumbbell job (keep_results). */
+int akmc_server_result_typed(akmc_server* s, int64_t job, uint16_t* types, int64_t n);
 /* Waits until no job is pending or running (timeout_s <= 0: no limit). */
 int akmc_server_wait_idle(akmc_server* s, double timeout_s);
 int akmc_server_get_stats(akmc_server* s, akmc_server_stats* st);

@@ -442,6 +442,8 @@ def _bind_batch(L):
     L.akmc_server_submit.argtypes = [vp, P(Params), i64, P(i64)]
     L.akmc_server_job_info.argtypes = [vp, i64, P(JobInfo)]
     L.akmc_server_result.argtypes = [vp, i64, vp, i64]
+    L.akmc_server_submit_itl.argtypes = [vp, P(Params), i64, i32, dbl, i64, P(i64)]
+    L.akmc_server_result_typed.argtypes = [vp, i64, vp, i64]
     L.akmc_server_wait_idle.argtypes = [vp, dbl]
     L.akmc_server_get_stats.argtypes = [vp, P(ServerStats)]
     L.akmc_server_clock.argtypes = [vp]
@@ -660,6 +662,19 @@ class Server:
         job = ctypes.c_int64()
         _check(_blib().akmc_server_submit(self.h, ctypes.byref(p), int(target_events), ctypes.byref(job)))
         return job.value
+
+    def submit_itl(self, config, n_itl, target_events, itl_atom=1, ef110=3.53):
+        """A job with n_itl self-interstitial dumbbells (akmc_server_submit_itl)."""
+        p = config if isinstance(config, Params) else params_from_config(config)
+        job = ctypes.c_int64()
+        _check(_blib().akmc_server_submit_itl(self.h, ctypes.byref(p), int(n_itl), int(itl_atom), float(ef110),
+                                              int(target_events), ctypes.byref(job)))
+        return job.value
+
+    def result_typed(self, job, n_sites):
+        out = np.zeros(int(n_sites), dtype=np.uint16)
+        _check(_blib().akmc_server_result_typed(self.h, int(job), out.ctypes.data, out.size))
+        return out
 
     def info(self, job):
         i = JobInfo()

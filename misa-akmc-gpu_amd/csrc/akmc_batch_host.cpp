@@ -21,7 +21,9 @@
 #include "../../include/akmc.h"
 #include "akmc_batch.h"
 #include "akmc_host.h"
+#include "akmc_mix_pool.h"
 #include "akmc_model.h"
+#include "akmc_plain.h"
 
 using akmc_host::fail;
 
@@ -1037,6 +1039,14 @@ struct Job {
   std::vector<uint8_t> result;
   akmc::ReplicaDesc h_desc{};   // sources of the admission copies
   akmc::ReplicaState h_state{};
+  int64_t bytes = 0;            // HBM bytes while live
+  // A job with self-interstitial dumbbells runs on the vacancy + dumbbell
+  // engine (akmc_mix_pool.h): its own slot space.
+  bool mix = false;
+  int64_t n_itl = 0;
+  int32_t itl_atom = 1;
+  double ef110 = 3.53;
+  std::vector<uint16_t> result_types;
 };
 
 }  // namespace
@@ -1060,6 +1070,11 @@ struct akmc_server {
   std::atomic<bool> stop{false};
   std::thread worker;
   hipStream_t init_stream = nullptr;  // job set-up, overlapping the running slice
+  // Jobs with dumbbells: the vacancy + dumbbell pool, its stream and slots.
+  akmc_mixpool::Pool* mix = nullptr;
+  hipStream_t mix_stream = nullptr;
+  std::vector<Job*> mix_job;
+  std::vector<int32_t> mix_free;
   bool items_dirty = false;
   int error = AKMC_OK;
   std::string error_msg;
@@ -1078,16 +1093,25 @@ int admit(akmc_server* s, bool* changed) {
     Job* j = nullptr;
     {
       std::lock_guard<std::mutex> lk(s->mu);
-      if (s->pending.empty() || s->free_slots.empty()) break;
+      if (s->pending.empty()) break;
       Job* h = s->pending.front();
-      if (s->bytes_live + h->rep.bytes > s->opt.hbm_budget) break;
+      if ((h->mix ? s->mix_free : s->free_slots).empty()) break;
+      if (s->bytes_live + h->bytes > s->opt.hbm_budget) break;
       s->pending.pop_front();
       j = h;
-      j->slot = s->free_slots.back();
-      s->free_slots.pop_back();
-      s->bytes_live += j->rep.bytes;
+      std::vector<int32_t>& fl = j->mix ? s->mix_free : s->free_slots;
+      j->slot = fl.back();
+      fl.pop_back();
+      s->bytes_live += j->bytes;
       s->st.bytes_peak = std::max(s->st.bytes_peak, s->bytes_live);
       s->st.live++;  // counted from here, so that the server never looks idle while it sets the job up
+    }
+    if (j->mix) {
+      const int rc = akmc_mixpool::pool_add_generated(s->mix, j->slot, j->p, j->n_itl, j->itl_atom, j->ef110,
+                                                      s->init_stream);
+      if (rc) return rc;
+      batch.push_back(j);
+      continue;
     }
     uint8_t* mem = nullptr;
     int rc = s->arena.alloc(j->rep.bytes, &mem);
@@ -1109,6 +1133,14 @@ int admit(akmc_server* s, bool* changed) {
   const double t = s->now();
   std::lock_guard<std::mutex> lk(s->mu);
   for (Job* j : batch) {
+    if (j->mix) {
+      akmc_mixpool::pool_setup_done(s->mix, j->slot);
+      s->mix_job[j->slot] = j;
+      j->status = AKMC_JOB_RUNNING;
+      j->t_admit = t;
+      s->st.admitted++;
+      continue;
+    }
     j->rep.release_setup();
     s->cap_of_slot[j->slot] = j->rep.cap;
     s->slot_job[j->slot] = j;
@@ -1122,20 +1154,36 @@ int admit(akmc_server* s, bool* changed) {
 }
 
 int retire(akmc_server* s, Job* j, int32_t rep_status) {
-  if (s->opt.keep_results) {
-    j->result.resize(j->rep.nsites);
-    HIP_TRY(hipMemcpy(j->result.data(), j->rep.sites, j->rep.nsites, hipMemcpyDeviceToHost));
+  if (j->mix) {
+    if (s->opt.keep_results) {
+      j->result_types.resize((size_t)akmc_mixpool::pool_sites(s->mix, j->slot));
+      const int rc = akmc_mixpool::pool_download(s->mix, j->slot, j->result_types.data(),
+                                                 (int64_t)j->result_types.size());
+      if (rc) return rc;
+    }
+    const int rc = akmc_mixpool::pool_remove(s->mix, j->slot);
+    if (rc) return rc;
+  } else {
+    if (s->opt.keep_results) {
+      j->result.resize(j->rep.nsites);
+      HIP_TRY(hipMemcpy(j->result.data(), j->rep.sites, j->rep.nsites, hipMemcpyDeviceToHost));
+    }
+    s->arena.release(j->rep.mem, j->rep.bytes);
+    j->rep.mem = nullptr;
   }
-  s->arena.release(j->rep.mem, j->rep.bytes);
-  j->rep.mem = nullptr;
   std::lock_guard<std::mutex> lk(s->mu);
   j->rep_status = rep_status;
   j->status = rep_status < 0 ? AKMC_JOB_FAILED : AKMC_JOB_DONE;
   j->t_done = s->now();
-  s->slot_job[j->slot] = nullptr;
-  s->free_slots.push_back(j->slot);
+  if (j->mix) {
+    s->mix_job[j->slot] = nullptr;
+    s->mix_free.push_back(j->slot);
+  } else {
+    s->slot_job[j->slot] = nullptr;
+    s->free_slots.push_back(j->slot);
+  }
   j->slot = -1;
-  s->bytes_live -= j->rep.bytes;
+  s->bytes_live -= j->bytes;
   s->st.bytes_live = s->bytes_live;
   s->st.live--;
   if (rep_status < 0) s->st.failed++; else s->st.done++;
@@ -1147,14 +1195,26 @@ int retire(akmc_server* s, Job* j, int32_t rep_status) {
 // pending jobs while it runs (their set-up overlaps the device work), then
 // collect the slice and retire the jobs that finished.
 int serve_step(akmc_server* s) {
-  std::vector<int32_t> live;
+  std::vector<int32_t> live, mlive;
+  std::vector<int64_t> mlim;
   int32_t hi = 0;
   {
     std::lock_guard<std::mutex> lk(s->mu);
     for (int32_t k = 0; k < (int32_t)s->slot_job.size(); k++)
       if (s->slot_job[k]) { live.push_back(k); hi = k + 1; }
+    for (int32_t k = 0; k < (int32_t)s->mix_job.size(); k++)
+      if (Job* j = s->mix_job[k]) {
+        mlive.push_back(k);
+        // Each dumbbell job advances by at most one slice of events per step.
+        mlim.push_back(std::min(j->target, j->events + std::max<int64_t>(1, s->opt.slice_events)));
+      }
   }
   int rc;
+  const double t_m = s->now();
+  if (!mlive.empty()) {
+    rc = akmc_mixpool::pool_launch(s->mix, mlive, mlim, s->mix_stream);
+    if (rc) return rc;
+  }
   if (s->items_dirty) {
     rc = s->R.set_items(live, s->cap_of_slot);
     if (rc) return rc;
@@ -1170,6 +1230,31 @@ int serve_step(akmc_server* s) {
   rc = admit(s, &changed);
   if (rc) return rc;
   if (changed) s->items_dirty = true;
+  if (!mlive.empty()) {
+    HIP_TRY(hipStreamSynchronize(s->mix_stream));
+    int64_t mev = 0;
+    for (int32_t k : mlive) {
+      Job* j = s->mix_job[k];
+      const akmc::mix::MixState& st = akmc_mixpool::pool_state(s->mix, k);
+      mev += st.events - j->events;
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        j->events = st.events;
+      }
+      int32_t rs = AKMC_REP_RUNNING;
+      if (st.status == AKMC_RUN_TIME_DONE) rs = AKMC_REP_TIME_DONE;
+      else if (st.events >= j->target) rs = AKMC_REP_TARGET_DONE;
+      else if (st.status < 0) rs = st.status;
+      if (rs != AKMC_REP_RUNNING) {
+        rc = retire(s, j, rs);
+        if (rc) return rc;
+      }
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->st.launches++;
+    s->st.events += mev;
+    if (!launched) s->st.busy_seconds += s->now() - t_m;
+  }
   if (!launched) return AKMC_OK;
   HIP_TRY(hipStreamSynchronize(s->R.s0));
   const double t_b = s->now();
@@ -1254,6 +1339,11 @@ int akmc_server_create(int device, const akmc_server_options* o, akmc_server** o
   if (rc) return rc;
   s->R.use_graph = false;  // the live set changes between launches
   HIP_TRY(hipStreamCreateWithFlags(&s->init_stream, hipStreamNonBlocking));
+  HIP_TRY(hipStreamCreateWithFlags(&s->mix_stream, hipStreamNonBlocking));
+  rc = akmc_mixpool::pool_create(device, opt.max_live, &s->mix);
+  if (rc) return rc;
+  s->mix_job.assign(opt.max_live, nullptr);
+  for (int32_t k = opt.max_live - 1; k >= 0; k--) s->mix_free.push_back(k);
   s->R.tun.vcap = opt.lds_vcap > 0 ? opt.lds_vcap : (opt.lds_vcap < 0 ? -1 : 256);
   s->R.tun.waves = opt.waves_per_group > 0 ? opt.waves_per_group : 2;
   s->R.tun.order = 1;
@@ -1280,8 +1370,9 @@ int akmc_server_submit(akmc_server* s, const akmc_params* p, int64_t target_even
   j->target = target_events;
   int rc = setup_rep(&j->rep, *p, s->R.fp32);
   if (rc) return rc;
-  if (j->rep.bytes > s->opt.hbm_budget)
-    return fail(AKMC_ERR_BUDGET, "job needs %lld bytes, budget %lld", (long long)j->rep.bytes,
+  j->bytes = j->rep.bytes;
+  if (j->bytes > s->opt.hbm_budget)
+    return fail(AKMC_ERR_BUDGET, "job needs %lld bytes, budget %lld", (long long)j->bytes,
                 (long long)s->opt.hbm_budget);
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->error) return fail(s->error, "server stopped: %s", s->error_msg.c_str());
@@ -1295,6 +1386,50 @@ int akmc_server_submit(akmc_server* s, const akmc_params* p, int64_t target_even
   return AKMC_OK;
 }
 
+int akmc_server_submit_itl(akmc_server* s, const akmc_params* p, int64_t n_itl, int32_t itl_atom, double ef110,
+                            int64_t target_events, int64_t* job) {
+  if (!s || !p || target_events < 0 || n_itl < 0) return fail(AKMC_ERR_INVALID, "bad arguments");
+  if (!(itl_atom == AKMC_TYPE_FE || itl_atom == AKMC_TYPE_CU || itl_atom == AKMC_TYPE_NI || itl_atom == AKMC_TYPE_MN))
+    return fail(AKMC_ERR_INVALID, "dumbbell atom 0x%x", itl_atom);
+  for (int d = 0; d < 3; d++)
+    if (p->box[d] < 8) return fail(AKMC_ERR_INVALID, "box edge %lld < 8 cells", (long long)p->box[d]);
+  if (2 * p->box[0] > 2048 || p->box[1] > 2047 || p->box[2] > 1023)
+    return fail(AKMC_ERR_INVALID, "box exceeds 1024 x 2047 x 1023 cells");
+  std::unique_ptr<Job> j(new Job());
+  j->p = *p;
+  j->target = target_events;
+  j->mix = true;
+  j->n_itl = n_itl;
+  j->itl_atom = itl_atom;
+  j->ef110 = ef110;
+  j->bytes = akmc_mixpool::pool_bytes(*p, n_itl);
+  if (j->bytes > s->opt.hbm_budget)
+    return fail(AKMC_ERR_BUDGET, "job needs %lld bytes, budget %lld", (long long)j->bytes,
+                (long long)s->opt.hbm_budget);
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->error) return fail(s->error, "server stopped: %s", s->error_msg.c_str());
+  j->id = s->next_id++;
+  j->t_submit = s->now();
+  if (job) *job = j->id;
+  s->pending.push_back(j.get());
+  s->jobs[j->id] = std::move(j);
+  s->st.submitted++;
+  s->cv.notify_all();
+  return AKMC_OK;
+}
+
+int akmc_server_result_typed(akmc_server* s, int64_t id, uint16_t* types, int64_t n) {
+  if (!s || !types) return fail(AKMC_ERR_INVALID, "bad arguments");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->jobs.find(id);
+  if (it == s->jobs.end()) return fail(AKMC_ERR_INVALID, "no job %lld", (long long)id);
+  const Job& j = *it->second;
+  if (j.status != AKMC_JOB_DONE) return fail(AKMC_ERR_INVALID, "job %lld has not finished", (long long)id);
+  if ((int64_t)j.result_types.size() != n) return fail(AKMC_ERR_INVALID, "no kept typed result of %lld sites", (long long)n);
+  memcpy(types, j.result_types.data(), (size_t)n * 2);
+  return AKMC_OK;
+}
+
 int akmc_server_job_info(akmc_server* s, int64_t id, akmc_job_info* info) {
   if (!s || !info) return fail(AKMC_ERR_INVALID, "bad arguments");
   std::lock_guard<std::mutex> lk(s->mu);
@@ -1305,7 +1440,7 @@ int akmc_server_job_info(akmc_server* s, int64_t id, akmc_job_info* info) {
   info->status = j.status;
   info->rep_status = j.rep_status;
   info->events = j.events;
-  info->bytes = j.rep.bytes;
+  info->bytes = j.bytes;
   info->t_submit = j.t_submit;
   info->t_admit = j.t_admit;
   info->t_done = j.t_done;
@@ -1353,6 +1488,8 @@ void akmc_server_destroy(akmc_server* s) {
   (void)hipSetDevice(s->R.device);
   (void)hipDeviceSynchronize();
   s->arena.destroy();
+  akmc_mixpool::pool_destroy(s->mix);
+  if (s->mix_stream) (void)hipStreamDestroy(s->mix_stream);
   if (s->init_stream) (void)hipStreamDestroy(s->init_stream);
   s->R.destroy();
   delete s;

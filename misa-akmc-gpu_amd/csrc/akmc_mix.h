@@ -72,3 +72,7 @@ struct MixDesc {
 }  // namespace akmc
 
 extern "C" int akmc_mix_launch(const akmc::mix::MixDesc* d_descs, int32_t n, hipStream_t stream);
+extern "C" int akmc_mix_fill_launch(uint16_t* types, int64_t n, uint64_t seed, const uint32_t* cum,
+                                    const uint16_t* tval, int32_t nt, hipStream_t stream);
+extern "C" int akmc_mix_defects_launch(uint16_t* types, const int64_t* vidx, int64_t nv, const int64_t* iidx,
+                                       int64_t ni, int32_t atom, hipStream_t stream);

@@ -990,3 +990,59 @@ extern "C" int akmc_mix_launch(const akmc::mix::MixDesc* d_descs, int32_t n, hip
   hipLaunchKernelGGL(akmc::mix::mix_kernel, dim3((unsigned)n), dim3(64), 0, stream, d_descs);
   return (int)hipGetLastError();
 }
+
+namespace akmc {
+namespace mix {
+
+// Initial lattice of a generated job: site i takes the type whose cumulative
+// ratio bracket holds splitmix64(seed ^ K (i + 1)) mod total (the replica
+// engine's composition draw, akmc_batch.hip init_sites_kernel).
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+__global__ void fill_kernel(uint16_t* types, int64_t n, uint64_t seed, const uint32_t* cum, const uint16_t* tval,
+                            int32_t nt) {
+  const uint32_t total = cum[nt - 1];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t h = splitmix64(seed ^ (0xD1B54A32D192ED03ull * (uint64_t)(i + 1)));
+    const uint32_t x = (uint32_t)(((h >> 32) * (uint64_t)total) >> 32);  // [0, total)
+    uint16_t v = (uint16_t)T_FE;
+    for (int t = nt - 1; t >= 0; t--)
+      if (x < cum[t]) v = tval[t];
+    types[i] = v;
+  }
+}
+// Vacancies at vidx, dumbbells (the site's atom and `atom`) at iidx.
+__global__ void defects_kernel(uint16_t* types, const int64_t* vidx, int64_t nv, const int64_t* iidx, int64_t ni,
+                               int32_t atom) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nv) types[vidx[i]] = (uint16_t)T_V;
+  if (i < ni) {
+    const int64_t s = iidx[i];
+    types[s] = (uint16_t)combine(types[s], atom);
+  }
+}
+
+}  // namespace mix
+}  // namespace akmc
+
+extern "C" int akmc_mix_fill_launch(uint16_t* types, int64_t n, uint64_t seed, const uint32_t* cum,
+                                    const uint16_t* tval, int32_t nt, hipStream_t stream) {
+  if (n <= 0) return 0;
+  const int64_t blocks = n / 256 + 1 < 8192 ? n / 256 + 1 : 8192;
+  hipLaunchKernelGGL(akmc::mix::fill_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, types, n, seed, cum, tval,
+                     nt);
+  return (int)hipGetLastError();
+}
+
+extern "C" int akmc_mix_defects_launch(uint16_t* types, const int64_t* vidx, int64_t nv, const int64_t* iidx,
+                                       int64_t ni, int32_t atom, hipStream_t stream) {
+  const int64_t m = nv > ni ? nv : ni;
+  if (m <= 0) return 0;
+  hipLaunchKernelGGL(akmc::mix::defects_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, stream, types, vidx,
+                     nv, iidx, ni, atom);
+  return (int)hipGetLastError();
+}

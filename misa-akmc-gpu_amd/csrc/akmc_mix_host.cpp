@@ -8,11 +8,13 @@
 #include <cstring>
 #include <memory>
 #include <random>
+#include <unordered_set>
 #include <vector>
 
 #include "../../include/akmc.h"
 #include "akmc_host.h"
 #include "akmc_mix.h"
+#include "akmc_mix_pool.h"
 #include "akmc_plain.h"
 
 using akmc_host::fail;
@@ -64,7 +66,30 @@ struct MixLat {
   int64_t target = 0;
   bool done = false;
   int32_t last_status = 0;
+  // Device set-up of a generated lattice (freed by pool_setup_done).
+  uint32_t* d_cum = nullptr;
+  uint16_t* d_tval = nullptr;
+  int64_t* d_idx = nullptr;
+  std::vector<int64_t> h_idx;
+  uint32_t h_cum[AKMC_MAX_ALLOY] = {};
+  uint16_t h_tval[AKMC_MAX_ALLOY] = {};
+  std::vector<uint32_t> h_vkey, h_ikey;
+  std::vector<int8_t> h_io;
+  akmc::mix::MixState h_init{};  // source of the initial state's upload
+  int64_t bytes = 0;
 };
+
+void free_lat(MixLat& L) {
+  void* ptrs[] = {L.types, L.vkey, L.ikey, L.io, L.iav, L.iterm, L.ipre, L.rkey, L.rkind, L.rslot, L.rate,
+                  L.pvac, L.tmask, L.risk, L.d_u, L.d_l, L.d_rec, L.d_log, L.d_cum, L.d_tval, L.d_idx};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+}
+
+// Device bytes of a lattice: types, lists and sector records.
+int64_t lat_bytes(int64_t nsites, int64_t vcap, int64_t icap) {
+  return 2 * nsites + 4 * vcap + 22 * icap + 90 * (vcap + icap);
+}
 
 template <class T>
 hipError_t dalloc(T** p, int64_t n) {
@@ -92,12 +117,7 @@ void release(akmc_mix* m) {
   if (!m) return;
   (void)hipSetDevice(m->device);
   if (m->stream) (void)hipStreamSynchronize(m->stream);
-  for (auto& L : m->lat) {
-    void* ptrs[] = {L->types, L->vkey, L->ikey, L->io, L->iav, L->iterm, L->ipre, L->rkey, L->rkind, L->rslot,
-                    L->rate, L->pvac, L->tmask, L->risk, L->d_u, L->d_l, L->d_rec, L->d_log};
-    for (void* q : ptrs)
-      if (q) (void)hipFree(q);
-  }
+  for (auto& L : m->lat) free_lat(*L);
   if (m->d_state) (void)hipFree(m->d_state);
   if (m->d_desc) (void)hipFree(m->d_desc);
   if (m->ev0) (void)hipEventDestroy(m->ev0);
@@ -106,8 +126,7 @@ void release(akmc_mix* m) {
   delete m;
 }
 
-akmc::mix::MixDesc make_desc(akmc_mix* m, int i) {
-  MixLat& L = *m->lat[(size_t)i];
+akmc::mix::MixDesc desc_of(MixLat& L, akmc::mix::MixState* d_st) {
   akmc::mix::MixDesc D{};
   D.types = L.types;
   D.vkey = L.vkey;
@@ -126,7 +145,7 @@ akmc::mix::MixDesc make_desc(akmc_mix* m, int i) {
   D.pvac = L.pvac;
   D.tmask = L.tmask;
   D.risk = L.risk;
-  D.st = m->d_state + i;
+  D.st = d_st;
   const bool host = L.p.device_rng == 0;
   D.u = host ? L.d_u : nullptr;
   D.l = host ? L.d_l : nullptr;
@@ -148,6 +167,57 @@ akmc::mix::MixDesc make_desc(akmc_mix* m, int i) {
   D.gen_rate = L.p.defect_gen_rate;
   D.rates_sector = -1;
   return D;
+}
+
+akmc::mix::MixDesc make_desc(akmc_mix* m, int i) { return desc_of(*m->lat[(size_t)i], m->d_state + i); }
+
+// Device buffers of a lattice (lists of the given capacities, host streams
+// when the lattice draws from the host).
+int alloc_lat(MixLat& L) {
+  HIP_TRY(dalloc(&L.types, L.nsites));
+  HIP_TRY(dalloc(&L.vkey, L.vcap));
+  HIP_TRY(dalloc(&L.ikey, L.icap));
+  HIP_TRY(dalloc(&L.io, L.icap));
+  HIP_TRY(dalloc(&L.iav, L.icap));
+  HIP_TRY(dalloc(&L.iterm, L.icap));
+  HIP_TRY(dalloc(&L.ipre, L.icap));
+  HIP_TRY(dalloc(&L.rkey, L.rcap));
+  HIP_TRY(dalloc(&L.rkind, L.rcap));
+  HIP_TRY(dalloc(&L.rslot, L.rcap));
+  HIP_TRY(dalloc(&L.rate, 8 * (int64_t)L.rcap));
+  HIP_TRY(dalloc(&L.pvac, L.rcap));
+  HIP_TRY(dalloc(&L.tmask, L.rcap));
+  HIP_TRY(dalloc(&L.risk, L.rcap));
+  if (L.p.device_rng == 0) {
+    HIP_TRY(dalloc(&L.d_u, kChunk));
+    HIP_TRY(dalloc(&L.d_l, kChunk));
+    HIP_TRY(dalloc(&L.d_rec, kRecWindow));
+    L.h_u.resize(kChunk);
+    L.h_l.resize(kChunk);
+    // r::rng is default-constructed (rand_generators.cpp:8); SubLattice
+    // seeds its time stream with the (double -> int64) config seed
+    // (simulation.h:70-76, sublattice.cpp:17).
+    L.time_rng.seed((std::mt19937::result_type)(int64_t)(double)L.p.seed_time_inc);
+    L.rec.seed((std::mt19937::result_type)L.p.seed_recombine);
+  }
+  L.bytes = lat_bytes(L.nsites, L.vcap, L.icap);
+  return AKMC_OK;
+}
+
+// Initial state: defect counts and the device streams' xoshiro256** state
+// (splitmix64 of the two seeds).
+akmc::mix::MixState initial_state(const MixLat& L, int64_t nv, int64_t ni) {
+  akmc::mix::MixState st{};
+  st.nvac = (int32_t)nv;
+  st.nitl = (int32_t)ni;
+  uint64_t x = (uint64_t)L.p.seed_time_inc * 0x9E3779B97F4A7C15ull ^ (uint64_t)L.p.seed_recombine;
+  for (int k = 0; k < 4; k++) {
+    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    st.rng[k] = z ^ (z >> 31);
+  }
+  return st;
 }
 
 // Host streams of lattice L cover events [ev, ev + n) and the stand-in draws
@@ -259,51 +329,19 @@ int akmc_mix_create(int32_t n, const akmc_mix_params* ps, const uint16_t* const*
     std::vector<uint32_t> vk;
     for (int64_t s = 0; s < L->nsites; s++)
       if (t[s] == 0) vk.push_back(akmc::key_of_index(L->d, s));
-    HIP_TRY(dalloc(&L->types, L->nsites));
-    HIP_TRY(dalloc(&L->vkey, L->vcap));
-    HIP_TRY(dalloc(&L->ikey, L->icap));
-    HIP_TRY(dalloc(&L->io, L->icap));
-    HIP_TRY(dalloc(&L->iav, L->icap));
-    HIP_TRY(dalloc(&L->iterm, L->icap));
-    HIP_TRY(dalloc(&L->ipre, L->icap));
-    HIP_TRY(dalloc(&L->rkey, L->rcap));
-    HIP_TRY(dalloc(&L->rkind, L->rcap));
-    HIP_TRY(dalloc(&L->rslot, L->rcap));
-    HIP_TRY(dalloc(&L->rate, 8 * (int64_t)L->rcap));
-    HIP_TRY(dalloc(&L->pvac, L->rcap));
-    HIP_TRY(dalloc(&L->tmask, L->rcap));
-    HIP_TRY(dalloc(&L->risk, L->rcap));
-    HIP_TRY(hipMemcpy(L->types, t, (size_t)L->nsites * 2, hipMemcpyHostToDevice));
-    if (nv) HIP_TRY(hipMemcpy(L->vkey, vk.data(), vk.size() * 4, hipMemcpyHostToDevice));
+    MixLat* Lp = L.get();
+    m->lat.push_back(std::move(L));  // freed with the handle from here on
+    {
+      const int rc = alloc_lat(*Lp);
+      if (rc) return rc;
+    }
+    HIP_TRY(hipMemcpy(Lp->types, t, (size_t)Lp->nsites * 2, hipMemcpyHostToDevice));
+    if (nv) HIP_TRY(hipMemcpy(Lp->vkey, vk.data(), vk.size() * 4, hipMemcpyHostToDevice));
     if (ni) {
-      HIP_TRY(hipMemcpy(L->ikey, ik.data(), ik.size() * 4, hipMemcpyHostToDevice));
-      HIP_TRY(hipMemcpy(L->io, io.data(), io.size(), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(Lp->ikey, ik.data(), ik.size() * 4, hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(Lp->io, io.data(), io.size(), hipMemcpyHostToDevice));
     }
-    if (p.device_rng == 0) {
-      HIP_TRY(dalloc(&L->d_u, kChunk));
-      HIP_TRY(dalloc(&L->d_l, kChunk));
-      HIP_TRY(dalloc(&L->d_rec, kRecWindow));
-      L->h_u.resize(kChunk);
-      L->h_l.resize(kChunk);
-      // r::rng is default-constructed (rand_generators.cpp:8); SubLattice
-      // seeds its time stream with the (double -> int64) config seed
-      // (simulation.h:70-76, sublattice.cpp:17).
-      L->time_rng.seed((std::mt19937::result_type)(int64_t)(double)p.seed_time_inc);
-      L->rec.seed((std::mt19937::result_type)p.seed_recombine);
-    }
-    akmc::mix::MixState st{};
-    st.nvac = (int32_t)nv;
-    st.nitl = (int32_t)ni;
-    // Device streams: xoshiro256** seeded by splitmix64 of the two seeds.
-    uint64_t x = (uint64_t)p.seed_time_inc * 0x9E3779B97F4A7C15ull ^ (uint64_t)p.seed_recombine;
-    for (int k = 0; k < 4; k++) {
-      uint64_t z = (x += 0x9E3779B97F4A7C15ull);
-      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-      st.rng[k] = z ^ (z >> 31);
-    }
-    m->h_state[(size_t)i] = st;
-    m->lat.push_back(std::move(L));
+    m->h_state[(size_t)i] = initial_state(*Lp, nv, ni);
   }
   HIP_TRY(hipMemcpy(m->d_state, m->h_state.data(), m->h_state.size() * sizeof(m->h_state[0]), hipMemcpyHostToDevice));
   *out = guard.release();
@@ -586,3 +624,201 @@ int akmc_mix_random_lattice(const akmc_params* p, const int32_t* itl_types, cons
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- pool
+namespace akmc_mixpool {
+
+struct Pool {
+  int device = 0;
+  int32_t nslots = 0;
+  akmc::mix::MixState* d_state = nullptr;
+  akmc::mix::MixDesc* d_desc = nullptr;
+  std::vector<akmc::mix::MixState> h_state;
+  std::vector<std::unique_ptr<MixLat>> slot;
+  int32_t hi = 0;  // states copied back: slots [0, hi)
+};
+
+int pool_create(int device, int32_t slots, Pool** out) {
+  if (!out || slots <= 0) return fail(AKMC_ERR_INVALID, "bad pool arguments");
+  std::unique_ptr<Pool> p(new Pool());
+  p->device = device;
+  p->nslots = slots;
+  HIP_TRY(hipSetDevice(device));
+  HIP_TRY(hipMalloc(&p->d_state, sizeof(akmc::mix::MixState) * (size_t)slots));
+  HIP_TRY(hipMalloc(&p->d_desc, sizeof(akmc::mix::MixDesc) * (size_t)slots));
+  p->h_state.resize((size_t)slots);
+  p->slot.resize((size_t)slots);
+  *out = p.release();
+  return AKMC_OK;
+}
+
+void pool_destroy(Pool* p) {
+  if (!p) return;
+  (void)hipSetDevice(p->device);
+  for (auto& L : p->slot)
+    if (L) free_lat(*L);
+  (void)hipFree(p->d_state);
+  (void)hipFree(p->d_desc);
+  delete p;
+}
+
+int64_t pool_bytes(const akmc_params& comp, int64_t n_itl) {
+  const int64_t n = 2 * comp.box[0] * comp.box[1] * comp.box[2];
+  return lat_bytes(n, std::max<int64_t>(1, comp.va_count), std::max<int64_t>(1, n_itl));
+}
+
+int pool_add_generated(Pool* p, int32_t slot, const akmc_params& comp, int64_t n_itl, int32_t itl_atom, double ef110,
+                       hipStream_t st) {
+  if (!p || slot < 0 || slot >= p->nslots || p->slot[(size_t)slot]) return fail(AKMC_ERR_INVALID, "bad pool slot");
+  for (int d = 0; d < 3; d++)
+    if (comp.box[d] < 8) return fail(AKMC_ERR_INVALID, "box edge %lld < 8 cells", (long long)comp.box[d]);
+  if (2 * comp.box[0] > 2048 || comp.box[1] > 2047 || comp.box[2] > 1023)
+    return fail(AKMC_ERR_INVALID, "box exceeds 1024 x 2047 x 1023 cells");
+  if (!(itl_atom == 1 || itl_atom == 2 || itl_atom == 4 || itl_atom == 8) || n_itl < 0 || comp.va_count < 0)
+    return fail(AKMC_ERR_INVALID, "bad dumbbell atom or counts");
+  auto L = std::make_unique<MixLat>();
+  akmc_mix_params& mp = L->p;
+  akmc_mix_params_default(&mp);
+  for (int d = 0; d < 3; d++) mp.box[d] = comp.box[d];
+  mp.temperature = comp.temperature;
+  mp.attempt_freq = comp.attempt_freq;
+  mp.sector_time = comp.sector_time;
+  mp.physics_time = comp.physics_time;
+  mp.ef110 = ef110;
+  mp.seed_time_inc = comp.seed_time_inc;
+  mp.seed_recombine = comp.seed_event_selection;
+  mp.device_rng = 1;
+  if (!(mp.sector_time > 0) || !(mp.temperature > 0) || !(mp.attempt_freq > 0))
+    return fail(AKMC_ERR_INVALID, "bad time, temperature or frequency");
+  L->d = akmc::box_dims(mp.box);
+  L->nsites = 2 * mp.box[0] * mp.box[1] * mp.box[2];
+  if (comp.va_count + n_itl > L->nsites / 2) return fail(AKMC_ERR_INVALID, "too many defects for the box");
+  // Composition brackets of the alloy, as the replica engine draws them.
+  int nt = 0;
+  uint64_t total = 0;
+  for (int i = 0; i < comp.n_alloy; i++) {
+    const int t = comp.alloy_type[i];
+    const int64_t q = comp.alloy_ratio[i];
+    if (q < 0 || !(t == 1 || t == 2 || t == 4 || t == 8)) return fail(AKMC_ERR_INVALID, "bad alloy entry");
+    if (q == 0) continue;
+    total += (uint64_t)q;
+    if (total > 0xFFFFFFFFull) return fail(AKMC_ERR_INVALID, "alloy ratios too large");
+    L->h_cum[nt] = (uint32_t)total;
+    L->h_tval[nt] = (uint16_t)t;
+    nt++;
+  }
+  if (nt == 0) return fail(AKMC_ERR_INVALID, "empty alloy");
+  // Vacancy sites as the replica engine draws them (repeats collapse), then
+  // distinct dumbbell sites off them, and orientation::random draws.
+  std::vector<int64_t> vidx((size_t)comp.va_count);
+  uint64_t x = (uint64_t)comp.seed_create_vacancy ^ 0x5A5A5A5A5A5A5A5Aull;
+  auto next = [](uint64_t& z) {
+    uint64_t r = (z += 0x9E3779B97F4A7C15ull);
+    r = (r ^ (r >> 30)) * 0xBF58476D1CE4E5B9ull;
+    r = (r ^ (r >> 27)) * 0x94D049BB133111EBull;
+    return r ^ (r >> 31);
+  };
+  for (auto& v : vidx) v = (int64_t)(((unsigned __int128)next(x) * (uint64_t)L->nsites) >> 64);
+  std::sort(vidx.begin(), vidx.end());
+  vidx.erase(std::unique(vidx.begin(), vidx.end()), vidx.end());
+  std::unordered_set<int64_t> taken(vidx.begin(), vidx.end());
+  std::vector<std::pair<int64_t, int8_t>> itl;
+  uint64_t y = (uint64_t)comp.seed_create_vacancy ^ 0xA5A5A5A5A5A5A5A5ull;
+  while ((int64_t)itl.size() < n_itl) {
+    const int64_t s = (int64_t)(((unsigned __int128)next(y) * (uint64_t)L->nsites) >> 64);
+    if (!taken.insert(s).second) continue;
+    const uint32_t v = (uint32_t)(next(y) >> 32) % 12u + 1u;  // orientation::random (orientation.h:38-42)
+    itl.emplace_back(s, (int8_t)(v % 2 == 0 ? (int)(v / 2) : -(int)((v + 1) / 2)));
+  }
+  std::sort(itl.begin(), itl.end());
+  L->vcap = (int32_t)std::max<int64_t>(1, (int64_t)vidx.size());
+  L->icap = (int32_t)std::max<int64_t>(1, n_itl);
+  L->rcap = L->vcap + L->icap;
+  MixLat* Lp = L.get();
+  p->slot[(size_t)slot] = std::move(L);
+  int rc = alloc_lat(*Lp);
+  if (rc) return rc;
+  Lp->h_idx = vidx;
+  for (auto& q : itl) Lp->h_idx.push_back(q.first);
+  for (int64_t s : vidx) Lp->h_vkey.push_back(akmc::key_of_index(Lp->d, s));
+  for (auto& q : itl) {
+    Lp->h_ikey.push_back(akmc::key_of_index(Lp->d, q.first));
+    Lp->h_io.push_back(q.second);
+  }
+  HIP_TRY(dalloc(&Lp->d_cum, nt));
+  HIP_TRY(dalloc(&Lp->d_tval, nt));
+  HIP_TRY(dalloc(&Lp->d_idx, (int64_t)Lp->h_idx.size()));
+  HIP_TRY(hipMemcpyAsync(Lp->d_cum, Lp->h_cum, sizeof(uint32_t) * nt, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(Lp->d_tval, Lp->h_tval, sizeof(uint16_t) * nt, hipMemcpyHostToDevice, st));
+  if (!Lp->h_idx.empty())
+    HIP_TRY(hipMemcpyAsync(Lp->d_idx, Lp->h_idx.data(), Lp->h_idx.size() * 8, hipMemcpyHostToDevice, st));
+  if (akmc_mix_fill_launch(Lp->types, Lp->nsites, (uint64_t)comp.seed_create_types, Lp->d_cum, Lp->d_tval, nt, st))
+    return fail(AKMC_ERR_HIP, "fill launch failed");
+  if (akmc_mix_defects_launch(Lp->types, Lp->d_idx, (int64_t)vidx.size(), Lp->d_idx + vidx.size(), (int64_t)itl.size(),
+                              itl_atom, st))
+    return fail(AKMC_ERR_HIP, "defect placement launch failed");
+  if (!Lp->h_vkey.empty())
+    HIP_TRY(hipMemcpyAsync(Lp->vkey, Lp->h_vkey.data(), Lp->h_vkey.size() * 4, hipMemcpyHostToDevice, st));
+  if (!Lp->h_ikey.empty()) {
+    HIP_TRY(hipMemcpyAsync(Lp->ikey, Lp->h_ikey.data(), Lp->h_ikey.size() * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(Lp->io, Lp->h_io.data(), Lp->h_io.size(), hipMemcpyHostToDevice, st));
+  }
+  // The upload's source stays in the lattice record: the pool's own state
+  // array is overwritten by launches that may run meanwhile.
+  Lp->h_init = initial_state(*Lp, (int64_t)vidx.size(), (int64_t)itl.size());
+  HIP_TRY(hipMemcpyAsync(p->d_state + slot, &Lp->h_init, sizeof(akmc::mix::MixState), hipMemcpyHostToDevice, st));
+  return AKMC_OK;
+}
+
+void pool_setup_done(Pool* p, int32_t slot) {
+  MixLat* L = p->slot[(size_t)slot].get();
+  if (!L) return;
+  void* q[] = {L->d_cum, L->d_tval, L->d_idx};
+  for (void* v : q)
+    if (v) (void)hipFree(v);
+  L->d_cum = nullptr;
+  L->d_tval = nullptr;
+  L->d_idx = nullptr;
+  std::vector<int64_t>().swap(L->h_idx);
+  std::vector<uint32_t>().swap(L->h_vkey);
+  std::vector<uint32_t>().swap(L->h_ikey);
+  std::vector<int8_t>().swap(L->h_io);
+}
+
+int pool_remove(Pool* p, int32_t slot) {
+  if (!p || slot < 0 || slot >= p->nslots || !p->slot[(size_t)slot]) return fail(AKMC_ERR_INVALID, "bad pool slot");
+  free_lat(*p->slot[(size_t)slot]);
+  p->slot[(size_t)slot].reset();
+  return AKMC_OK;
+}
+
+int pool_launch(Pool* p, const std::vector<int32_t>& slots, const std::vector<int64_t>& limits, hipStream_t st) {
+  if (slots.empty()) return AKMC_OK;
+  std::vector<akmc::mix::MixDesc> ds;
+  int32_t hi = 0;
+  for (size_t k = 0; k < slots.size(); k++) {
+    MixLat& L = *p->slot[(size_t)slots[k]];
+    L.target = limits[k];
+    ds.push_back(desc_of(L, p->d_state + slots[k]));
+    hi = std::max(hi, slots[k] + 1);
+  }
+  HIP_TRY(hipMemcpyAsync(p->d_desc, ds.data(), ds.size() * sizeof(ds[0]), hipMemcpyHostToDevice, st));
+  const int rc = akmc_mix_launch(p->d_desc, (int32_t)ds.size(), st);
+  if (rc) return fail(AKMC_ERR_HIP, "mix kernel launch failed");
+  HIP_TRY(hipMemcpyAsync(p->h_state.data(), p->d_state, (size_t)hi * sizeof(akmc::mix::MixState),
+                         hipMemcpyDeviceToHost, st));
+  return AKMC_OK;
+}
+
+const akmc::mix::MixState& pool_state(Pool* p, int32_t slot) { return p->h_state[(size_t)slot]; }
+
+int64_t pool_sites(Pool* p, int32_t slot) { return p->slot[(size_t)slot] ? p->slot[(size_t)slot]->nsites : 0; }
+
+int pool_download(Pool* p, int32_t slot, uint16_t* types, int64_t n) {
+  MixLat* L = p->slot[(size_t)slot].get();
+  if (!L || !types || n != L->nsites) return fail(AKMC_ERR_INVALID, "bad download");
+  HIP_TRY(hipMemcpy(types, L->types, (size_t)n * 2, hipMemcpyDeviceToHost));
+  return AKMC_OK;
+}
+
+}  // namespace akmc_mixpool

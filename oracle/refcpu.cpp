@@ -251,36 +251,65 @@ class RefCpu {
     }
   }
 
+  // The sector's vacancies in sweep order, z, then y, then x in [2 x_lo,
+  // 2 x_hi) (kmc.cpp:72-75): f(id, x, y, z, record) returns false to stop.
+  // The reference visits every site of the sector through the hash
+  // (list_sweep false); list_sweep visits only the vacancies, sorted by id,
+  // which is the same sequence (ids ascend in z, y, x) at a fraction of the
+  // cost, for writing long fixtures.
+  template <class F>
+  void sweep_vacancies(int s, F f) {
+    int64_t lo[3], hi[3];
+    region(s, lo, hi);
+    if (!list_sweep) {
+      for (int64_t z = lo[2]; z < hi[2]; z++)
+        for (int64_t y = lo[1]; y < hi[1]; y++)
+          for (int64_t x = 2 * lo[0]; x < 2 * hi[0]; x++) {
+            const int64_t id = getId(x, y, z);
+            auto it = vac_hash.find(id);
+            if (it == vac_hash.end()) continue;
+            if (!f(id, x, y, z, it->second)) return;
+          }
+      return;
+    }
+    std::vector<int64_t> ids;
+    for (auto& kv : vac_hash) {
+      int64_t x, y, z;
+      coord(kv.first, &x, &y, &z);
+      if (z >= lo[2] && z < hi[2] && y >= lo[1] && y < hi[1] && x >= 2 * lo[0] && x < 2 * hi[0])
+        ids.push_back(kv.first);
+    }
+    std::sort(ids.begin(), ids.end());
+    for (int64_t id : ids) {
+      int64_t x, y, z;
+      coord(id, &x, &y, &z);
+      if (!f(id, x, y, z, vac_hash.at(id))) return;
+    }
+  }
+
   // ABVIModel::calcRates (models/abvi/kmc.cpp:68-115) with
   // VacancyHash::beforeRatesUpdate2/updateRates2 (vac_hash.cpp:7-46).
   double calcRates(int s) {
-    int64_t lo[3], hi[3];
-    region(s, lo, hi);
     double sum_rates = 0;
-    for (int64_t z = lo[2]; z < hi[2]; z++)
-      for (int64_t y = lo[1]; y < hi[1]; y++)
-        for (int64_t x = 2 * lo[0]; x < 2 * hi[0]; x++) {
-          const int64_t id = getId(x, y, z);
-          auto it = vac_hash.find(id);
-          if (it == vac_hash.end()) continue;
-          VacancyHash& vh = it->second;  // node reference survives rehash
-          int64_t n1[8];
-          int t1[8];
-          nn1(x, y, z, n1);
-          for (int b = 0; b < 8; b++) t1[b] = readout(n1[b]);
-          for (int b = 0; b < 8; b++) vh.nn_rates[b] = 0;
-          unsigned char avail = 0;
-          for (int b = 0; b < 8; b++)
-            if (t1[b] != V) avail |= (unsigned char)(1u << b);
-          vh.avail_trans_dir = avail;
-          for (int b = 0; b < 8; b++) {
-            if ((avail >> b) & 1) {
-              const double r = rate(id, n1[b], t1[b]);
-              sum_rates += r;
-              vh.nn_rates[b] = r;
-            }
-          }
+    sweep_vacancies(s, [&](int64_t id, int64_t x, int64_t y, int64_t z, VacancyHash& vh) {
+      int64_t n1[8];
+      int t1[8];
+      nn1(x, y, z, n1);
+      for (int b = 0; b < 8; b++) t1[b] = readout(n1[b]);
+      for (int b = 0; b < 8; b++) vh.nn_rates[b] = 0;
+      unsigned char avail = 0;
+      for (int b = 0; b < 8; b++)
+        if (t1[b] != V) avail |= (unsigned char)(1u << b);
+      vh.avail_trans_dir = avail;
+      for (int b = 0; b < 8; b++) {
+        if ((avail >> b) & 1) {
+          const double r = rate(id, n1[b], t1[b]);
+          sum_rates += r;
+          vh.nn_rates[b] = r;
         }
+      }
+      return true;
+    });
     sum_rates += 0.0;  // defectGenRate(): env::global_env.defect_gen_rate == 0
     return sum_rates;
   }
@@ -288,26 +317,22 @@ class RefCpu {
   // ABVIModel::select (models/abvi/kmc.cpp:242-309): first (vacancy, dir) in
   // sweep order whose running sum is > excepted_rate.
   bool select(int s, double excepted_rate, int64_t* from, int64_t* to, int* dir) {
-    int64_t lo[3], hi[3];
-    region(s, lo, hi);
     double acc = 0.0;
-    for (int64_t z = lo[2]; z < hi[2]; z++)
-      for (int64_t y = lo[1]; y < hi[1]; y++)
-        for (int64_t x = 2 * lo[0]; x < 2 * hi[0]; x++) {
-          const int64_t id = getId(x, y, z);
-          auto it = vac_hash.find(id);
-          if (it == vac_hash.end()) continue;
-          for (int b = 0; b < 8; b++) {
-            acc += it->second.nn_rates[b];
-            if (acc > excepted_rate) {
-              int64_t n1[8];
-              nn1(x, y, z, n1);  // LatListMeta::getIdBy1nnOffset
-              *from = id; *to = n1[b]; *dir = b;
-              return true;
-            }
-          }
+    bool found = false;
+    sweep_vacancies(s, [&](int64_t id, int64_t x, int64_t y, int64_t z, VacancyHash& vh) {
+      for (int b = 0; b < 8; b++) {
+        acc += vh.nn_rates[b];
+        if (acc > excepted_rate) {
+          int64_t n1[8];
+          nn1(x, y, z, n1);  // LatListMeta::getIdBy1nnOffset
+          *from = id; *to = n1[b]; *dir = b;
+          found = true;
+          return false;
         }
-    return false;  // the reference returns event::DefectGen, which perform() asserts on
+      }
+      return true;
+    });
+    return found;  // not found: the reference returns event::DefectGen, which perform() asserts on
   }
 
   // ABVIModel::perform (models/abvi/kmc.cpp:358-396), vacancy branch.
@@ -501,6 +526,7 @@ class RefCpu {
   int64_t events = 0;
   std::vector<int64_t>* log_from = nullptr;
   std::vector<int64_t>* log_to = nullptr;
+  bool list_sweep = false;               // sweep_vacancies: the vacancy list instead of every site
   // Ghost-image form (set before createRandom): see sector_sync().
   bool ghost_images = false;
   bool ghost_sync = true;                // false only in a test that plants a missing sync
@@ -581,6 +607,9 @@ void ref_get_sites(void* h, uint8_t* out) {
     f = (uint8_t)((f & ~first) | 1);
   }
 }
+
+// Vacancy-list sweep on or off (the same sweep order; see sweep_vacancies).
+void ref_set_list_sweep(void* h, int on) { ((refcpu::RefCpu*)h)->list_sweep = on != 0; }
 
 // Ghost-image form on or off (before ref_create_random / ref_set_sites).
 void ref_set_ghost_images(void* h, int on) { ((refcpu::RefCpu*)h)->ghost_images = on != 0; }

@@ -83,6 +83,8 @@ def lib():
         L.ref_streams.restype = None
         L.ref_set_ghost_images.argtypes = [vp, ctypes.c_int]
         L.ref_set_ghost_images.restype = None
+        L.ref_set_list_sweep.argtypes = [vp, ctypes.c_int]
+        L.ref_set_list_sweep.restype = None
         L.ref_set_ghost_sync.argtypes = [vp, ctypes.c_int]
         L.ref_set_ghost_sync.restype = None
         L.ref_get_ext_sites.argtypes = [vp, vp, ctypes.c_int64]
@@ -130,11 +132,13 @@ class RefCpu:
     them reach the owner after the sector and changed sites refresh their
     copies (src/pack/*, sublattice.inl:161-163)."""
 
-    def __init__(self, cfg, create=True, ghost_images=False):
+    def __init__(self, cfg, create=True, ghost_images=False, list_sweep=False):
         self.L = lib()
         self._p = make_params(cfg)
         self.h = self.L.ref_create(ctypes.byref(self._p))
         self.n_sites = self.L.ref_n_sites(self.h)
+        if list_sweep:
+            self.L.ref_set_list_sweep(self.h, 1)
         if ghost_images:
             self.L.ref_set_ghost_images(self.h, 1)
         if create:

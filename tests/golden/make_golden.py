@@ -54,7 +54,8 @@ def cfg0_hashes():
 # sector threshold 1e-6 s): the first 1e5 events.  Every 1000 events the
 # SHA-256 of that block's (from, to) site indices (int64 pairs) is stored,
 # every 10,000 events the SHA-256 of the whole site array, and the sector
-# clocks at the end.  About an hour of oracle time (~30 events/s).
+# clocks at the end.  The oracle visits the sector's vacancies through its
+# list (the same sweep order as the site sweep, tests/test_ghost_oracle.py).
 CFG1 = make_cfg(n=256, va=6711, T=573.0, sector_T=1e-6, physics=1.0)
 CFG1_EVENTS, CFG1_EVERY, CFG1_SITES_EVERY = 100000, 1000, 10000
 
@@ -62,7 +63,7 @@ CFG1_EVENTS, CFG1_EVERY, CFG1_SITES_EVERY = 100000, 1000, 10000
 def cfg1_hashes(resume_dir=None):
     """resume_dir (optional): where the oracle state and the digests so far
     are kept after every 10,000 events, so that an interrupted run resumes."""
-    r = refcpu.RefCpu(CFG1)
+    r = refcpu.RefCpu(CFG1, list_sweep=True)
     log_digests, site_digests = [], []
     out = os.path.join(HERE, "hashes_cfg1_256cube.npz")
     k0 = 0

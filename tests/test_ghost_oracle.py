@@ -115,3 +115,20 @@ def test_ghost_form_catches_a_missing_sync():
     rb, fb, tb = b.run(3000, log=True)
     k = min(a.state()["events"], b.state()["events"])
     assert not (np.array_equal(fa[:k], fb[:k]) and np.array_equal(ta[:k], tb[:k]))
+
+
+@pytest.mark.parametrize("box,va", [((16, 16, 16), 40), ((17, 13, 11), 60), ((12, 12, 12), 200)])
+def test_list_sweep_equals_the_site_sweep(box, va):
+    """The oracle's vacancy-list sweep (used to write long fixtures) visits the
+    sector's vacancies in the reference's sweep order: identical runs."""
+    cfg = make_cfg(box=box, va=va, T=600.0, alloy=ALLOY_RICH)
+    a = refcpu.RefCpu(cfg)
+    b = refcpu.RefCpu(cfg, list_sweep=True)
+    ra, fa, ta = a.run(3000, log=True)
+    rb, fb, tb = b.run(3000, log=True)
+    assert np.array_equal(fa, fb) and np.array_equal(ta, tb) and np.array_equal(a.sites(), b.sites())
+    assert np.array_equal(a.state()["evolution"], b.state()["evolution"])
+    for s in (0, 5):
+        sa, pa, qa = a.sector_rates(s)
+        sb, pb, qb = b.sector_rates(s)
+        assert np.array_equal(sa, sb) and np.array_equal(pa, pb) and qa == qb
