@@ -564,6 +564,7 @@ typedef struct akmc_mix_stats {
   double seconds;            /* wall time of the handle's last akmc_mix_run */
   double kernel_seconds;     /* device time of its launches */
   int64_t launches;
+  int64_t n_skipped_steps;   /* time steps passed in one go: every occupied sector's clock was ahead */
 } akmc_mix_stats;
 
 typedef struct akmc_mix akmc_mix;

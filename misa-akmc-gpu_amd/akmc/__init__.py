@@ -762,7 +762,8 @@ class MixStats(ctypes.Structure):
                 ("stop", ctypes.c_int32), ("nvac", ctypes.c_int32), ("nitl", ctypes.c_int32),
                 ("reserved", ctypes.c_int32), ("n_recombined", ctypes.c_int64), ("n_generated", ctypes.c_int64),
                 ("n_itl_events", ctypes.c_int64), ("n_eval", ctypes.c_int64), ("seconds", ctypes.c_double),
-                ("kernel_seconds", ctypes.c_double), ("launches", ctypes.c_int64)]
+                ("kernel_seconds", ctypes.c_double), ("launches", ctypes.c_int64),
+                ("n_skipped_steps", ctypes.c_int64)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("evolution", "reserved")}

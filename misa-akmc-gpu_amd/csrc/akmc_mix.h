@@ -27,6 +27,7 @@ struct MixState {
   int64_t n_generated;     // Frenkel pairs generated
   int64_t n_eval;          // record evaluations
   int64_t n_itl_events;    // dumbbell transitions
+  int64_t n_skipped;       // time steps taken in one go (no event could happen in them)
   uint64_t rng[4];         // xoshiro256** state (device streams)
 };
 

@@ -595,6 +595,33 @@ __device__ double rec_delta(const Ctx& c, uint32_t v, uint32_t dk, int nitl) {
   return (-1 * D.ef110) + e_after - e_before;
 }
 
+// Sectors that hold a vacancy or a dumbbell: bit s for sector s.
+__device__ uint32_t occupied_sectors(const Ctx& c, int nvac, int nitl) {
+  const MixDesc& D = *c.D;
+  const int l = lane_id();
+  uint32_t m = 0;
+  for (int j = l; j < nvac; j += 64) m |= 1u << key_sector(c.d, D.vkey[j]);
+  for (int j = l; j < nitl; j += 64) m |= 1u << key_sector(c.d, D.ikey[j]);
+  uint32_t out = 0;
+  for (int b = 0; b < 8; b++)
+    if (ballot((m >> b) & 1u)) out |= 1u << b;
+  return out;
+}
+
+// The first step k >= k0 at whose start a sector with clock e has time left:
+// thr = fl((k + 1) T) - e > 0, i.e. fl((k + 1) T) > e, monotone in k.
+// Returns kmax when that step lies at or beyond kmax (a NaN clock never runs
+// again), and k0 when the search does not settle (no skipping then).
+__device__ int64_t first_step_above(double e, double T, int64_t k0, int64_t kmax) {
+  if (static_cast<double>(k0 + 1) * T > e) return k0;
+  if (!(e < static_cast<double>(kmax) * T)) return kmax;
+  int64_t k = (int64_t)(e / T) - 2;  // (k + 1) T <= e - T: not yet
+  if (k < k0) k = k0;
+  for (int i = 0; i < 64; i++, k++)
+    if (static_cast<double>(k + 1) * T > e) return k < kmax ? k : kmax;
+  return k0;
+}
+
 struct Draws {
   const MixDesc* D;
   MixState* st;
@@ -637,6 +664,41 @@ __global__ __launch_bounds__(64) void mix_kernel(const MixDesc* descs) {
     if (!st.in_sector) {
       if (st.step >= D.time_steps) { status = AKMC_RUN_TIME_DONE; break; }
       if (++visits > (int64_t)1 << 16) { status = AKMC_RUN_YIELD; break; }
+      // Whole steps in which no event can happen: every occupied sector's
+      // clock is ahead (thr <= 0: the visit changes nothing) and every empty
+      // sector (zero total rate) only catches its clock up.  An empty sector
+      // entering step j with clock e < a_j = fl((j + 1) T) ends it at
+      // fl(e + fl(a_j - e)) = a_j exactly: e >= a_{j-1} >= a_j / 2 from the
+      // step before (j >= 1), so the difference is exact (Sterbenz).  Such
+      // steps are taken in one go, up to the first step at which an occupied
+      // sector has time left.  Not with defect generation (no sector has a
+      // zero total then), nor where the loop would stop at the next entry.
+      if (st.sect == 0 && st.step >= 1 && !(D.gen_rate > 0) && D.rates_sector < 0 && st.events < D.max_events &&
+          !(D.u && st.events >= D.rng_end) && !(D.rec && st.rec_pos + 4 > D.rec_end)) {
+        const uint32_t occ = occupied_sectors(c, nvac, nitl);
+        int64_t k = D.time_steps;
+        for (int q = 0; q < 8; q++)
+          if ((occ >> q) & 1u) k = min(k, first_step_above(evo[q], D.sector_T, st.step, D.time_steps));
+        if (k > st.step) {
+          const double a = static_cast<double>(k) * D.sector_T;  // a_{k-1}
+          // The last visit of step k - 1 leaves its thr and t behind (the
+          // loop's state where the time loop ends).
+          const int s7 = ring_sector(7);
+          double e7 = evo[s7];
+          wave_sync();
+          if (l < 8 && !((occ >> l) & 1u) && evo[l] < a) evo[l] = a;
+          wave_sync();
+          if (k - 1 > st.step) {
+            const double ap = static_cast<double>(k - 1) * D.sector_T;  // a_{k-2}
+            if (!((occ >> s7) & 1u) && e7 < ap) e7 = ap;
+          }
+          st.thr = a - e7;
+          st.t = (!((occ >> s7) & 1u) && st.thr > 0.0) ? st.thr : 0.0;
+          st.n_skipped += k - st.step;
+          st.step = k;
+          continue;
+        }
+      }
       const int s0 = ring_sector(st.sect);
       st.thr = static_cast<double>(st.step + 1) * D.sector_T - evo[s0];  // sublattice.inl:72
       st.t = 0.0;
@@ -659,11 +721,16 @@ __global__ __launch_bounds__(64) void mix_kernel(const MixDesc* descs) {
       __threadfence_block();
       nr = uni_i(collect(c, s, nvac, nitl));
       if (nr < 0) { status = AKMC_RUN_ERR_CAPACITY; break; }
-      for (int base = 0; base < nitl; base += 64)
-        if (base + l < nitl) refresh_term(c, base + l);
-      __threadfence_block();
-      wave_sync();
-      prefix_from(c, 0, nitl);
+      // Edumb terms and prefixes serve the sector's rates and the edits that
+      // follow an event; a sector with no record and no generation has a zero
+      // total and performs none.
+      if (nr > 0 || D.gen_rate > 0 || D.rates_sector >= 0) {
+        for (int base = 0; base < nitl; base += 64)
+          if (base + l < nitl) refresh_term(c, base + l);
+        __threadfence_block();
+        wave_sync();
+        prefix_from(c, 0, nitl);
+      }
       for (int base = 0; base < nr; base += 64) {
         const int cnt = min(64, nr - base);
         if (l < cnt) list[l] = base + l;

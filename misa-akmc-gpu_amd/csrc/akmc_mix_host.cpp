@@ -442,6 +442,7 @@ int akmc_mix_stats_of(akmc_mix* m, int32_t i, akmc_mix_stats* s) {
   s->seconds = m->seconds;
   s->kernel_seconds = m->kernel_seconds;
   s->launches = m->launches;
+  s->n_skipped_steps = st.n_skipped;
   return AKMC_OK;
 }
 

@@ -106,6 +106,42 @@ def test_trajectory_bit_exact(name, box, seed, f_itl, f_v, kw, n):
         assert (kinds == 1).sum() > 0
 
 
+def _sparse_lattice(box, seed, n_vac, n_itl):
+    """Fe with 1 % Cu and 1 % Ni, n_vac vacancies and n_itl Fe dumbbells at
+    random sites: once the vacancies have recombined, the dumbbells sit in
+    configurations whose rates differ by ten orders of magnitude, so sector
+    clocks run far ahead and long stretches of time steps pass without an
+    event."""
+    n = 2 * box[0] * box[1] * box[2]
+    rng = np.random.default_rng(seed)
+    t = np.ones(n, np.uint16)
+    u = rng.random(n)
+    t[u < 0.02] = 4
+    t[u < 0.01] = 2
+    idx = rng.choice(n, n_vac + n_itl, replace=False)
+    t[idx[:n_vac]] = 0
+    s = np.sort(idx[n_vac:])
+    for q in s:
+        a = int(t[q])
+        t[q] = (a << 8) | 1 if a <= 1 else (1 << 8) | a
+    o = rng.choice([-6, -5, -4, -3, -2, -1, 1, 2, 3, 4, 5, 6], s.size).astype(np.int8)
+    return t, s, o
+
+
+@pytest.mark.parametrize("physics_time,n", [(1e3, 3000), (2e-4, 5000)], ids=["events", "time_end"])
+def test_idle_steps_taken_in_one_go_match(physics_time, n):
+    """Time steps with no possible event are passed in one go on the device;
+    the oracle walks every sector of every step.  Same events, clocks and
+    step counts bit for bit, also where the time loop ends among them."""
+    say("mix idle steps", physics_time, n)
+    box = (12, 12, 12)
+    t, s, o = _sparse_lattice(box, 1, 7, 20)
+    st, kinds = _compare(box, t, s, o, n, pieces=3, temperature=650.0, sector_time=2e-9, physics_time=physics_time)
+    assert st["n_skipped_steps"] > 10000
+    if physics_time < 1:
+        assert st["events"] < n and st["stop"] == akmc.AKMC_STOP_TIME
+
+
 def test_recombination_happens_and_matches():
     say("mix recombination")
     box = (8, 8, 8)
