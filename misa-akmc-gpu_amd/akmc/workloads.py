@@ -9,7 +9,8 @@ replicas       independent boxes: edge U{16..96} cells, Cu U[0.1,1.5] at%,
                T U[500,700] K, one seed set per replica (splitmix64 of a base)
 jobs           serving jobs: edge U{32..320}, composition as `replicas`,
                vacancies U[1e-4,5e-4], T U[500,700] K, target events
-               log-uniform in [lo, hi]
+               log-uniform in [lo, hi]; jobs_with_itl adds dumbbells at 1e-5
+               of the sites to a fraction of them
 """
 import math
 
@@ -96,6 +97,19 @@ def jobs(n=4000, base=0x5EED0002, lo=32, hi=320, ev_lo=1e5, ev_hi=1e7):
         T = st.uniform(500.0, 700.0)
         target = int(round(math.exp(st.uniform(math.log(ev_lo), math.log(ev_hi)))))
         out.append((_cfg(L, alloy, va, T, st), target))
+    return out
+
+
+def jobs_with_itl(n=4000, base=0x5EED0002, lo=32, hi=320, ev_lo=1e5, ev_hi=1e7, itl_frac=0.1, itl_density=1e-5):
+    """(config, target_events, n_itl) per job: the `jobs` law, and a fraction
+    itl_frac of the jobs also carry self-interstitial dumbbells at
+    itl_density of the sites (at least one)."""
+    out = []
+    for i, (cfg, target) in enumerate(jobs(n, base, lo, hi, ev_lo, ev_hi)):
+        st = Stream((base ^ 0xD0BB) ^ (i * 0x9E3779B97F4A7C15 & MASK64))
+        L = cfg["box"][0]
+        n_itl = max(1, int(round(itl_density * 2 * L ** 3))) if st.uniform(0.0, 1.0) < itl_frac else 0
+        out.append((cfg, target, n_itl))
     return out
 
 

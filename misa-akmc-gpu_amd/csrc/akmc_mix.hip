@@ -159,10 +159,26 @@ __device__ __forceinline__ int lb_lane(const uint32_t* key, int n, uint32_t k) {
   return lo;
 }
 
+// LDS working space of the dumbbell evaluation passes (16 records a pass).
+constexpr int kItlPass = 16;
+constexpr int kMaxCand = 48;  // dumbbells within reach of one record
+constexpr int kMaxAff = 16;   // terms one jump changes
+struct ItlWork {
+  uint32_t wkey[kItlPass];
+  int32_t own[kItlPass];
+  int32_t ncand[kItlPass];
+  int32_t cand[kItlPass][kMaxCand];
+  int32_t aq[kMaxAff][64];  // per lane: list index of each changed term, ascending
+  double at[kMaxAff][64];   // and the term on the lattice after the jump
+  int32_t vlist[64], ilist[64];
+  int32_t ibuf[128];        // dumbbell records gathered for full passes
+};
+
 struct Ctx {
   const MixDesc* D;
   Dims td;
   BoxDims d;
+  ItlWork* W;
 };
 
 __device__ __forceinline__ int64_t site_of_key(const Ctx& c, uint32_t k) { return (int64_t)key_index(c.d, k); }
@@ -251,6 +267,14 @@ __device__ double vac_rate(const Ctx& c, uint32_t w, int b) {
   return D.nu * exp_glibc(-ea / D.kT, c_exp_tab);
 }
 
+// A dumbbell jump's rate from its energies: e_before = bonds + Edumb before,
+// e_after = bonds after + Edumb after, summed in this order.
+__device__ __forceinline__ double itl_finish(const Ctx& c, int ghost, double e_before, double bonds_after, double sa) {
+  const double e_after = bonds_after + sa;
+  const double ea = e0_itl(ghost) + (e_after - e_before) / 2;
+  return c.D->nu * exp_glibc(-ea / c.D->kT, c_exp_tab);
+}
+
 // Rate of dumbbell j (at key w) jumping towards 1nn tag b (ItlRatesSolver::
 // deltaE, itl_rates_solver.cpp:31-80): both sites' bonds and the whole list's
 // Edumb before and after the jumping atom moves; Edumb after is the prefix
@@ -292,49 +316,217 @@ __device__ double itl_rate(const Ctx& c, uint32_t w, int j, int b, int n) {
       sa += D.iterm[q];
     }
   }
-  const double e_before = e_src_b + e_des_b + sb;
-  const double e_after = e_src_a + e_des_a + sa;
-  const double ea = e0_itl(ghost) + (e_after - e_before) / 2;
-  return D.nu * exp_glibc(-ea / D.kT, c_exp_tab);
+  return itl_finish(c, ghost, e_src_b + e_des_b + sb, e_src_a + e_des_a, sa);
 }
 
-// Evaluates the records idx[0 .. cnt) (a wave-uniform list in LDS): lane
-// (a, b) takes direction b of record a, eight records per pass.  A dumbbell's
-// slot 2i and 2i + 1 hold half the rate of its i-th orientation direction when
-// that direction is available (Itl::ratesIndex, itl_list.cpp:67-74).
+// The jump of the dumbbell at key w (list entry j, orientation o) towards 1nn
+// tag b: the sites, types and bond energies before and after.
+struct ItlJump {
+  uint32_t w, wd;
+  int64_t src, dst;
+  int ghost, src_after, dst_after;
+  double e_before_bonds, e_after_bonds;
+};
+__device__ ItlJump itl_jump(const Ctx& c, uint32_t w, int o, int b) {
+  const MixDesc& D = *c.D;
+  ItlJump J;
+  int x, y, z;
+  key_xyz(w, x, y, z);
+  J.w = w;
+  J.src = itl::site_of(c.td, x, y, z);
+  const int type = D.types[J.src];
+  J.ghost = ((c_high[o + 6] >> b) & 1) ? hi_end(type) : lo_end(type);
+  int ox, oy, oz;
+  nn1_offset(x & 1, b, &ox, &oy, &oz);
+  const int tx = wrapc(x + ox, c.d.X), ty = wrapc(y + oy, c.d.Y), tz = wrapc(z + oz, c.d.Z);
+  J.dst = itl::site_of(c.td, tx, ty, tz);
+  J.wd = make_key((uint32_t)tx, (uint32_t)ty, (uint32_t)tz);
+  const int tdst = D.types[J.dst];
+  J.src_after = diff_type(type, J.ghost);
+  J.dst_after = combine(tdst, J.ghost);
+  const View L0{D.types, -1, -1, 0, 0};
+  const View L1{D.types, J.src, J.dst, J.src_after, J.dst_after};
+  J.e_before_bonds = itl::count(c.td, L0, x, y, z, J.ghost) + itl::count(c.td, L0, tx, ty, tz, tdst);
+  J.e_after_bonds = itl::count(c.td, L1, x, y, z, J.src_after) + itl::count(c.td, L1, tx, ty, tz, J.ghost);
+  return J;
+}
+
+__device__ __forceinline__ int wave_min_i(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return uni_i(v);
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return uni_i(v);
+}
+
+// Rates of the dumbbell records rr[0 .. m), m <= 16: lane (a, i) takes the
+// i-th direction of record a's orientation, and slots 2i and 2i + 1 hold half
+// its rate when that direction is available (Itl::ratesIndex, itl_list.cpp:
+// 67-74).  Edumb after the jump is the same left fold as itl_rate's, taken by
+// all lanes in step: one pass over the list from the lowest changed index,
+// each lane adding the stored term or, at its own changed indices, the term
+// recomputed on the lattice after its jump.  The changed terms (dumbbells
+// whose term reads the jump's source or target) are found among the dumbbells
+// within reach of the record, which one scan of the list collects for the 16
+// records at once; a lane with more candidates or changed terms than the
+// working space holds takes itl_rate's own serial fold.
+__device__ void itl_pass(const Ctx& c, MixSector& S, const int32_t* rr, int m, int nitl) {
+  const MixDesc& D = *c.D;
+  ItlWork& W = *c.W;
+  const int l = lane_id();
+  if (l < kItlPass) {
+    W.wkey[l] = l < m ? D.rkey[rr[l]] : 0u;
+    W.ncand[l] = 0;
+    W.own[l] = -1;
+  }
+  __threadfence_block();
+  wave_sync();
+  // The list is sorted by key (z major): only entries within two planes of
+  // the records' z range can be within reach.
+  const int zlo = uni_i((int)key_z(W.wkey[0]) - 2), zhi = uni_i((int)key_z(W.wkey[m - 1]) + 2);
+  int q0 = 0, q1 = nitl;
+  if (zlo >= 0 && zhi < c.d.Z) {
+    q0 = uni_i(lower_bound_key(D.ikey, nitl, make_key(0u, 0u, (uint32_t)zlo)));
+    q1 = zhi + 1 < c.d.Z ? uni_i(lower_bound_key(D.ikey, nitl, make_key(0u, 0u, (uint32_t)(zhi + 1)))) : nitl;
+  }
+  for (int base = q0; base < q1; base += 64) {
+    const int q = base + l;
+    const uint32_t kq = q < q1 ? D.ikey[q] : 0u;
+    for (int a = 0; a < m; a++) {
+      const uint32_t wr = W.wkey[a];
+      const bool hit = q < q1 && in_reach(c.d, wr, kq);
+      const uint64_t hm = ballot(hit);
+      if (!hm) continue;
+      const int n0 = W.ncand[a];
+      if (hit) {
+        const int pos = n0 + __popcll(hm & ((1ull << l) - 1));
+        if (pos < kMaxCand) W.cand[a][pos] = q;
+        if (kq == wr) W.own[a] = q;
+      }
+      __threadfence_block();
+      wave_sync();
+      if (l == 0) W.ncand[a] = n0 + __popcll(hm);
+      __threadfence_block();
+      wave_sync();
+    }
+  }
+  const int a = l >> 2, i = l & 3;
+  const bool rec = a < m;
+  const int r = rec ? rr[a] : 0;
+  const int j = rec ? W.own[a] : -1;
+  const int nca = rec ? W.ncand[a] : 0;
+  int o = 0;
+  uint8_t dirs = 0, av = 0;
+  if (j >= 0) {
+    o = D.io[j];
+    dirs = dirs_of(o);
+    av = D.iav[j];
+  }
+  int dsel = 0;
+  for (int b = 0, seen = 0; b < N1; b++)
+    if ((dirs >> b) & 1) {
+      if (seen == i) dsel = b;
+      seen++;
+    }
+  const bool act = j >= 0 && ((av >> dsel) & 1);
+  bool slow = act && nca > kMaxCand;
+  ItlJump J{};
+  if (act) J = itl_jump(c, W.wkey[a], o, dsel);
+  const View L1{D.types, J.src, J.dst, J.src_after, J.dst_after};
+  int na = 0, qf = nitl;
+  const int cmax = wave_max_i(act && !slow ? nca : 0);
+  for (int t = 0; t < cmax; t++) {
+    if (act && !slow && t < nca) {
+      const int q = W.cand[a][t];
+      const uint32_t kq = D.ikey[q];
+      if (term_reads(c.d, kq, J.w) || term_reads(c.d, kq, J.wd)) {
+        if (na < kMaxAff) {
+          int qx, qy, qz;
+          key_xyz(kq, qx, qy, qz);
+          W.aq[na][l] = q;
+          W.at[na][l] = edumb_term(c.td, L1, qx, qy, qz, L1.at(site_of_key(c, kq)), D.io[q], D.iav[q]);
+          if (na == 0) qf = q;
+          na++;
+        } else {
+          slow = true;
+        }
+      }
+    }
+  }
+  const bool fast = act && !slow && na > 0;
+  if (!fast) qf = nitl;
+  double sa = fast && qf > 0 ? D.ipre[qf - 1] : 0.0;
+  int k = 0, nq = fast ? W.aq[0][l] : nitl;
+  const int qmin = wave_min_i(qf);
+  for (int base = qmin & ~63; base < nitl; base += 64) {
+    const double t = base + l < nitl ? D.iterm[base + l] : 0.0;
+    const int cnt = min(64, nitl - base);
+    for (int ii = 0; ii < cnt; ii++) {
+      const int q = base + ii;
+      const double tq = bcast_d(t, ii);
+      if (q >= qf) {
+        double v = tq;
+        if (q == nq) {
+          v = W.at[k][l];
+          k++;
+          nq = k < na ? W.aq[k][l] : nitl;
+        }
+        sa += v;
+      }
+    }
+  }
+  double rate = 0.0;
+  if (fast) {
+    const double sb = nitl > 0 ? D.ipre[nitl - 1] : 0.0;
+    rate = itl_finish(c, J.ghost, J.e_before_bonds + sb, J.e_after_bonds, sa);
+  } else if (act) {
+    rate = itl_rate(c, W.wkey[a], j, dsel, nitl);
+  }
+  const double v = act ? rate / 2 : 0.0;
+  if (rec) {
+    S.rate[8 * r + 2 * i] = v;
+    S.rate[8 * r + 2 * i + 1] = v;
+  }
+  const int ti = rec ? tie_index(v) : -1;
+  uint64_t tm = ti >= 0 ? 1ull << ti : 0ull;
+  uint32_t lo = (uint32_t)tm, hi = (uint32_t)(tm >> 32);
+  lo |= __shfl_xor(lo, 1);
+  hi |= __shfl_xor(hi, 1);
+  lo |= __shfl_xor(lo, 2);
+  hi |= __shfl_xor(hi, 2);
+  if (rec && i == 0) S.tmask[r] = ((uint64_t)hi << 32) | lo;
+  __threadfence_block();
+  wave_sync();
+}
+
+// Evaluates the records idx[0 .. cnt), cnt <= 64 (a wave-uniform list in
+// LDS): the vacancies eight per pass, lane (a, b) taking direction b of
+// record a; the dumbbells sixteen per pass (itl_pass).
 __device__ void eval_records(const Ctx& c, MixSector& S, const int32_t* idx, int cnt, int nitl, int64_t& n_eval) {
   const MixDesc& D = *c.D;
+  ItlWork& W = *c.W;
   const int l = lane_id();
-  for (int p = 0; p < cnt; p += 8) {
+  const int r0 = l < cnt ? idx[l] : 0;
+  const bool isv = l < cnt && D.rkind[r0] == K_VAC;
+  const uint64_t mv = ballot(isv), mi = ballot(l < cnt && !isv);
+  if (isv) W.vlist[__popcll(mv & ((1ull << l) - 1))] = r0;
+  if (l < cnt && !isv) W.ilist[__popcll(mi & ((1ull << l) - 1))] = r0;
+  __threadfence_block();
+  wave_sync();
+  const int nv = __popcll(mv), ni = __popcll(mi);
+  for (int p = 0; p < nv; p += 8) {
     const int a = l >> 3, b = l & 7;
-    const bool act = p + a < cnt;
-    const int r = act ? idx[p + a] : 0;
-    const uint32_t w = D.rkey[r];
-    const int kind = act ? D.rkind[r] : K_VAC;
-    double rate = 0.0;
-    uint8_t dirs = 0, av = 0;
-    if (act && kind == K_VAC) {
-      rate = vac_rate(c, w, b);
-    } else if (act && nitl > 0) {
-      const int j = min(lb_lane(D.ikey, nitl, w), nitl - 1);  // the record's own list entry
-      dirs = dirs_of(D.io[j]);
-      av = D.iav[j];
-      if ((av >> b) & 1) rate = itl_rate(c, w, j, b, nitl);
-    }
-    // Slot b of a dumbbell: the (b / 2)-th set direction of its orientation.
-    int dsel = 0;
-    for (int i = 0, seen = 0; i < N1; i++)
-      if ((dirs >> i) & 1) {
-        if (seen == (b >> 1)) dsel = i;
-        seen++;
-      }
-    const double rd = __shfl(rate, (l & ~7) + dsel, 64);
-    const double v = kind == K_VAC ? rate : (((av >> dsel) & 1) ? rd / 2 : 0.0);
+    const bool act = p + a < nv;
+    const int r = act ? W.vlist[p + a] : 0;
+    const double v = act ? vac_rate(c, D.rkey[r], b) : 0.0;
     if (act) S.rate[8 * r + b] = v;
     const int ti = act ? tie_index(v) : -1;
     const uint64_t tm = or8_u64(ti >= 0 ? 1ull << ti : 0ull);
     if (act && b == 0) S.tmask[r] = tm;
   }
+  if (ni > 0 && nitl > 0)
+    for (int p = 0; p < ni; p += kItlPass) itl_pass(c, S, W.ilist + p, min(kItlPass, ni - p), nitl);
   n_eval += cnt;
   __threadfence_block();
   wave_sync();
@@ -539,28 +731,49 @@ __device__ int collect(const Ctx& c, int s, int nvac, int nitl) {
 }
 
 // Evaluates every record in [lo, hi) that is within reach of a changed site
-// (or every dumbbell when `all_itl`), through the LDS index list.  Returns
-// the lowest evaluated position (hi if none).
+// (or every dumbbell when `all_itl`, or every record when `every`), through
+// the LDS index list.  Vacancies are evaluated chunk by chunk; dumbbells are
+// gathered until 64 are pending, so that their passes run full.  Returns the
+// lowest evaluated position (hi if none).
 __device__ int eval_near(const Ctx& c, MixSector& S, int lo, int hi, const uint32_t* cs, int nc, bool all_itl,
-                         int nitl, int32_t* list, int64_t& n_eval) {
+                         bool every, int nitl, int32_t* list, int64_t& n_eval) {
   const MixDesc& D = *c.D;
+  ItlWork& W = *c.W;
   const int l = lane_id();
-  int first = hi;
+  int first = hi, ni = 0;
   for (int base = lo; base < hi; base += 64) {
     const int r = base + l;
-    bool hit = false;
+    bool hit = false, dumb = false;
     if (r < hi) {
       const uint32_t k = D.rkey[r];
-      hit = all_itl & (D.rkind[r] == K_ITL);
+      dumb = D.rkind[r] == K_ITL;
+      hit = every | (all_itl & dumb);
       for (int i = 0; i < nc; i++) hit = hit | in_reach(c.d, k, cs[i]);
     }
     const uint64_t m = ballot(hit);
     if (!m) continue;
     if (first == hi) first = base + first_set(m);
-    if (hit) list[__popcll(m & ((1ull << l) - 1))] = r;
+    const bool hv = hit & !dumb, hd = hit & dumb;
+    const uint64_t mv = ballot(hv), md = ballot(hd);
+    if (hv) list[__popcll(mv & ((1ull << l) - 1))] = r;
+    if (hd) W.ibuf[ni + __popcll(md & ((1ull << l) - 1))] = r;
+    __threadfence_block();
     wave_sync();
-    eval_records(c, S, list, __popcll(m), nitl, n_eval);
+    ni += __popcll(md);
+    if (mv) eval_records(c, S, list, __popcll(mv), nitl, n_eval);
+    if (ni >= 64) {
+      eval_records(c, S, W.ibuf, 64, nitl, n_eval);
+      const int rest = ni - 64;
+      const int32_t v = l < rest ? W.ibuf[64 + l] : 0;
+      __threadfence_block();
+      wave_sync();
+      if (l < rest) W.ibuf[l] = v;
+      __threadfence_block();
+      wave_sync();
+      ni = rest;
+    }
   }
+  if (ni > 0) eval_records(c, S, W.ibuf, ni, nitl, n_eval);
   return uni_i(first);
 }
 
@@ -641,7 +854,8 @@ __global__ __launch_bounds__(64) void mix_kernel(const MixDesc* descs) {
   __shared__ int32_t list[64];
   __shared__ uint32_t cs[8];
   const int l = lane_id();
-  Ctx c{&D, Dims{D.d.X, D.d.Y, D.d.Z}, D.d};
+  __shared__ ItlWork iw;
+  Ctx c{&D, Dims{D.d.X, D.d.Y, D.d.Z}, D.d, &iw};
   MixSector S{D.rate, D.pvac, D.tmask, D.risk};
   MixState st = *D.st;
   if (l < 8) evo[l] = st.evo[l];
@@ -731,12 +945,7 @@ __global__ __launch_bounds__(64) void mix_kernel(const MixDesc* descs) {
         wave_sync();
         prefix_from(c, 0, nitl);
       }
-      for (int base = 0; base < nr; base += 64) {
-        const int cnt = min(64, nr - base);
-        if (l < cnt) list[l] = base + l;
-        wave_sync();
-        eval_records(c, S, list, cnt, nitl, st.n_eval);
-      }
+      eval_near(c, S, 0, nr, cs, 0, false, true, nitl, list, st.n_eval);
       sum = nr > 0 ? fold_range(S, 0, nr, 0.0) : 0.0;
       rebuild = false;
       if (D.rates_sector >= 0) {
@@ -1030,7 +1239,7 @@ __global__ __launch_bounds__(64) void mix_kernel(const MixDesc* descs) {
     // ---- the next event's rates: records within reach of a changed site,
     // and every dumbbell when a term of Edumb changed.
     const bool all_itl = jlo < nitl || nitl != st.nitl;
-    const int e_lo = eval_near(c, S, 0, nr, cs, nc, all_itl, nitl, list, st.n_eval);
+    const int e_lo = eval_near(c, S, 0, nr, cs, nc, all_itl, false, nitl, list, st.n_eval);
     const int c_lo = min(e_lo, edit_lo);
     // Records before c_lo kept their sums; when nothing at or after it is
     // left (the last record deleted), the total is the last kept sum.
