@@ -168,6 +168,7 @@ struct ItlWork {
   int32_t own[kItlPass];
   int32_t ncand[kItlPass];
   int32_t cand[kItlPass][kMaxCand];
+  uint32_t ckey[kItlPass][kMaxCand];
   int32_t aq[kMaxAff][64];  // per lane: list index of each changed term, ascending
   double at[kMaxAff][64];   // and the term on the lattice after the jump
   int32_t vlist[64], ilist[64];
@@ -207,9 +208,11 @@ __device__ void prefix_from(const Ctx& c, int j0, int n) {
   const int l = lane_id();
   j0 = uni_i(j0);
   double acc = uni_d(j0 > 0 ? D.ipre[j0 - 1] : 0.0);
+  double tn = j0 + l < n ? D.iterm[j0 + l] : 0.0;  // the next chunk loads while this one folds
   for (int base = j0; base < n; base += 64) {
     const int j = base + l;
-    const double t = j < n ? D.iterm[j] : 0.0;
+    const double t = tn;
+    if (base + 64 < n) tn = j + 64 < n ? D.iterm[j + 64] : 0.0;
     const int cnt = min(64, n - base);
     double mine = 0.0;
     for (int i = 0; i < cnt; i++) {
@@ -228,11 +231,18 @@ __device__ int refresh_near(const Ctx& c, const uint32_t* cs, int nc, int n) {
   const MixDesc& D = *c.D;
   const int l = lane_id();
   int jmin = n;
+  // Keys load four chunks ahead of the tests.
+  uint32_t k0 = l < n ? D.ikey[l] : 0u, k1 = 64 + l < n ? D.ikey[64 + l] : 0u;
+  uint32_t k2 = 128 + l < n ? D.ikey[128 + l] : 0u, k3 = 192 + l < n ? D.ikey[192 + l] : 0u;
   for (int base = 0; base < n; base += 64) {
     const int j = base + l;
+    const uint32_t k = k0;
+    k0 = k1;
+    k1 = k2;
+    k2 = k3;
+    k3 = j + 256 < n ? D.ikey[j + 256] : 0u;
     bool hit = false;
     if (j < n) {
-      const uint32_t k = D.ikey[j];
       for (int i = 0; i < nc; i++) hit = hit | term_reads(c.d, k, cs[i]);
     }
     if (hit) refresh_term(c, j);
@@ -401,7 +411,10 @@ __device__ void itl_pass(const Ctx& c, MixSector& S, const int32_t* rr, int m, i
       const int n0 = W.ncand[a];
       if (hit) {
         const int pos = n0 + __popcll(hm & ((1ull << l) - 1));
-        if (pos < kMaxCand) W.cand[a][pos] = q;
+        if (pos < kMaxCand) {
+          W.cand[a][pos] = q;
+          W.ckey[a][pos] = kq;
+        }
         if (kq == wr) W.own[a] = q;
       }
       __threadfence_block();
@@ -439,7 +452,7 @@ __device__ void itl_pass(const Ctx& c, MixSector& S, const int32_t* rr, int m, i
   for (int t = 0; t < cmax; t++) {
     if (act && !slow && t < nca) {
       const int q = W.cand[a][t];
-      const uint32_t kq = D.ikey[q];
+      const uint32_t kq = W.ckey[a][t];
       if (term_reads(c.d, kq, J.w) || term_reads(c.d, kq, J.wd)) {
         if (na < kMaxAff) {
           int qx, qy, qz;
@@ -459,8 +472,11 @@ __device__ void itl_pass(const Ctx& c, MixSector& S, const int32_t* rr, int m, i
   double sa = fast && qf > 0 ? D.ipre[qf - 1] : 0.0;
   int k = 0, nq = fast ? W.aq[0][l] : nitl;
   const int qmin = wave_min_i(qf);
+  // The next chunk's terms load while this one folds.
+  double tn = (qmin & ~63) + l < nitl ? D.iterm[(qmin & ~63) + l] : 0.0;
   for (int base = qmin & ~63; base < nitl; base += 64) {
-    const double t = base + l < nitl ? D.iterm[base + l] : 0.0;
+    const double t = tn;
+    if (base + 64 < nitl) tn = base + 64 + l < nitl ? D.iterm[base + 64 + l] : 0.0;
     const int cnt = min(64, nitl - base);
     for (int ii = 0; ii < cnt; ii++) {
       const int q = base + ii;
@@ -741,12 +757,26 @@ __device__ int eval_near(const Ctx& c, MixSector& S, int lo, int hi, const uint3
   ItlWork& W = *c.W;
   const int l = lane_id();
   int first = hi, ni = 0;
+  // Keys and kinds load four chunks ahead of the tests.
+  uint32_t k0 = lo + l < hi ? D.rkey[lo + l] : 0u, k1 = lo + 64 + l < hi ? D.rkey[lo + 64 + l] : 0u;
+  uint32_t k2 = lo + 128 + l < hi ? D.rkey[lo + 128 + l] : 0u, k3 = lo + 192 + l < hi ? D.rkey[lo + 192 + l] : 0u;
+  uint8_t d0 = lo + l < hi ? D.rkind[lo + l] : 0, d1 = lo + 64 + l < hi ? D.rkind[lo + 64 + l] : 0;
+  uint8_t d2 = lo + 128 + l < hi ? D.rkind[lo + 128 + l] : 0, d3 = lo + 192 + l < hi ? D.rkind[lo + 192 + l] : 0;
   for (int base = lo; base < hi; base += 64) {
     const int r = base + l;
+    const uint32_t k = k0;
+    const uint8_t kd = d0;
+    k0 = k1;
+    k1 = k2;
+    k2 = k3;
+    d0 = d1;
+    d1 = d2;
+    d2 = d3;
+    k3 = r + 256 < hi ? D.rkey[r + 256] : 0u;
+    d3 = r + 256 < hi ? D.rkind[r + 256] : 0;
     bool hit = false, dumb = false;
     if (r < hi) {
-      const uint32_t k = D.rkey[r];
-      dumb = D.rkind[r] == K_ITL;
+      dumb = kd == K_ITL;
       hit = every | (all_itl & dumb);
       for (int i = 0; i < nc; i++) hit = hit | in_reach(c.d, k, cs[i]);
     }
