@@ -18,8 +18,9 @@ dumbbell-rate step (--no-itl):
            advances every replica by --batch-events events.
   served   the serving loop on a Poisson stream of --jobs jobs (edge U{32..320},
            vacancies U[1e-4,5e-4], targets log-uniform in [1e5,1e7] events scaled
-           by --serve-scale), admitted by a 250 GB HBM budget; reports
-           throughput and p50/p99 completion latency.
+           by --serve-scale; a tenth of them also carry dumbbells at 1e-5),
+           admitted by a 250 GB HBM budget; reports throughput and p50/p99
+           completion latency, overall and for the dumbbell jobs.
   plain_engines  four plain FP64 engines at once (the 256^3 workload, seeds
            differing), one host thread and stream each: aggregate events/s of
            bit-exact trajectories.
@@ -215,57 +216,86 @@ def batched_leg(args, plain_value):
 
 
 def served_leg(args, batched_value):
-    """Serving loop on a Poisson job stream: throughput and completion latency."""
+    """Serving loop on a Poisson job stream: throughput and completion latency.
+    A fraction --serve-itl-frac of the jobs also carry self-interstitial
+    dumbbells at 1e-5 of their sites (akmc_server_submit_itl)."""
     import random
     import akmc
     from akmc import workloads
-    jobs = workloads.jobs(args.jobs, ev_lo=1e5 * args.serve_scale, ev_hi=1e7 * args.serve_scale)
-    mean_events = sum(t for _, t in jobs) / len(jobs)
+    lo, hi = 1e5 * args.serve_scale, 1e7 * args.serve_scale
+    jobs = workloads.jobs_with_itl(args.jobs, ev_lo=lo, ev_hi=hi, itl_frac=args.serve_itl_frac)
+    mean_events = sum(t for _, t, _ in jobs) / len(jobs)
+
+    def submit(srv, cfg, target, n_itl):
+        return srv.submit_itl(cfg, n_itl, target) if n_itl else srv.submit(cfg, target)
+
     capacity = None
     if not args.serve_rate:
-        # Capacity: a separate stream of jobs of the same law and size, all
-        # submitted at once; the offered rate is then 70% of the jobs/s it
-        # sustains.  (A burst much smaller than the stream measures the
-        # longest job's serial event chain instead of the engine's rate.)
-        probe = workloads.jobs(max(200, args.jobs), base=0x5EED0003, ev_lo=1e5 * args.serve_scale,
-                               ev_hi=1e7 * args.serve_scale)
+        # Capacity: a burst of jobs of the same law, all submitted at once.
+        # Its event rate while the server is saturated (until half of the
+        # jobs are done, at most --probe-seconds; the tail after that is the
+        # longest jobs' serial event chains) divided by the mean target gives
+        # the jobs/s the server sustains, and the offered rate is 70% of it.
+        probe = workloads.jobs_with_itl(args.probe_jobs, base=0x5EED0003, ev_lo=lo, ev_hi=hi,
+                                        itl_frac=args.serve_itl_frac)
         srv = akmc.Server(device=0, hbm_budget=int(250e9))
         t0 = srv.clock()
-        for cfg, target in probe:
-            srv.submit(cfg, target)
-        srv.wait_idle(0)
-        capacity = len(probe) / (srv.clock() - t0)
+        for cfg, target, n_itl in probe:
+            submit(srv, cfg, target, n_itl)
+        while True:
+            time.sleep(0.05)
+            st = srv.stats()
+            if st["done"] + st["failed"] >= len(probe) // 2 or srv.clock() - t0 >= args.probe_seconds:
+                break
+        probe_events = st["events"]
+        probe_span = srv.clock() - t0
         srv.close()
-        print("bench: serving capacity %.0f jobs/s" % capacity, file=sys.stderr, flush=True)
+        probe_mean = sum(t for _, t, _ in probe) / len(probe)
+        capacity = probe_events / probe_span / probe_mean
+        print("bench: serving capacity %.0f jobs/s (%.3g events/s over %.1f s)" %
+              (capacity, probe_events / probe_span, probe_span), file=sys.stderr, flush=True)
     rate = args.serve_rate or 0.7 * capacity
     srv = akmc.Server(device=0, hbm_budget=int(250e9))
     rng = random.Random(12345)
     t_start = srv.clock()
     t_next = t_start
     ids = []
-    for cfg, target in jobs:
+    for cfg, target, n_itl in jobs:
         t_next += rng.expovariate(rate)
         while True:
             now = srv.clock()
             if now >= t_next:
                 break
             time.sleep(min(0.002, t_next - now))
-        ids.append((srv.submit(cfg, target), target))
+        ids.append((submit(srv, cfg, target, n_itl), target, n_itl))
     srv.wait_idle(0)
     st = srv.stats()
-    infos = [srv.info(j) for j, _ in ids]
+    infos = [srv.info(j) for j, _, _ in ids]
     srv.close()
-    lat = sorted(i["t_done"] - i["t_submit"] for i in infos)
-    q = lambda p: lat[min(len(lat) - 1, int(p * len(lat)))]
+
+    def quantiles(sel):
+        lat = sorted(i["t_done"] - i["t_submit"] for i in sel)
+        if not lat:
+            return None, None, None
+        q = lambda p: lat[min(len(lat) - 1, int(p * len(lat)))]
+        return q(0.50), q(0.99), lat[-1]
+
+    p50, p99, pmax = quantiles(infos)
+    dumb = [i for i, (_, _, n) in zip(infos, ids) if n]
+    d50, d99, dmax = quantiles(dumb)
     span = max(i["t_done"] for i in infos) - t_start
     ev = sum(i["events"] for i in infos)
     return {"value": ev / span, "unit": "events/s", "jobs": len(ids), "offered_jobs_per_s": rate,
             "capacity_jobs_per_s": capacity,
             "mean_target_events": mean_events, "target_scale": args.serve_scale,
-            "p50_latency_s": q(0.50), "p99_latency_s": q(0.99), "max_latency_s": lat[-1],
+            "p50_latency_s": p50, "p99_latency_s": p99, "max_latency_s": pmax,
+            "dumbbell_jobs": len(dumb), "dumbbell_events": sum(i["events"] for i in dumb),
+            "dumbbell_p50_latency_s": d50, "dumbbell_p99_latency_s": d99, "dumbbell_max_latency_s": dmax,
             "makespan_s": span, "gpu_busy_frac": st["busy_seconds"] / span, "launches": st["launches"],
             "hbm_peak_bytes": st["bytes_peak"], "failed": st["failed"],
-            "data": "synthetic jobs, vacancy hops only (the reference defines no dumbbell event loop; dumbbell rates are the itl leg)"}
+            "data": "synthetic jobs (edge U{32..320}, vacancies U[1e-4,5e-4], T U[500,700] K, targets "
+                    "log-uniform in [%g, %g] events); %d of them also carry dumbbells at 1e-5 of the sites, "
+                    "served by the vacancy+dumbbell engine" % (lo, hi, len(dumb))}
 
 
 CONFIG0 = dict(box=(40, 40, 40), alloy=[("Fe", 9666), ("Cu", 134), ("Mn", 130), ("Ni", 70)], va_count=26,
@@ -427,7 +457,11 @@ def main():
     ap.add_argument("--jobs", type=int, default=4000)
     ap.add_argument("--serve-scale", type=float, default=0.01,
                     help="scale of the job targets (1 = the full 1e5..1e7 events)")
-    ap.add_argument("--serve-rate", type=float, default=0.0, help="offered jobs/s (0: 70%% of batched capacity)")
+    ap.add_argument("--serve-rate", type=float, default=0.0, help="offered jobs/s (0: 70%% of the probed capacity)")
+    ap.add_argument("--serve-itl-frac", type=float, default=0.1,
+                    help="fraction of served jobs that also carry dumbbells at 1e-5 of the sites")
+    ap.add_argument("--probe-jobs", type=int, default=400, help="jobs in the capacity probe burst")
+    ap.add_argument("--probe-seconds", type=float, default=20.0, help="longest capacity probe")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: runs on one GPU only (one lattice, batch or server per GPU; sharding replicas, "

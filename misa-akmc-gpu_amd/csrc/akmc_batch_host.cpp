@@ -1047,6 +1047,8 @@ struct Job {
   int32_t itl_atom = 1;
   double ef110 = 3.53;
   std::vector<uint16_t> result_types;
+  double ev_per_tick = 0;       // events per 100 MHz tick of its wavefront, from its last launch
+  int64_t ticks = 0;            // its wavefront's device time so far
 };
 
 }  // namespace
@@ -1075,6 +1077,12 @@ struct akmc_server {
   hipStream_t mix_stream = nullptr;
   std::vector<Job*> mix_job;
   std::vector<int32_t> mix_free;
+  // The pool's launch runs beside the replica slices and is collected once
+  // it has ended (mix_done), so a slow lattice never holds the others back.
+  hipEvent_t mix_done = nullptr;
+  bool mix_inflight = false;
+  std::vector<int32_t> mix_launched;
+  double t_mix_launch = 0;
   bool items_dirty = false;
   int error = AKMC_OK;
   std::string error_msg;
@@ -1134,7 +1142,7 @@ int admit(akmc_server* s, bool* changed) {
   std::lock_guard<std::mutex> lk(s->mu);
   for (Job* j : batch) {
     if (j->mix) {
-      akmc_mixpool::pool_setup_done(s->mix, j->slot);
+      akmc_mixpool::pool_setup_done(s->mix, j->slot, s->init_stream);
       s->mix_job[j->slot] = j;
       j->status = AKMC_JOB_RUNNING;
       j->t_admit = t;
@@ -1161,7 +1169,7 @@ int retire(akmc_server* s, Job* j, int32_t rep_status) {
                                                  (int64_t)j->result_types.size());
       if (rc) return rc;
     }
-    const int rc = akmc_mixpool::pool_remove(s->mix, j->slot);
+    const int rc = akmc_mixpool::pool_remove(s->mix, j->slot, s->mix_stream);
     if (rc) return rc;
   } else {
     if (s->opt.keep_results) {
@@ -1191,28 +1199,89 @@ int retire(akmc_server* s, Job* j, int32_t rep_status) {
   return AKMC_OK;
 }
 
+// Launches the pool over the live dumbbell jobs.  Each lattice runs up to
+// the events that its measured rate fits into a few replica slices (at least
+// 16, at most slice_events, never past its target), so that one launch takes
+// about that long whatever the lattices' sizes.
+int mix_launch(akmc_server* s) {
+  std::vector<int32_t> mlive;
+  std::vector<int64_t> mlim;
+  const double ticks = std::max(1e5, 5e8 * s->opt.slice_seconds);  // 100 MHz ticks
+  const int64_t cap = std::max<int64_t>(16, s->opt.slice_events);
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (int32_t k = 0; k < (int32_t)s->mix_job.size(); k++)
+      if (Job* j = s->mix_job[k]) {
+        const int64_t want = j->ev_per_tick > 0 ? (int64_t)(j->ev_per_tick * ticks) : 256;
+        mlive.push_back(k);
+        mlim.push_back(std::min(j->target, j->events + std::min(cap, std::max<int64_t>(16, want))));
+      }
+  }
+  if (mlive.empty()) return AKMC_OK;
+  int rc = akmc_mixpool::pool_launch(s->mix, mlive, mlim, s->mix_stream);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(s->mix_done, s->mix_stream));
+  s->mix_inflight = true;
+  s->mix_launched = mlive;
+  s->t_mix_launch = s->now();
+  return AKMC_OK;
+}
+
+// Takes the results of the ended pool launch: progress, rates, retirement.
+int mix_collect(akmc_server* s) {
+  s->mix_inflight = false;
+  int64_t mev = 0;
+  for (int32_t k : s->mix_launched) {
+    Job* j = s->mix_job[k];
+    const akmc::mix::MixState& st = akmc_mixpool::pool_state(s->mix, k);
+    const int64_t dt = st.busy_ticks - j->ticks;
+    if (dt > 0 && st.events > j->events) j->ev_per_tick = (double)(st.events - j->events) / (double)dt;
+    j->ticks = st.busy_ticks;
+    mev += st.events - j->events;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      j->events = st.events;
+    }
+    int32_t rs = AKMC_REP_RUNNING;
+    if (st.status == AKMC_RUN_TIME_DONE) rs = AKMC_REP_TIME_DONE;
+    else if (st.events >= j->target) rs = AKMC_REP_TARGET_DONE;
+    else if (st.status < 0) rs = st.status;
+    if (rs != AKMC_REP_RUNNING) {
+      const int rc = retire(s, j, rs);
+      if (rc) return rc;
+    }
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->st.launches++;
+  s->st.events += mev;
+  return AKMC_OK;
+}
+
 // One step of the request loop: launch a slice over the live jobs, admit
 // pending jobs while it runs (their set-up overlaps the device work), then
-// collect the slice and retire the jobs that finished.
+// collect the slice and retire the jobs that finished.  The dumbbell jobs'
+// pool launch runs on its own stream beside the slices: collected when it
+// has ended, then launched again.
 int serve_step(akmc_server* s) {
-  std::vector<int32_t> live, mlive;
-  std::vector<int64_t> mlim;
+  std::vector<int32_t> live;
   int32_t hi = 0;
   {
     std::lock_guard<std::mutex> lk(s->mu);
     for (int32_t k = 0; k < (int32_t)s->slot_job.size(); k++)
       if (s->slot_job[k]) { live.push_back(k); hi = k + 1; }
-    for (int32_t k = 0; k < (int32_t)s->mix_job.size(); k++)
-      if (Job* j = s->mix_job[k]) {
-        mlive.push_back(k);
-        // Each dumbbell job advances by at most one slice of events per step.
-        mlim.push_back(std::min(j->target, j->events + std::max<int64_t>(1, s->opt.slice_events)));
-      }
   }
   int rc;
-  const double t_m = s->now();
-  if (!mlive.empty()) {
-    rc = akmc_mixpool::pool_launch(s->mix, mlive, mlim, s->mix_stream);
+  if (s->mix_inflight) {
+    const hipError_t q = hipEventQuery(s->mix_done);
+    if (q == hipSuccess) {
+      rc = mix_collect(s);
+      if (rc) return rc;
+    } else if (q != hipErrorNotReady) {
+      return fail(AKMC_ERR_HIP, "dumbbell jobs' launch: %s", hipGetErrorString(q));
+    }
+  }
+  if (!s->mix_inflight) {
+    rc = mix_launch(s);
     if (rc) return rc;
   }
   if (s->items_dirty) {
@@ -1230,32 +1299,18 @@ int serve_step(akmc_server* s) {
   rc = admit(s, &changed);
   if (rc) return rc;
   if (changed) s->items_dirty = true;
-  if (!mlive.empty()) {
-    HIP_TRY(hipStreamSynchronize(s->mix_stream));
-    int64_t mev = 0;
-    for (int32_t k : mlive) {
-      Job* j = s->mix_job[k];
-      const akmc::mix::MixState& st = akmc_mixpool::pool_state(s->mix, k);
-      mev += st.events - j->events;
-      {
-        std::lock_guard<std::mutex> lk(s->mu);
-        j->events = st.events;
-      }
-      int32_t rs = AKMC_REP_RUNNING;
-      if (st.status == AKMC_RUN_TIME_DONE) rs = AKMC_REP_TIME_DONE;
-      else if (st.events >= j->target) rs = AKMC_REP_TARGET_DONE;
-      else if (st.status < 0) rs = st.status;
-      if (rs != AKMC_REP_RUNNING) {
-        rc = retire(s, j, rs);
-        if (rc) return rc;
-      }
+  if (!launched) {
+    // Only dumbbell jobs run: wait for their launch instead of spinning.
+    if (s->mix_inflight) {
+      HIP_TRY(hipEventSynchronize(s->mix_done));
+      const double t_m = s->t_mix_launch;
+      rc = mix_collect(s);
+      if (rc) return rc;
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->st.busy_seconds += s->now() - std::max(t_m, t_a);
     }
-    std::lock_guard<std::mutex> lk(s->mu);
-    s->st.launches++;
-    s->st.events += mev;
-    if (!launched) s->st.busy_seconds += s->now() - t_m;
+    return AKMC_OK;
   }
-  if (!launched) return AKMC_OK;
   HIP_TRY(hipStreamSynchronize(s->R.s0));
   const double t_b = s->now();
   HIP_TRY(hipMemcpy(s->h_state.data(), s->R.d_state, hi * sizeof(akmc::ReplicaState), hipMemcpyDeviceToHost));
@@ -1340,6 +1395,7 @@ int akmc_server_create(int device, const akmc_server_options* o, akmc_server** o
   s->R.use_graph = false;  // the live set changes between launches
   HIP_TRY(hipStreamCreateWithFlags(&s->init_stream, hipStreamNonBlocking));
   HIP_TRY(hipStreamCreateWithFlags(&s->mix_stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreateWithFlags(&s->mix_done, hipEventDisableTiming));
   rc = akmc_mixpool::pool_create(device, opt.max_live, &s->mix);
   if (rc) return rc;
   s->mix_job.assign(opt.max_live, nullptr);
@@ -1489,6 +1545,7 @@ void akmc_server_destroy(akmc_server* s) {
   (void)hipDeviceSynchronize();
   s->arena.destroy();
   akmc_mixpool::pool_destroy(s->mix);
+  if (s->mix_done) (void)hipEventDestroy(s->mix_done);
   if (s->mix_stream) (void)hipStreamDestroy(s->mix_stream);
   if (s->init_stream) (void)hipStreamDestroy(s->init_stream);
   s->R.destroy();

@@ -28,6 +28,7 @@ struct MixState {
   int64_t n_eval;          // record evaluations
   int64_t n_itl_events;    // dumbbell transitions
   int64_t n_skipped;       // time steps taken in one go (no event could happen in them)
+  int64_t busy_ticks;      // device time of this lattice's wavefront in its launches (100 MHz ticks)
   uint64_t rng[4];         // xoshiro256** state (device streams)
 };
 

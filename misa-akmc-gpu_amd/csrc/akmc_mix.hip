@@ -880,6 +880,7 @@ struct Draws {
 
 __global__ __launch_bounds__(64) void mix_kernel(const MixDesc* descs) {
   const MixDesc& D = descs[blockIdx.x];
+  const uint64_t tick0 = __builtin_amdgcn_s_memrealtime();
   __shared__ double evo[8];
   __shared__ int32_t list[64];
   __shared__ uint32_t cs[8];
@@ -1279,6 +1280,7 @@ __global__ __launch_bounds__(64) void mix_kernel(const MixDesc* descs) {
     st.nitl = nitl;  // Edumb terms are current again
   }
   if (l == 0 && D.rates_sector < 0) {
+    st.busy_ticks += (int64_t)(__builtin_amdgcn_s_memrealtime() - tick0);
     st.status = status;
     st.nvac = nvac;
     st.nitl = nitl;

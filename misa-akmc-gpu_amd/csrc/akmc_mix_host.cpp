@@ -32,6 +32,7 @@ bool valid_dumbbell(int t) {
 
 struct MixLat {
   akmc_mix_params p{};
+  hipStream_t mem_stream = nullptr;  // set: device buffers are stream-ordered (hipMallocAsync / hipFreeAsync)
   akmc::BoxDims d{};
   int64_t nsites = 0;
   int32_t vcap = 0, icap = 0, rcap = 0;
@@ -79,11 +80,13 @@ struct MixLat {
   int64_t bytes = 0;
 };
 
-void free_lat(MixLat& L) {
+// Frees a lattice's device buffers; stream-ordered ones on `fs` (their own
+// allocation stream if null), after the work queued there.
+void free_lat(MixLat& L, hipStream_t fs = nullptr) {
   void* ptrs[] = {L.types, L.vkey, L.ikey, L.io, L.iav, L.iterm, L.ipre, L.rkey, L.rkind, L.rslot, L.rate,
                   L.pvac, L.tmask, L.risk, L.d_u, L.d_l, L.d_rec, L.d_log, L.d_cum, L.d_tval, L.d_idx};
   for (void* q : ptrs)
-    if (q) (void)hipFree(q);
+    if (q) (void)(L.mem_stream ? hipFreeAsync(q, fs ? fs : L.mem_stream) : hipFree(q));
 }
 
 // Device bytes of a lattice: types, lists and sector records.
@@ -92,8 +95,9 @@ int64_t lat_bytes(int64_t nsites, int64_t vcap, int64_t icap) {
 }
 
 template <class T>
-hipError_t dalloc(T** p, int64_t n) {
-  return hipMalloc((void**)p, (size_t)std::max<int64_t>(n, 1) * sizeof(T));
+hipError_t dalloc(T** p, int64_t n, hipStream_t st = nullptr) {
+  const size_t bytes = (size_t)std::max<int64_t>(n, 1) * sizeof(T);
+  return st ? hipMallocAsync((void**)p, bytes, st) : hipMalloc((void**)p, bytes);
 }
 
 }  // namespace
@@ -174,24 +178,24 @@ akmc::mix::MixDesc make_desc(akmc_mix* m, int i) { return desc_of(*m->lat[(size_
 // Device buffers of a lattice (lists of the given capacities, host streams
 // when the lattice draws from the host).
 int alloc_lat(MixLat& L) {
-  HIP_TRY(dalloc(&L.types, L.nsites));
-  HIP_TRY(dalloc(&L.vkey, L.vcap));
-  HIP_TRY(dalloc(&L.ikey, L.icap));
-  HIP_TRY(dalloc(&L.io, L.icap));
-  HIP_TRY(dalloc(&L.iav, L.icap));
-  HIP_TRY(dalloc(&L.iterm, L.icap));
-  HIP_TRY(dalloc(&L.ipre, L.icap));
-  HIP_TRY(dalloc(&L.rkey, L.rcap));
-  HIP_TRY(dalloc(&L.rkind, L.rcap));
-  HIP_TRY(dalloc(&L.rslot, L.rcap));
-  HIP_TRY(dalloc(&L.rate, 8 * (int64_t)L.rcap));
-  HIP_TRY(dalloc(&L.pvac, L.rcap));
-  HIP_TRY(dalloc(&L.tmask, L.rcap));
-  HIP_TRY(dalloc(&L.risk, L.rcap));
+  HIP_TRY(dalloc(&L.types, L.nsites, L.mem_stream));
+  HIP_TRY(dalloc(&L.vkey, L.vcap, L.mem_stream));
+  HIP_TRY(dalloc(&L.ikey, L.icap, L.mem_stream));
+  HIP_TRY(dalloc(&L.io, L.icap, L.mem_stream));
+  HIP_TRY(dalloc(&L.iav, L.icap, L.mem_stream));
+  HIP_TRY(dalloc(&L.iterm, L.icap, L.mem_stream));
+  HIP_TRY(dalloc(&L.ipre, L.icap, L.mem_stream));
+  HIP_TRY(dalloc(&L.rkey, L.rcap, L.mem_stream));
+  HIP_TRY(dalloc(&L.rkind, L.rcap, L.mem_stream));
+  HIP_TRY(dalloc(&L.rslot, L.rcap, L.mem_stream));
+  HIP_TRY(dalloc(&L.rate, 8 * (int64_t)L.rcap, L.mem_stream));
+  HIP_TRY(dalloc(&L.pvac, L.rcap, L.mem_stream));
+  HIP_TRY(dalloc(&L.tmask, L.rcap, L.mem_stream));
+  HIP_TRY(dalloc(&L.risk, L.rcap, L.mem_stream));
   if (L.p.device_rng == 0) {
-    HIP_TRY(dalloc(&L.d_u, kChunk));
-    HIP_TRY(dalloc(&L.d_l, kChunk));
-    HIP_TRY(dalloc(&L.d_rec, kRecWindow));
+    HIP_TRY(dalloc(&L.d_u, kChunk, L.mem_stream));
+    HIP_TRY(dalloc(&L.d_l, kChunk, L.mem_stream));
+    HIP_TRY(dalloc(&L.d_rec, kRecWindow, L.mem_stream));
     L.h_u.resize(kChunk);
     L.h_l.resize(kChunk);
     // r::rng is default-constructed (rand_generators.cpp:8); SubLattice
@@ -634,7 +638,9 @@ struct Pool {
   int32_t nslots = 0;
   akmc::mix::MixState* d_state = nullptr;
   akmc::mix::MixDesc* d_desc = nullptr;
-  std::vector<akmc::mix::MixState> h_state;
+  // Pinned host copies, so that a launch's copies stay asynchronous.
+  akmc::mix::MixState* h_state = nullptr;
+  akmc::mix::MixDesc* h_desc = nullptr;
   std::vector<std::unique_ptr<MixLat>> slot;
   int32_t hi = 0;  // states copied back: slots [0, hi)
 };
@@ -647,7 +653,9 @@ int pool_create(int device, int32_t slots, Pool** out) {
   HIP_TRY(hipSetDevice(device));
   HIP_TRY(hipMalloc(&p->d_state, sizeof(akmc::mix::MixState) * (size_t)slots));
   HIP_TRY(hipMalloc(&p->d_desc, sizeof(akmc::mix::MixDesc) * (size_t)slots));
-  p->h_state.resize((size_t)slots);
+  HIP_TRY(hipHostMalloc((void**)&p->h_state, sizeof(akmc::mix::MixState) * (size_t)slots, hipHostMallocDefault));
+  HIP_TRY(hipHostMalloc((void**)&p->h_desc, sizeof(akmc::mix::MixDesc) * (size_t)slots, hipHostMallocDefault));
+  memset(p->h_state, 0, sizeof(akmc::mix::MixState) * (size_t)slots);
   p->slot.resize((size_t)slots);
   *out = p.release();
   return AKMC_OK;
@@ -660,6 +668,8 @@ void pool_destroy(Pool* p) {
     if (L) free_lat(*L);
   (void)hipFree(p->d_state);
   (void)hipFree(p->d_desc);
+  if (p->h_state) (void)hipHostFree(p->h_state);
+  if (p->h_desc) (void)hipHostFree(p->h_desc);
   delete p;
 }
 
@@ -735,6 +745,7 @@ int pool_add_generated(Pool* p, int32_t slot, const akmc_params& comp, int64_t n
   L->vcap = (int32_t)std::max<int64_t>(1, (int64_t)vidx.size());
   L->icap = (int32_t)std::max<int64_t>(1, n_itl);
   L->rcap = L->vcap + L->icap;
+  L->mem_stream = st;
   MixLat* Lp = L.get();
   p->slot[(size_t)slot] = std::move(L);
   int rc = alloc_lat(*Lp);
@@ -746,9 +757,9 @@ int pool_add_generated(Pool* p, int32_t slot, const akmc_params& comp, int64_t n
     Lp->h_ikey.push_back(akmc::key_of_index(Lp->d, q.first));
     Lp->h_io.push_back(q.second);
   }
-  HIP_TRY(dalloc(&Lp->d_cum, nt));
-  HIP_TRY(dalloc(&Lp->d_tval, nt));
-  HIP_TRY(dalloc(&Lp->d_idx, (int64_t)Lp->h_idx.size()));
+  HIP_TRY(dalloc(&Lp->d_cum, nt, st));
+  HIP_TRY(dalloc(&Lp->d_tval, nt, st));
+  HIP_TRY(dalloc(&Lp->d_idx, (int64_t)Lp->h_idx.size(), st));
   HIP_TRY(hipMemcpyAsync(Lp->d_cum, Lp->h_cum, sizeof(uint32_t) * nt, hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(Lp->d_tval, Lp->h_tval, sizeof(uint16_t) * nt, hipMemcpyHostToDevice, st));
   if (!Lp->h_idx.empty())
@@ -771,12 +782,12 @@ int pool_add_generated(Pool* p, int32_t slot, const akmc_params& comp, int64_t n
   return AKMC_OK;
 }
 
-void pool_setup_done(Pool* p, int32_t slot) {
+void pool_setup_done(Pool* p, int32_t slot, hipStream_t st) {
   MixLat* L = p->slot[(size_t)slot].get();
   if (!L) return;
   void* q[] = {L->d_cum, L->d_tval, L->d_idx};
   for (void* v : q)
-    if (v) (void)hipFree(v);
+    if (v) (void)(L->mem_stream ? hipFreeAsync(v, st) : hipFree(v));
   L->d_cum = nullptr;
   L->d_tval = nullptr;
   L->d_idx = nullptr;
@@ -786,27 +797,27 @@ void pool_setup_done(Pool* p, int32_t slot) {
   std::vector<int8_t>().swap(L->h_io);
 }
 
-int pool_remove(Pool* p, int32_t slot) {
+int pool_remove(Pool* p, int32_t slot, hipStream_t st) {
   if (!p || slot < 0 || slot >= p->nslots || !p->slot[(size_t)slot]) return fail(AKMC_ERR_INVALID, "bad pool slot");
-  free_lat(*p->slot[(size_t)slot]);
+  free_lat(*p->slot[(size_t)slot], st);
   p->slot[(size_t)slot].reset();
   return AKMC_OK;
 }
 
 int pool_launch(Pool* p, const std::vector<int32_t>& slots, const std::vector<int64_t>& limits, hipStream_t st) {
   if (slots.empty()) return AKMC_OK;
-  std::vector<akmc::mix::MixDesc> ds;
+  if ((int64_t)slots.size() > p->nslots) return fail(AKMC_ERR_INVALID, "more lattices than pool slots");
   int32_t hi = 0;
   for (size_t k = 0; k < slots.size(); k++) {
     MixLat& L = *p->slot[(size_t)slots[k]];
     L.target = limits[k];
-    ds.push_back(desc_of(L, p->d_state + slots[k]));
+    p->h_desc[k] = desc_of(L, p->d_state + slots[k]);
     hi = std::max(hi, slots[k] + 1);
   }
-  HIP_TRY(hipMemcpyAsync(p->d_desc, ds.data(), ds.size() * sizeof(ds[0]), hipMemcpyHostToDevice, st));
-  const int rc = akmc_mix_launch(p->d_desc, (int32_t)ds.size(), st);
+  HIP_TRY(hipMemcpyAsync(p->d_desc, p->h_desc, slots.size() * sizeof(akmc::mix::MixDesc), hipMemcpyHostToDevice, st));
+  const int rc = akmc_mix_launch(p->d_desc, (int32_t)slots.size(), st);
   if (rc) return fail(AKMC_ERR_HIP, "mix kernel launch failed");
-  HIP_TRY(hipMemcpyAsync(p->h_state.data(), p->d_state, (size_t)hi * sizeof(akmc::mix::MixState),
+  HIP_TRY(hipMemcpyAsync(p->h_state, p->d_state, (size_t)hi * sizeof(akmc::mix::MixState),
                          hipMemcpyDeviceToHost, st));
   return AKMC_OK;
 }

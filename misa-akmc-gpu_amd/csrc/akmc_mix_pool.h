@@ -29,14 +29,17 @@ int pool_add(Pool* p, int32_t slot, const akmc_mix_params& prm, const uint16_t* 
 // pool_setup_done).  Device streams (xoshiro) drive its events.
 int pool_add_generated(Pool* p, int32_t slot, const akmc_params& comp, int64_t n_itl, int32_t itl_atom, double ef110,
                        hipStream_t st);
-void pool_setup_done(Pool* p, int32_t slot);
-int pool_remove(Pool* p, int32_t slot);
-// Device bytes a generated lattice of these parameters holds.
+void pool_setup_done(Pool* p, int32_t slot, hipStream_t st);
+int pool_remove(Pool* p, int32_t slot, hipStream_t st);
+// Device bytes a generated lattice of these parameters holds.  A generated
+// lattice's device buffers are stream-ordered: allocated on the stream of
+// pool_add_generated, freed on the stream given to pool_setup_done (set-up
+// buffers) and pool_remove (after the work queued there).
 int64_t pool_bytes(const akmc_params& comp, int64_t n_itl);
 
 // Launches the given slots on `st`, each up to events limits[i] (absolute),
-// and enqueues the copy of their states back; call pool_states_ready after
-// the stream is synchronized.
+// and enqueues the copy of their states back (pinned, asynchronous); read
+// pool_state once the stream has passed that copy.
 int pool_launch(Pool* p, const std::vector<int32_t>& slots, const std::vector<int64_t>& limits, hipStream_t st);
 const akmc::mix::MixState& pool_state(Pool* p, int32_t slot);
 int pool_download(Pool* p, int32_t slot, uint16_t* types, int64_t n_sites);
