@@ -6,6 +6,7 @@ sector clocks and the partial sector time.  Run from the repository root:
     python3 tests/golden/make_golden.py cfg0   # hashes_cfg0_40cube.npz (1e5 events, a few minutes)
     python3 tests/golden/make_golden.py itl    # itl_dense_6x4x9.npz (dumbbell rates)
     python3 tests/golden/make_golden.py cfg1 [resume_dir]   # hashes_cfg1_256cube.npz (1e5 events of 256^3, ~1 h)
+    python3 tests/golden/make_golden.py cfg2   # cfg2_512cube.npz (512^3: sector rates, 200 events)
 """
 import hashlib
 import json
@@ -99,6 +100,36 @@ def cfg1_hashes(resume_dir=None):
                 os.replace(dg_path + ".tmp.npz", dg_path)
 
 
+# Config 2's lattice (512^3 cells, Fe-1.34Cu-0.70Ni-1.30Mn, 2e-4 vacancies,
+# 563 K): sector 0's rates (SHA-256 of all rows, the first 512 rows in full)
+# and the first 200 events' site pairs, occupancy and clocks.
+CFG2_EVENTS, CFG2_ROWS = 200, 512
+
+
+def cfg2_fixture():
+    sys.path.insert(0, os.path.join(HERE, "..", "..", "misa-akmc-gpu_amd"))
+    from akmc import workloads
+    cfg = workloads.precision_512(0)
+    r = refcpu.RefCpu(cfg, list_sweep=True)
+    print("cfg2 created", flush=True)
+    sites, rates, total = r.sector_rates(0, cap=1 << 15)
+    print("cfg2 sector 0:", sites.size, "vacancies", flush=True)
+    rc, lf, lt = r.run(CFG2_EVENTS, log=True)
+    assert rc == 1 and r.state()["events"] == CFG2_EVENTS
+    st = r.state()
+    pairs = np.stack([lf, lt], axis=1).astype(np.int64)
+    np.savez_compressed(os.path.join(HERE, "cfg2_512cube.npz"), cfg=json.dumps(cfg), sector=np.int64(0),
+                        n_rows=np.int64(sites.size), rates_sha256=np.frombuffer(
+                            hashlib.sha256(np.ascontiguousarray(rates).tobytes()).digest(), dtype=np.uint8),
+                        sites_sha256_rows=np.frombuffer(hashlib.sha256(sites.tobytes()).digest(), dtype=np.uint8),
+                        head_sites=sites[:CFG2_ROWS], head_rates=rates[:CFG2_ROWS], total=np.float64(total),
+                        events=np.int64(CFG2_EVENTS), log=pairs,
+                        final_sha256=np.frombuffer(hashlib.sha256(r.sites().tobytes()).digest(), dtype=np.uint8),
+                        evolution=st["evolution"], sector_time=np.float64(st["sector_time"]),
+                        step=np.int64(st["step"]), sect=np.int64(st["sect"]))
+    print("cfg2 written", flush=True)
+
+
 def itl_rates():
     """Dumbbell rates of a dense typed lattice (6 x 4 x 9 cells, 10 % dumbbell
     sites of every kind and orientation, Cu/Ni/Mn atoms and vacancies) from
@@ -129,6 +160,9 @@ def main():
         return
     if len(sys.argv) > 1 and sys.argv[1] == "cfg1":
         cfg1_hashes(sys.argv[2] if len(sys.argv) > 2 else None)
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "cfg2":
+        cfg2_fixture()
         return
     if len(sys.argv) > 1 and sys.argv[1] == "itl":
         itl_rates()

@@ -343,6 +343,37 @@ def test_config0_full_length_hashes():
     eng.close()
 
 
+def test_config1_golden_hashes():
+    """Config 1 (256^3 cells, 6,711 vacancies, 573 K, 1e-6 s sector
+    threshold) for the oracle fixture's 1e5 events: SHA-256 of the (from, to)
+    site pairs of every 1000 events, of the whole occupancy every 10,000
+    events, and the sector clocks at the end (tests/golden/make_golden.py
+    cfg1)."""
+    say("test_config1_golden_hashes")
+    path = os.path.join(os.path.dirname(__file__), "golden", "hashes_cfg1_256cube.npz")
+    z = np.load(path, allow_pickle=False)
+    cfg = json.loads(str(z["cfg"]))
+    every, sites_every, total = int(z["every"]), int(z["sites_every"]), int(z["events"])
+    assert total >= 100000 and total % sites_every == 0
+    eng = akmc.Engine(cfg)
+    eng.set_event_log(total)
+    for k, want in enumerate(z["sites_sha256"]):
+        st = eng.run(sites_every)
+        assert st["events"] == (k + 1) * sites_every
+        got = np.frombuffer(hashlib.sha256(eng.download().tobytes()).digest(), dtype=np.uint8)
+        assert np.array_equal(got, want), "occupancy differs after %d events" % ((k + 1) * sites_every)
+    lf, lt = eng.event_log(total)
+    assert lf.size == total
+    for k, want in enumerate(z["log_sha256"]):
+        pairs = np.stack([lf[k * every:(k + 1) * every], lt[k * every:(k + 1) * every]], axis=1).astype(np.int64)
+        got = np.frombuffer(hashlib.sha256(pairs.tobytes()).digest(), dtype=np.uint8)
+        assert np.array_equal(got, want), "events %d..%d differ" % (k * every, (k + 1) * every)
+    assert np.array_equal(_bits(st["evolution"]), _bits(z["evolution"]))
+    assert st["sector_time"] == float(z["sector_time"]) and st["step"] == int(z["step"])
+    assert st["sector_pos"] == int(z["sect"])
+    eng.close()
+
+
 def test_pipe_created_lattice_bit_exact(tmp_path):
     # Pipe creation: the random creation's atoms and vacancies from a file;
     # the engine and the oracle start from the same sites and must agree.
