@@ -260,6 +260,13 @@ def served_leg(args, batched_value):
     t_start = srv.clock()
     t_next = t_start
     ids = []
+    t_note = t_start
+
+    def note(what):
+        st = srv.stats()
+        print("bench: served %s: %d submitted, %d done, %d live, %.3g events" %
+              (what, len(ids), st["done"], st["live"], st["events"]), file=sys.stderr, flush=True)
+
     for cfg, target, n_itl in jobs:
         t_next += rng.expovariate(rate)
         while True:
@@ -268,7 +275,17 @@ def served_leg(args, batched_value):
                 break
             time.sleep(min(0.002, t_next - now))
         ids.append((submit(srv, cfg, target, n_itl), target, n_itl))
-    srv.wait_idle(0)
+        if srv.clock() - t_note > 30:
+            t_note = srv.clock()
+            note("%.0f s" % (t_note - t_start))
+    while True:  # a progress line every 30 s until the last job is done
+        try:
+            srv.wait_idle(30)
+            break
+        except akmc.AkmcError as e:
+            if "timeout" not in str(e):
+                raise
+            note("%.0f s" % (srv.clock() - t_start))
     st = srv.stats()
     infos = [srv.info(j) for j, _, _ in ids]
     srv.close()
