@@ -647,7 +647,7 @@ struct Pool {
 
 int pool_create(int device, int32_t slots, Pool** out) {
   if (!out || slots <= 0) return fail(AKMC_ERR_INVALID, "bad pool arguments");
-  std::unique_ptr<Pool> p(new Pool());
+  std::unique_ptr<Pool, void (*)(Pool*)> p(new Pool(), pool_destroy);  // frees what was made on an error
   p->device = device;
   p->nslots = slots;
   HIP_TRY(hipSetDevice(device));
@@ -666,8 +666,8 @@ void pool_destroy(Pool* p) {
   (void)hipSetDevice(p->device);
   for (auto& L : p->slot)
     if (L) free_lat(*L);
-  (void)hipFree(p->d_state);
-  (void)hipFree(p->d_desc);
+  if (p->d_state) (void)hipFree(p->d_state);
+  if (p->d_desc) (void)hipFree(p->d_desc);
   if (p->h_state) (void)hipHostFree(p->h_state);
   if (p->h_desc) (void)hipHostFree(p->h_desc);
   delete p;
