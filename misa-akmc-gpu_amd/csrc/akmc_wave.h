@@ -234,28 +234,28 @@ __device__ __forceinline__ int first_above(const double* pvac, int n, double tar
 
 // ---------------------------------------------------------------- fold
 // Serial fold of the cnt <= 8 vacancies v .. v + cnt - 1 from P, in the
-// reference order: lane 8j + b loads slot b of vacancy v + j (one LDS load),
-// the fold reads the slots through readlane, and lane j writes vacancy
+// reference order.  Every lane reads the same rates (broadcast LDS reads, all
+// issued before the adds) and folds them in registers; lane j writes vacancy
 // v + j's running sum and tie risk.  Returns the sum after the last.
 template <class SecT>
 __device__ __forceinline__ double fold_serial(SecT& S, int v, int cnt, double P) {
   const int l = lane_id();
-  const int last = v + cnt - 1;
-  const double rl = (double)S.rate[8 * min(v + (l >> 3), last) + (l & 7)];
-  const uint64_t tl = S.tmask[min(v + (l & 7), last)];
   double pw = 0.0, pw0 = 0.0;
 #pragma unroll
   for (int j = 0; j < 8; j++) {
     if (j < cnt) {
+      double r[8];
+#pragma unroll
+      for (int b = 0; b < 8; b++) r[b] = (double)S.rate[8 * (v + j) + b];
       const double p0 = P;
 #pragma unroll
-      for (int b = 0; b < 8; b++) P += bcast_d(rl, 8 * j + b);
+      for (int b = 0; b < 8; b++) P += r[b];
       if (l == j) { pw = P; pw0 = p0; }
     }
   }
   if (l < cnt) {
     S.pvac[v + l] = pw;
-    S.risk[v + l] = fold_risk(pw0, pw, tl);
+    S.risk[v + l] = fold_risk(pw0, pw, S.tmask[v + l]);
   }
   return uni_d(P);
 }
@@ -429,40 +429,29 @@ __device__ __forceinline__ double tail_update(SecT& S, int v0, int n, double Pn,
     }
     // Serial exact fold from `stop` until a folded vacancy's old and new sums
     // share a binade again (normally one to three vacancies next to a power
-    // of two, or the risky vacancy alone).  Four vacancies per round: lane
-    // 8j + b loads slot b of vacancy w + j and lane j its old sum and tie
-    // mask; the fold reads them through readlane, and lane j writes vacancy
-    // w + j.
+    // of two, or the risky vacancy alone), at most 64.  One vacancy per step:
+    // every lane reads its eight rates, old sum and tie mask (broadcast LDS
+    // reads) and folds in registers; lane 0 writes its new sum and risk.
     double p = uni_d(stop == v ? Pprev : S.pvac[stop - 1]);
     double old_w = 0.0;
     int w = stop;
     bool same = false;
     for (;;) {
-      const double rl = (double)S.rate[8 * min(w + ((l >> 3) & 3), n - 1) + (l & 7)];
-      const int wo = min(w + (l & 3), n - 1);
-      const double ol = S.pvac[wo];
-      const uint64_t tl = S.tmask[wo];
-      const int kmax = min(4, n - w);
-      int k = 0;
-      double pw = 0.0, pw0 = 0.0;
+      double r[8];
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        if (j < kmax && !same && w + j - stop < 64) {
-          const double p0 = p;
+      for (int b = 0; b < 8; b++) r[b] = (double)S.rate[8 * w + b];
+      const double ow = S.pvac[w];
+      const uint64_t tw = S.tmask[w];
+      const double p0 = p;
 #pragma unroll
-          for (int b = 0; b < 8; b++) p += bcast_d(rl, 8 * j + b);
-          old_w = bcast_d(ol, j);
-          same = (exponent_of(old_w) == exponent_of(p)) & (old_w >= 0x1p-969);
-          if (l == j) { pw = p; pw0 = p0; }
-          k = j + 1;
-        }
+      for (int b = 0; b < 8; b++) p += r[b];
+      old_w = ow;
+      same = uni_i((exponent_of(old_w) == exponent_of(p)) & (old_w >= 0x1p-969)) != 0;
+      if (l == 0) {
+        S.pvac[w] = p;
+        S.risk[w] = fold_risk(p0, p, tw);
       }
-      const uint64_t tw = tl;
-      if (l < k) {
-        S.pvac[w + l] = pw;
-        S.risk[w + l] = fold_risk(pw0, pw, tw);
-      }
-      w += k;
+      w++;
       if (w >= n || same || w - stop >= 64) break;
     }
     wave_sync();
