@@ -299,15 +299,20 @@ def served_leg(args, batched_value):
 
     p50, p99, pmax = quantiles(infos)
     dumb = [i for i, (_, _, n) in zip(infos, ids) if n]
+    vac = [i for i, (_, _, n) in zip(infos, ids) if not n]
     d50, d99, dmax = quantiles(dumb)
+    v50, v99, vmax = quantiles(vac)
     span = max(i["t_done"] for i in infos) - t_start
     ev = sum(i["events"] for i in infos)
+    vspan = max(i["t_done"] for i in vac) - t_start if vac else None
     return {"value": ev / span, "unit": "events/s", "jobs": len(ids), "offered_jobs_per_s": rate,
             "capacity_jobs_per_s": capacity,
             "mean_target_events": mean_events, "target_scale": args.serve_scale,
             "p50_latency_s": p50, "p99_latency_s": p99, "max_latency_s": pmax,
             "dumbbell_jobs": len(dumb), "dumbbell_events": sum(i["events"] for i in dumb),
             "dumbbell_p50_latency_s": d50, "dumbbell_p99_latency_s": d99, "dumbbell_max_latency_s": dmax,
+            "vacancy_jobs_events_per_s": sum(i["events"] for i in vac) / vspan if vac else None,
+            "vacancy_jobs_p50_latency_s": v50, "vacancy_jobs_p99_latency_s": v99,
             "makespan_s": span, "gpu_busy_frac": st["busy_seconds"] / span, "launches": st["launches"],
             "hbm_peak_bytes": st["bytes_peak"], "failed": st["failed"],
             "data": "synthetic jobs (edge U{32..320}, vacancies U[1e-4,5e-4], T U[500,700] K, targets "

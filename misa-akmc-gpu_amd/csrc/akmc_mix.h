@@ -21,7 +21,9 @@ struct MixState {
   double thr, t;
   int64_t step, events;
   int32_t sect, in_sector, status, nvac;
-  int32_t nitl, pad;
+  int32_t nitl, nr;         // dumbbells; records of the current sector
+  double rsum;              // the records' total (valid with rec_valid)
+  int32_t rec_valid, pad;   // 1: the sector's records, rates and sums are current (a launch resumes on them)
   int64_t rec_pos;         // stand-in draws consumed
   int64_t n_recombined;    // vacancy-dumbbell pairs recombined
   int64_t n_generated;     // Frenkel pairs generated

@@ -894,6 +894,13 @@ __global__ __launch_bounds__(64) void mix_kernel(const MixDesc* descs) {
   int nvac = st.nvac, nitl = st.nitl, nr = 0;
   bool rebuild = true;
   double sum = 0.0;
+  // A launch that stopped inside a sector left its records current: resume
+  // on them (rebuilding would give the same rates and sums).
+  if (D.rates_sector < 0 && st.in_sector && st.rec_valid) {
+    rebuild = false;
+    nr = st.nr;
+    sum = st.rsum;
+  }
   int32_t status = AKMC_RUN_OK;
   int64_t visits = 0;
   Draws dr{&D, &st};
@@ -1263,10 +1270,12 @@ __global__ __launch_bounds__(64) void mix_kernel(const MixDesc* descs) {
     // ---- residence time (sublattice.inl:118-120)
     st.t += lg / total;
     st.events++;
-    if (!(st.t < st.thr) || st.events >= D.max_events || (D.u && st.events >= D.rng_end)) {
+    if (!(st.t < st.thr)) {
       rebuild = true;
       continue;
     }
+    // (At an event or stream limit the records are still brought up to date,
+    // so that the next launch resumes on them.)
     // ---- the next event's rates: records within reach of a changed site,
     // and every dumbbell when a term of Edumb changed.
     const bool all_itl = jlo < nitl || nitl != st.nitl;
@@ -1281,6 +1290,9 @@ __global__ __launch_bounds__(64) void mix_kernel(const MixDesc* descs) {
   }
   if (l == 0 && D.rates_sector < 0) {
     st.busy_ticks += (int64_t)(__builtin_amdgcn_s_memrealtime() - tick0);
+    st.nr = nr;
+    st.rsum = sum;
+    st.rec_valid = (!rebuild && st.in_sector) ? 1 : 0;
     st.status = status;
     st.nvac = nvac;
     st.nitl = nitl;
@@ -1288,6 +1300,7 @@ __global__ __launch_bounds__(64) void mix_kernel(const MixDesc* descs) {
     *D.st = st;
   }
   if (l == 0 && D.rates_sector >= 0 && status != AKMC_RUN_OK) *D.out_n = -100 - status;
+  if (l == 0 && D.rates_sector >= 0) D.st->rec_valid = 0;  // the hook used the record arrays
 }
 
 }  // namespace mix
