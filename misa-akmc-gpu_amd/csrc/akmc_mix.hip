@@ -173,6 +173,7 @@ struct ItlWork {
   double at[kMaxAff][64];   // and the term on the lattice after the jump
   int32_t vlist[64], ilist[64];
   int32_t ibuf[128];        // dumbbell records gathered for full passes
+  double tbuf[64];          // one chunk of Edumb terms, read by every lane
 };
 
 struct Ctx {
@@ -205,24 +206,28 @@ __device__ void refresh_term(const Ctx& c, int j) {
 // 64 terms per load, folded in order through lane reads.
 __device__ void prefix_from(const Ctx& c, int j0, int n) {
   const MixDesc& D = *c.D;
+  ItlWork& W = *c.W;
   const int l = lane_id();
   j0 = uni_i(j0);
   double acc = uni_d(j0 > 0 ? D.ipre[j0 - 1] : 0.0);
   double tn = j0 + l < n ? D.iterm[j0 + l] : 0.0;  // the next chunk loads while this one folds
   for (int base = j0; base < n; base += 64) {
     const int j = base + l;
-    const double t = tn;
+    W.tbuf[l] = tn;
+    __threadfence_block();
+    wave_sync();
     if (base + 64 < n) tn = j + 64 < n ? D.iterm[j + 64] : 0.0;
     const int cnt = min(64, n - base);
     double mine = 0.0;
+#pragma unroll 8
     for (int i = 0; i < cnt; i++) {
-      acc += bcast_d(t, i);
+      acc += W.tbuf[i];
       mine = l == i ? acc : mine;
     }
     if (j < n) D.ipre[j] = mine;
+    __threadfence_block();
+    wave_sync();
   }
-  __threadfence_block();
-  wave_sync();
 }
 
 // Masks and terms of every dumbbell whose term reads one of the sites cs[0 ..
@@ -469,18 +474,29 @@ __device__ void itl_pass(const Ctx& c, MixSector& S, const int32_t* rr, int m, i
   }
   const bool fast = act && !slow && na > 0;
   if (!fast) qf = nitl;
+  const int qa = fast ? W.aq[na - 1][l] : -1;  // the lane's last changed term
   double sa = fast && qf > 0 ? D.ipre[qf - 1] : 0.0;
   int k = 0, nq = fast ? W.aq[0][l] : nitl;
-  const int qmin = wave_min_i(qf);
-  // The next chunk's terms load while this one folds.
-  double tn = (qmin & ~63) + l < nitl ? D.iterm[(qmin & ~63) + l] : 0.0;
-  for (int base = qmin & ~63; base < nitl; base += 64) {
-    const double t = tn;
+  // Up to the last changed term of any lane: each lane adds from its first
+  // changed term on, its own changed terms in place of the stored ones.
+  // After it: every lane adds the stored terms (lanes without a fast fold
+  // add too; their sums are not used).  The next chunk loads while this one
+  // folds.
+  const int qmin = wave_min_i(qf), qend = wave_max_i(qa + 1);
+  int base = qend > 0 ? qmin & ~63 : nitl;  // no fast lane: nothing to fold
+  double tn = base + l < nitl ? D.iterm[base + l] : 0.0;
+  for (; base < nitl; base += 64) {
+    // The chunk's terms go to LDS, where every lane reads them (one
+    // broadcast read per term); the next chunk loads meanwhile.
+    W.tbuf[l] = tn;
+    __threadfence_block();
+    wave_sync();
     if (base + 64 < nitl) tn = base + 64 + l < nitl ? D.iterm[base + 64 + l] : 0.0;
     const int cnt = min(64, nitl - base);
-    for (int ii = 0; ii < cnt; ii++) {
+    int ii = 0;
+    for (; ii < cnt && base + ii < qend; ii++) {
       const int q = base + ii;
-      const double tq = bcast_d(t, ii);
+      const double tq = W.tbuf[ii];
       if (q >= qf) {
         double v = tq;
         if (q == nq) {
@@ -491,6 +507,10 @@ __device__ void itl_pass(const Ctx& c, MixSector& S, const int32_t* rr, int m, i
         sa += v;
       }
     }
+#pragma unroll 8
+    for (; ii < cnt; ii++) sa += W.tbuf[ii];
+    __threadfence_block();
+    wave_sync();
   }
   double rate = 0.0;
   if (fast) {
