@@ -1451,6 +1451,26 @@ int akmc_server_submit_itl(akmc_server* s, const akmc_params* p, int64_t n_itl, 
     if (p->box[d] < 8) return fail(AKMC_ERR_INVALID, "box edge %lld < 8 cells", (long long)p->box[d]);
   if (2 * p->box[0] > 2048 || p->box[1] > 2047 || p->box[2] > 1023)
     return fail(AKMC_ERR_INVALID, "box exceeds 1024 x 2047 x 1023 cells");
+  // Everything admission checks, checked here, so that a bad job is refused
+  // at once instead of stopping the server when its turn comes.
+  const int64_t nsites = 2 * p->box[0] * p->box[1] * p->box[2];
+  if (p->va_count < 0 || p->va_count + n_itl > nsites / 2)
+    return fail(AKMC_ERR_INVALID, "%lld vacancies and %lld dumbbells do not fit the box", (long long)p->va_count,
+                (long long)n_itl);
+  if (!(p->sector_time > 0) || !(p->temperature > 0) || !(p->attempt_freq > 0))
+    return fail(AKMC_ERR_INVALID, "bad time, temperature or frequency");
+  if (p->n_alloy < 0 || p->n_alloy > AKMC_MAX_ALLOY || !std::isfinite(ef110))
+    return fail(AKMC_ERR_INVALID, "bad alloy count or dumbbell energy");
+  {
+    uint64_t total = 0;
+    for (int i = 0; i < p->n_alloy; i++) {
+      const int t = p->alloy_type[i];
+      if (p->alloy_ratio[i] < 0 || !(t == AKMC_TYPE_FE || t == AKMC_TYPE_CU || t == AKMC_TYPE_NI || t == AKMC_TYPE_MN))
+        return fail(AKMC_ERR_INVALID, "bad alloy entry %d", i);
+      total += (uint64_t)p->alloy_ratio[i];
+    }
+    if (total == 0 || total > 0xFFFFFFFFull) return fail(AKMC_ERR_INVALID, "alloy ratios sum to %llu", (unsigned long long)total);
+  }
   std::unique_ptr<Job> j(new Job());
   j->p = *p;
   j->target = target_events;

@@ -559,7 +559,8 @@ int akmc_mix_set_event_log(akmc_mix* m, int64_t capacity) {
 }
 
 int akmc_mix_get_event_log(akmc_mix* m, int32_t i, int64_t* from, int64_t* to, int32_t* kind, int64_t cap, int64_t* n) {
-  if (!m || i < 0 || i >= (int32_t)m->lat.size() || !n) return fail(AKMC_ERR_INVALID, "bad arguments");
+  if (!m || i < 0 || i >= (int32_t)m->lat.size() || !n || cap < 0 || (cap > 0 && (!from || !to || !kind)))
+    return fail(AKMC_ERR_INVALID, "bad arguments");
   MixLat& L = *m->lat[(size_t)i];
   const int64_t have = std::min<int64_t>(m->h_state[(size_t)i].events, L.log_cap);
   *n = have;

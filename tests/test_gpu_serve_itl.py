@@ -129,6 +129,11 @@ def test_bad_dumbbell_jobs_are_refused():
         srv.submit_itl(_cfg(12, 1), 10, 100, itl_atom=3)
     with pytest.raises(akmc.AkmcError):
         srv.submit_itl(_cfg(12, 1), -1, 100)
+    with pytest.raises(akmc.AkmcError):
+        srv.submit_itl(_cfg(12, 1), 2 * 12 ** 3, 100)     # more dumbbells than the box holds: refused at once
+    j0 = srv.submit_itl(_cfg(10, 3), 5, 50)              # and the server still serves
+    srv.wait_idle(60)
+    assert srv.info(j0)["status"] == akmc.JOB_DONE
     with pytest.raises(akmc.AkmcError) as e:
         srv.submit_itl(make_cfg(n=200, va=10), 10, 10)
     assert e.value.code == -6
