@@ -17,7 +17,8 @@ pytestmark = pytest.mark.gpu
 def test_precision_study_small():
     print("test_precision_study_small", flush=True)
     sys.stdout.flush()
-    res = studies.precision(L=32, seeds=6, events=20000, hist_cap=32, chunk=10000, log=lambda m: None)
+    res = studies.precision(L=32, seeds=6, events=20000, hist_cap=32, chunk=10000, log=lambda m: None,
+                            error_events=20000)
     assert res["rate_error"]["rates_compared"] > 10000
     assert res["rate_error"]["max_rel"] <= 1e-6
     assert res["identical_final_lattices"] >= 4
