@@ -139,13 +139,25 @@ def batched_leg(args, plain_value):
     t0 = time.perf_counter()
     tuning = {}
     if args.batch_vcap:
-        tuning = dict(lds_vcap=args.batch_vcap, waves_per_group=args.batch_waves, order=args.batch_order)
+        tuning = dict(lds_vcap=args.batch_vcap, waves_per_group=args.batch_waves, order=args.batch_order,
+                      interleave=args.batch_interleave)
     b = akmc.Batch(cfgs, device=0, precision=akmc.AKMC_PREC_FP32, init=akmc.AKMC_INIT_DEVICE,
                    slice_events=args.batch_slice, **tuning)
     t_create = time.perf_counter() - t0
     print("bench: %d replicas created in %.1f s" % (len(cfgs), t_create), file=sys.stderr, flush=True)
     tun = b.tune(trial_seconds=args.tune_seconds) if not (args.no_tune or tuning) else b.tuning()
     print("bench: tuning %s" % tun, file=sys.stderr, flush=True)
+    # The interleave-width sweep: the best trial of the tuner at each width
+    # (replica waves per SIMD), the other knobs free.
+    sweep = {}
+    for t in (b.tune_trials() if not (args.no_tune or tuning) else []):
+        k = str(t["interleave"])
+        if k not in sweep or t["events_per_second"] > sweep[k]["events_per_second"]:
+            sweep[k] = {"events_per_second": t["events_per_second"], "lds_vcap": t["lds_vcap"],
+                        "waves_per_group": t["waves_per_group"], "order": t["order"]}
+    if sweep:
+        print("bench: interleave sweep %s" % {k: "%.3g" % v["events_per_second"] for k, v in sweep.items()},
+              file=sys.stderr, flush=True)
     for _ in range(args.batch_warmup):
         b.run(args.batch_events)
     akmc.sync(0)
@@ -205,7 +217,8 @@ def batched_leg(args, plain_value):
             "events_timed": ev, "ms_per_step": dt / args.steps * 1e3,
             "deadline": deadline,
             "kernel_seconds": ksec, "launches": launches, "dtype": "f32 rates, f64 sums/search/dt",
-            "tuning": tun, "vs_plain": value / plain_value if plain_value else None,
+            "tuning": tun, "interleave_sweep": sweep or None,
+            "vs_plain": value / plain_value if plain_value else None,
             "evaluations_per_event": d_eval / max(1, ev),
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS,
@@ -464,6 +477,7 @@ def main():
     ap.add_argument("--batch-vcap", type=int, default=0, help="batched leg: fixed tuning (LDS capacity) instead of the tuner")
     ap.add_argument("--batch-waves", type=int, default=1)
     ap.add_argument("--batch-order", type=int, default=0)
+    ap.add_argument("--batch-interleave", type=int, default=1, help="batched leg, fixed tuning: replica waves per SIMD")
     ap.add_argument("--batch-warmup", type=int, default=1)
     ap.add_argument("--no-deadline", action="store_true", help="batched leg: skip the deadline launches")
     ap.add_argument("--deadline-steps", type=int, default=3)

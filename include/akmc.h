@@ -277,7 +277,8 @@ typedef struct akmc_batch_options {
                                LDS (0 = autotune, -1 = none) */
   int32_t waves_per_group;  /* tuning: replicas per workgroup, 1, 2 or 4 (0 = autotune) */
   int32_t order;            /* tuning: 0 replica order, 1 largest first (-1 = autotune) */
-  int32_t reserved;
+  int32_t interleave;       /* tuning: replica waves per SIMD, 1, 2 or 4 (0 = 1); above 1 the workgroups
+                               hold 4 * interleave waves and waves_per_group is ignored */
   int64_t slice_events;     /* events per replica per launch (0 = default 1024) */
 } akmc_batch_options;
 
@@ -313,7 +314,7 @@ typedef struct akmc_batch_tuning {
   int32_t lds_vcap;
   int32_t waves_per_group;
   int32_t order;
-  int32_t reserved;
+  int32_t interleave;       /* replica waves per SIMD */
   int64_t slice_events;
   double events_per_second; /* best trial */
   int32_t trials;
@@ -339,13 +340,18 @@ int akmc_batch_run(akmc_batch* b, int64_t events_per_replica, akmc_batch_stats* 
  * replica then differ).  Trajectories are those of akmc_batch_run. */
 int akmc_batch_run_for(akmc_batch* b, double seconds, akmc_batch_stats* stats);
 
-/* Searches the tuning space (LDS capacity, replicas per workgroup, work
- * order) with one deadline launch of real work per setting (the replicas
+/* Searches the tuning space (LDS capacity, replicas per workgroup, interleave
+ * width = replica waves per SIMD, work order) with one deadline launch of
+ * real work per setting (the replicas
  * advance; their trajectories do not depend on the tuning), keeps the
  * setting with the most events per device second, then brings every running
  * replica to the same event count. */
 int akmc_batch_tune(akmc_batch* b, double trial_seconds, akmc_batch_tuning* chosen);
 int akmc_batch_get_tuning(akmc_batch* b, akmc_batch_tuning* t);
+/* Every setting the last akmc_batch_tune tried, with its events per device
+ * second (events_per_second; trials = 1 each): *n is the number of settings,
+ * of which the first min(cap, *n) are written to out. */
+int akmc_batch_tune_trials(akmc_batch* b, akmc_batch_tuning* out, int32_t cap, int32_t* n);
 
 int32_t akmc_batch_size(const akmc_batch* b);
 int64_t akmc_batch_num_sites(const akmc_batch* b, int32_t replica);

@@ -358,7 +358,7 @@ JOB_PENDING, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
 class BatchOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("init", ctypes.c_int32), ("error_stats", ctypes.c_int32),
                 ("use_graph", ctypes.c_int32), ("lds_vcap", ctypes.c_int32), ("waves_per_group", ctypes.c_int32),
-                ("order", ctypes.c_int32), ("reserved", ctypes.c_int32), ("slice_events", ctypes.c_int64)]
+                ("order", ctypes.c_int32), ("interleave", ctypes.c_int32), ("slice_events", ctypes.c_int64)]
 
 
 class BatchStats(ctypes.Structure):
@@ -385,7 +385,7 @@ class ReplicaStats(ctypes.Structure):
 
 class BatchTuning(ctypes.Structure):
     _fields_ = [("lds_vcap", ctypes.c_int32), ("waves_per_group", ctypes.c_int32), ("order", ctypes.c_int32),
-                ("reserved", ctypes.c_int32), ("slice_events", ctypes.c_int64),
+                ("interleave", ctypes.c_int32), ("slice_events", ctypes.c_int64),
                 ("events_per_second", ctypes.c_double), ("trials", ctypes.c_int32), ("reserved2", ctypes.c_int32)]
 
     def as_dict(self):
@@ -426,6 +426,7 @@ def _bind_batch(L):
     L.akmc_batch_run_for.argtypes = [vp, dbl, P(BatchStats)]
     L.akmc_batch_tune.argtypes = [vp, dbl, P(BatchTuning)]
     L.akmc_batch_get_tuning.argtypes = [vp, P(BatchTuning)]
+    L.akmc_batch_tune_trials.argtypes = [vp, vp, ctypes.c_int32, P(ctypes.c_int32)]
     L.akmc_batch_size.argtypes = [vp]
     L.akmc_batch_num_sites.argtypes = [vp, i32]
     L.akmc_batch_num_sites.restype = i64
@@ -602,6 +603,14 @@ class Batch:
         t = BatchTuning()
         _check(_blib().akmc_batch_get_tuning(self.h, ctypes.byref(t)))
         return t.as_dict()
+
+    def tune_trials(self):
+        """Every setting the last tune() tried, with its events per device second."""
+        n = ctypes.c_int32()
+        _check(_blib().akmc_batch_tune_trials(self.h, None, 0, ctypes.byref(n)))
+        arr = (BatchTuning * max(1, n.value))()
+        _check(_blib().akmc_batch_tune_trials(self.h, ctypes.cast(arr, ctypes.c_void_p), n.value, ctypes.byref(n)))
+        return [arr[i].as_dict() for i in range(n.value)]
 
     def replica_stats(self, i):
         s = ReplicaStats()

@@ -62,6 +62,22 @@ AKMC_HD inline WsLayout ws_layout(int64_t cap, int rate_bytes) {
   return w;
 }
 
+// Brick layout of a replica's sites in device memory: bricks of 8 (x,
+// doubled) x 4 (y) x 4 (z) sites, 128 bytes = one cache line each, stored
+// x-fastest, then y, then z, the box padded to whole bricks.  A vacancy's
+// 55-site stencil (x +-3, y +-2, z +-2) then spans about 7 lines instead of
+// the ~26 rows of the plain x-fastest order.  Inside a brick: byte
+// (z & 3) * 32 + (y & 3) * 8 + (x & 7).  The host sees logical order
+// (x + X (y + Y z)) only: uploads and downloads convert.
+struct BrickDims {
+  int32_t nbx, nby, nbz;
+};
+AKMC_HD inline BrickDims brick_dims(const BoxDims& d) { return BrickDims{(d.X + 7) >> 3, (d.Y + 3) >> 2, (d.Z + 3) >> 2}; }
+AKMC_HD inline int64_t brick_bytes(const BrickDims& b) { return (int64_t)b.nbx * b.nby * b.nbz * 128; }
+AKMC_HD inline int64_t brick_index(int32_t nbx, int32_t nby, uint32_t x, uint32_t y, uint32_t z) {
+  return ((((int64_t)(z >> 2) * nby + (y >> 2)) * nbx + (x >> 3)) << 7) | (int64_t)(((z & 3u) << 5) | ((y & 3u) << 3) | (x & 7u));
+}
+
 // Per-replica constants (read-only during a launch).
 struct ReplicaDesc {
   uint8_t* sites;       // one flag byte per site, x + X*(y + Y*z), x doubled
@@ -73,6 +89,7 @@ struct ReplicaDesc {
   int32_t cap;
   int32_t nhi;          // entries of tab_hi
   int32_t m_min;        // barrier index of tab_hi[0] * tab_lo[0]
+  int32_t nbx, nby;     // bricks along x and y (the site layout)
   int32_t pad;
   double nu;
   double kT;
@@ -129,12 +146,17 @@ struct BatchArgs {
 
 }  // namespace akmc
 
-extern "C" int akmc_batch_launch(const akmc::BatchArgs* a, int fp32, int lds, int waves_per_group, int grid,
+// il: interleave width (replica waves per SIMD: 1, 2 or 4; workgroups of
+// 4 * il waves when il > 1).
+extern "C" int akmc_batch_launch(const akmc::BatchArgs* a, int fp32, int lds, int il, int waves_per_group, int grid,
                                  hipStream_t stream);
 extern "C" int akmc_batch_reset_launch(uint64_t* queue, hipStream_t stream);
 extern "C" int akmc_batch_set_stencil(const akmc::StencilTables* t);
-extern "C" int akmc_batch_init_sites_launch(uint8_t* sites, int64_t n, uint64_t seed, const uint32_t* cum,
+// Both take the replica's box and write its brick layout; site i of the
+// logical order draws from counter i whatever the layout.
+extern "C" int akmc_batch_init_sites_launch(uint8_t* sites, akmc::BoxDims d, uint64_t seed, const uint32_t* cum,
                                             const uint8_t* flag, int32_t n_types, hipStream_t stream);
-extern "C" int akmc_batch_place_vacancies_launch(uint8_t* sites, const int64_t* idx, int64_t n, hipStream_t stream);
+extern "C" int akmc_batch_place_vacancies_launch(uint8_t* sites, akmc::BoxDims d, const int64_t* idx, int64_t n,
+                                                 hipStream_t stream);
 extern "C" int akmc_batch_region_bytes(int fp32, int vcap);
-extern "C" int akmc_batch_occupancy(int fp32, int lds, int waves_per_group, int region_bytes, int* blocks_per_cu);
+extern "C" int akmc_batch_occupancy(int fp32, int lds, int il, int waves_per_group, int region_bytes, int* blocks_per_cu);

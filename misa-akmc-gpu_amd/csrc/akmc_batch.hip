@@ -83,7 +83,13 @@ struct RView {
   BoxDims d;
   double nu, kT;
   int32_t nhi, m_min;
+  int32_t nbx, nby;
 };
+
+// Device address of the site with key k (brick layout, akmc_batch.h).
+__device__ __forceinline__ uint64_t site_at(const RView& R, uint32_t k) {
+  return (uint64_t)brick_index(R.nbx, R.nby, key_x(k), key_y(k), key_z(k));
+}
 
 struct ErrAcc {
   double emax, esum;
@@ -190,7 +196,7 @@ __device__ void eval_rep(const Sec<RateT>& S, RStage& G, const RTabs& T, const R
     for (int i = 0; i < BATCH_EVB; i++) {
       const int vv = idx ? idx[bs + min(i, bc - 1)] : base + bs + min(i, bc - 1);
       f[i] = pf;
-      if (i < bc && vv != pf_vi) f[i] = R.sites[key_index(R.d, qk[i])];
+      if (i < bc && vv != pf_vi) f[i] = R.sites[site_at(R, qk[i])];
     }
 #pragma unroll
     for (int i = 0; i < BATCH_EVB; i++) {
@@ -249,7 +255,7 @@ __device__ void eval_rep(const Sec<RateT>& S, RStage& G, const RTabs& T, const R
                                     (((mni >> sh) & 0xFF) ? BIT_NI : 0));
       const uint8_t f0 = vi == pt.moved ? pt.f1 : (own_from_sec ? S.own[vi] : G.cflag[a]);
       const uint8_t nf = (uint8_t)(f0 & ~clr);
-      if (nf != f0) R.sites[key_index(R.d, S.key[vi])] = nf;
+      if (nf != f0) R.sites[site_at(R, S.key[vi])] = nf;
       S.own[vi] = nf;
       S.vt[vi] = vt;
     }
@@ -351,7 +357,7 @@ __device__ void eval_one(const Sec<RateT>& S, RStage& G, const RTabs& T, const R
     const uint8_t clr = (uint8_t)((cu ? BIT_CU : 0) | (mn ? BIT_MN : 0) | (ni ? BIT_NI : 0));
     const uint8_t f0 = vi == pt.moved ? pt.f1 : (own_from_sec ? S.own[vi] : (uint8_t)f_own);
     const uint8_t nf = (uint8_t)(f0 & ~clr);
-    if (nf != f0) R.sites[key_index(R.d, w)] = nf;
+    if (nf != f0) R.sites[site_at(R, w)] = nf;
     S.own[vi] = nf;
     S.vt[vi] = vt;
   }
@@ -398,7 +404,9 @@ __device__ __forceinline__ int scan_pick(double x, int n, double tl, double& bas
   return k;
 }
 
-// Collects the vacancies of sector s in vacancy-list order.
+// Collects the vacancies of sector s and sorts them by key (z, y, x), so
+// that each 64-vacancy block covers a few z planes: after a hop only the
+// blocks whose z range reaches the hop are scanned for stencil hits.
 template <typename RateT>
 __device__ int collect(const Sec<RateT>& S, const RView& R, int s) {
   const int l = lane_id();
@@ -413,6 +421,7 @@ __device__ int collect(const Sec<RateT>& S, const RView& R, int s) {
     n += __popcll(m);
   }
   wave_sync();
+  sort_by_key(S.key, S.slot, n);
   return n;
 }
 
@@ -459,7 +468,7 @@ __device__ void run_item(const BatchArgs& a, const RTabs& T, RStage& G, uint8_t*
   const bool rates_only = (a.flags & AKMC_BF_RATES_ONLY) != 0;
   if (!rates_only && st.status != AKMC_REP_RUNNING) return;
   const uint64_t t_item = (uint64_t)wall_clock64();
-  const RView R{D.sites, D.vac_key, D.nvac, D.d, D.nu, D.kT, D.nhi, D.m_min};
+  const RView R{D.sites, D.vac_key, D.nvac, D.d, D.nu, D.kT, D.nhi, D.m_min, D.nbx, D.nby};
   if (FP32)
     for (int i = l; i < 256 + D.nhi; i += 64) G.tab[i] = D.exp_tab[i];
   if (l < 8) {
@@ -638,7 +647,7 @@ __device__ void run_item(const BatchArgs& a, const RTabs& T, RStage& G, uint8_t*
     if (l < T.st.n_sites) {
       const int pp = par ^ 1;
       pf_key = key_add(R.d, to, T.st.off[pp][l][0], T.st.off[pp][l][1], T.st.off[pp][l][2]);
-      pf = R.sites[key_index(R.d, pf_key)];
+      pf = R.sites[site_at(R, pf_key)];
     }
     const uint8_t ff = S.own[v];
     const uint8_t gt = S.nb1[v * 8 + bsel];
@@ -648,8 +657,8 @@ __device__ void run_item(const BatchArgs& a, const RTabs& T, RStage& G, uint8_t*
     const int32_t vslot = S.slot[v];
     const bool stays = key_sector(R.d, to) == s;
     if (l == 0) {
-      R.sites[key_index(R.d, from)] = nf_from;
-      R.sites[key_index(R.d, to)] = nf_to;
+      R.sites[site_at(R, from)] = nf_from;
+      R.sites[site_at(R, to)] = nf_to;
       R.vac_key[vslot] = to;
       if (stays) {
         S.key[v] = to;
@@ -700,7 +709,7 @@ __device__ void run_item(const BatchArgs& a, const RTabs& T, RStage& G, uint8_t*
             const int ph = (int)(key_x(wh) & 1);
             const int lk = min(l, T.st.n_sites - 1);
             fk = key_add(R.d, wh, T.st.off[ph][lk][0], T.st.off[ph][lk][1], T.st.off[ph][lk][2]);
-            f = R.sites[key_index(R.d, fk)];
+            f = R.sites[site_at(R, fk)];
           }
           if (fk == from) f = nf_from;
           if (fk == to) f = nf_to;
@@ -751,8 +760,14 @@ __device__ void run_item(const BatchArgs& a, const RTabs& T, RStage& G, uint8_t*
   if (l == 0) a.state[r] = st;
 }
 
-template <typename RateT, bool FP32, bool LDS>
-__global__ __launch_bounds__(256) void replica_kernel(BatchArgs a) {
+// IL (interleave width) is the number of replica waves that share one SIMD:
+// the kernel is compiled for at most 512 / IL registers per lane, so that IL
+// waves fit on a SIMD, and launched in workgroups of 4 * IL waves (one
+// workgroup per CU, IL waves on each of its four SIMDs; IL = 1 also takes
+// workgroups of 1 or 2 waves).  While one wave waits on a gather or a
+// dependent chain, the SIMD issues another replica's instructions.
+template <typename RateT, bool FP32, bool LDS, int IL>
+__global__ __launch_bounds__(256 * IL, IL) void replica_kernel(BatchArgs a) {
   __shared__ RTabs T;
   extern __shared__ __align__(16) uint8_t dyn[];
   {
@@ -795,61 +810,90 @@ __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
   x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
   return x ^ (x >> 31);
 }
-__global__ void init_sites_kernel(uint8_t* sites, int64_t n, uint64_t seed, const uint32_t* cum, const uint8_t* flag,
+// The loop runs over the brick layout (every byte, padding included, is
+// written once); the draw of a site uses its logical index.
+__global__ void init_sites_kernel(uint8_t* sites, BoxDims d, uint64_t seed, const uint32_t* cum, const uint8_t* flag,
                                   int32_t nt) {
   const uint32_t total = cum[nt - 1];
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const uint64_t h = splitmix64(seed ^ (0xD1B54A32D192ED03ull * (uint64_t)(i + 1)));
-    const uint32_t x = (uint32_t)(((h >> 32) * (uint64_t)total) >> 32);  // [0, total)
+  const BrickDims b = brick_dims(d);
+  const int64_t nb = brick_bytes(b);
+  for (int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; a < nb; a += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t brick = a >> 7;
+    const int w = (int)(a & 127);
+    const int64_t bxy = (int64_t)b.nbx * b.nby;
+    const int x = (int)(brick % b.nbx) * 8 + (w & 7);
+    const int y = (int)((brick / b.nbx) % b.nby) * 4 + ((w >> 3) & 3);
+    const int z = (int)(brick / bxy) * 4 + (w >> 5);
     uint8_t f = 0;
-    for (int t = nt - 1; t >= 0; t--)
-      if (x < cum[t]) f = flag[t];
-    sites[i] = f;
+    if (x < d.X && y < d.Y && z < d.Z) {
+      const int64_t i = (int64_t)x + (int64_t)d.X * ((int64_t)y + (int64_t)d.Y * z);
+      const uint64_t h = splitmix64(seed ^ (0xD1B54A32D192ED03ull * (uint64_t)(i + 1)));
+      const uint32_t r = (uint32_t)(((h >> 32) * (uint64_t)total) >> 32);  // [0, total)
+      for (int t = nt - 1; t >= 0; t--)
+        if (r < cum[t]) f = flag[t];
+    }
+    sites[a] = f;
   }
 }
 // Vacancy placement as the reference's createRandom does it: the site gains
 // the V tag and loses a Cu tag; Mn/Ni tags stay (creation.cpp:131-164).
-__global__ void place_vacancies_kernel(uint8_t* sites, const int64_t* idx, int64_t n) {
+// idx holds logical site indices.
+__global__ void place_vacancies_kernel(uint8_t* sites, BoxDims d, const int64_t* idx, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
-    const int64_t s = idx[i];
+    const int64_t li = idx[i];
+    const BrickDims b = brick_dims(d);
+    const uint32_t x = (uint32_t)(li % d.X), y = (uint32_t)((li / d.X) % d.Y), z = (uint32_t)(li / ((int64_t)d.X * d.Y));
+    const int64_t s = brick_index(b.nbx, b.nby, x, y, z);
     sites[s] = (uint8_t)((sites[s] & ~BIT_CU) | BIT_V);
   }
 }
 
 template <typename RateT, bool FP32, bool LDS>
-static int launch_t(const BatchArgs* a, int waves, int grid, hipStream_t stream) {
+static int launch_t(const BatchArgs* a, int il, int waves, int grid, hipStream_t stream) {
   const size_t lds = (size_t)waves * a->region_bytes;
-  hipLaunchKernelGGL((replica_kernel<RateT, FP32, LDS>), dim3(grid), dim3(64 * waves), lds, stream, *a);
+  if (il == 4)
+    hipLaunchKernelGGL((replica_kernel<RateT, FP32, LDS, 4>), dim3(grid), dim3(64 * waves), lds, stream, *a);
+  else if (il == 2)
+    hipLaunchKernelGGL((replica_kernel<RateT, FP32, LDS, 2>), dim3(grid), dim3(64 * waves), lds, stream, *a);
+  else
+    hipLaunchKernelGGL((replica_kernel<RateT, FP32, LDS, 1>), dim3(grid), dim3(64 * waves), lds, stream, *a);
   return (int)hipGetLastError();
 }
 
 template <typename RateT, bool FP32, bool LDS>
-static int occupancy_t(int waves, int region_bytes, int* blocks) {
-  return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, replica_kernel<RateT, FP32, LDS>, 64 * waves,
-                                                            (size_t)waves * region_bytes);
+static int occupancy_t(int il, int waves, int region_bytes, int* blocks) {
+  const size_t lds = (size_t)waves * region_bytes;
+  if (il == 4)
+    return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, replica_kernel<RateT, FP32, LDS, 4>, 64 * waves, lds);
+  if (il == 2)
+    return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, replica_kernel<RateT, FP32, LDS, 2>, 64 * waves, lds);
+  return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, replica_kernel<RateT, FP32, LDS, 1>, 64 * waves, lds);
 }
 
 }  // namespace akmc
 
 using akmc::BatchArgs;
 
-extern "C" int akmc_batch_launch(const BatchArgs* a, int fp32, int lds, int waves, int grid, hipStream_t stream) {
+extern "C" int akmc_batch_launch(const BatchArgs* a, int fp32, int lds, int il, int waves, int grid,
+                                 hipStream_t stream) {
+  if (il != 1 && il != 2 && il != 4) return (int)hipErrorInvalidValue;
   if (fp32) {
-    return lds ? akmc::launch_t<float, true, true>(a, waves, grid, stream)
-               : akmc::launch_t<float, true, false>(a, waves, grid, stream);
+    return lds ? akmc::launch_t<float, true, true>(a, il, waves, grid, stream)
+               : akmc::launch_t<float, true, false>(a, il, waves, grid, stream);
   }
-  return lds ? akmc::launch_t<double, false, true>(a, waves, grid, stream)
-             : akmc::launch_t<double, false, false>(a, waves, grid, stream);
+  return lds ? akmc::launch_t<double, false, true>(a, il, waves, grid, stream)
+             : akmc::launch_t<double, false, false>(a, il, waves, grid, stream);
 }
 
-extern "C" int akmc_batch_occupancy(int fp32, int lds, int waves, int region_bytes, int* blocks) {
+extern "C" int akmc_batch_occupancy(int fp32, int lds, int il, int waves, int region_bytes, int* blocks) {
+  if (il != 1 && il != 2 && il != 4) return (int)hipErrorInvalidValue;
   if (fp32) {
-    return lds ? akmc::occupancy_t<float, true, true>(waves, region_bytes, blocks)
-               : akmc::occupancy_t<float, true, false>(waves, region_bytes, blocks);
+    return lds ? akmc::occupancy_t<float, true, true>(il, waves, region_bytes, blocks)
+               : akmc::occupancy_t<float, true, false>(il, waves, region_bytes, blocks);
   }
-  return lds ? akmc::occupancy_t<double, false, true>(waves, region_bytes, blocks)
-             : akmc::occupancy_t<double, false, false>(waves, region_bytes, blocks);
+  return lds ? akmc::occupancy_t<double, false, true>(il, waves, region_bytes, blocks)
+             : akmc::occupancy_t<double, false, false>(il, waves, region_bytes, blocks);
 }
 
 extern "C" int akmc_batch_reset_launch(uint64_t* queue, hipStream_t stream) {
@@ -866,18 +910,20 @@ extern "C" int akmc_batch_region_bytes(int fp32, int vcap) {
   return (int)(akmc::RSTAGE_BYTES + sec);
 }
 
-extern "C" int akmc_batch_init_sites_launch(uint8_t* sites, int64_t n, uint64_t seed, const uint32_t* cum,
+extern "C" int akmc_batch_init_sites_launch(uint8_t* sites, akmc::BoxDims d, uint64_t seed, const uint32_t* cum,
                                             const uint8_t* flag, int32_t nt, hipStream_t stream) {
+  const int64_t n = akmc::brick_bytes(akmc::brick_dims(d));
   if (n <= 0) return 0;
   const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
-  hipLaunchKernelGGL(akmc::init_sites_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, sites, n, seed, cum, flag,
+  hipLaunchKernelGGL(akmc::init_sites_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, sites, d, seed, cum, flag,
                      nt);
   return (int)hipGetLastError();
 }
 
-extern "C" int akmc_batch_place_vacancies_launch(uint8_t* sites, const int64_t* idx, int64_t n, hipStream_t stream) {
+extern "C" int akmc_batch_place_vacancies_launch(uint8_t* sites, akmc::BoxDims d, const int64_t* idx, int64_t n,
+                                                 hipStream_t stream) {
   if (n <= 0) return 0;
-  hipLaunchKernelGGL(akmc::place_vacancies_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, sites, idx,
-                     n);
+  hipLaunchKernelGGL(akmc::place_vacancies_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, sites, d,
+                     idx, n);
   return (int)hipGetLastError();
 }

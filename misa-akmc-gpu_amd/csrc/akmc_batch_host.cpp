@@ -107,6 +107,7 @@ struct HostRep {
   akmc_params p{};
   akmc::BoxDims d{};
   int64_t nsites = 0;
+  int64_t sbytes = 0;  // device bytes of the sites in the brick layout (>= nsites)
   int32_t cap = 1;
   int32_t nvac = 0;
   double kT = 0;
@@ -147,6 +148,7 @@ int setup_rep(HostRep* r, const akmc_params& p, bool fp32) {
   r->p = p;
   r->d = akmc::box_dims(p.box);
   r->nsites = (int64_t)r->d.X * r->d.Y * r->d.Z;
+  r->sbytes = akmc::brick_bytes(akmc::brick_dims(r->d));
   r->cap = (int32_t)std::max<int64_t>(1, p.va_count);
   r->kT = AKMC_KB * p.temperature;
   r->time_steps = (int64_t)std::ceil(p.physics_time / p.sector_time);
@@ -154,7 +156,7 @@ int setup_rep(HostRep* r, const akmc_params& p, bool fp32) {
     const int rc = make_exp_table(r->kT, p.attempt_freq, &r->tab, &r->nhi, &r->m_min);
     if (rc) return rc;
   }
-  r->bytes = a256(r->nsites) + a256(4 * (int64_t)r->cap) + a256(akmc::ws_layout(r->cap, fp32 ? 4 : 8).total) +
+  r->bytes = a256(r->sbytes) + a256(4 * (int64_t)r->cap) + a256(akmc::ws_layout(r->cap, fp32 ? 4 : 8).total) +
              a256(4 * (int64_t)(256 + r->nhi)) + a256(8 * (int64_t)r->cap) + a256(4 * AKMC_MAX_ALLOY) +
              a256(AKMC_MAX_ALLOY);
   return AKMC_OK;
@@ -163,7 +165,7 @@ int setup_rep(HostRep* r, const akmc_params& p, bool fp32) {
 void carve(HostRep* r, uint8_t* base, bool fp32) {
   r->mem = base;
   r->sites = base;
-  base += a256(r->nsites);
+  base += a256(r->sbytes);
   r->vac = reinterpret_cast<uint32_t*>(base);
   base += a256(4 * (int64_t)r->cap);
   r->ws = base;
@@ -188,6 +190,9 @@ akmc::ReplicaDesc make_desc(const HostRep& r) {
   D.cap = r.cap;
   D.nhi = r.nhi;
   D.m_min = r.m_min;
+  const akmc::BrickDims bd = akmc::brick_dims(r.d);
+  D.nbx = bd.nbx;
+  D.nby = bd.nby;
   D.nu = r.p.attempt_freq;
   D.kT = r.kT;
   D.sector_T = r.p.sector_time;
@@ -249,7 +254,7 @@ int init_device_async(HostRep* r, hipStream_t st) {
   if (nt == 0) return fail(AKMC_ERR_INVALID, "empty alloy");
   HIP_TRY(hipMemcpyAsync(r->dcum, r->h_cum, sizeof(uint32_t) * nt, hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(r->dflag, r->h_flag, nt, hipMemcpyHostToDevice, st));
-  if (akmc_batch_init_sites_launch(r->sites, r->nsites, (uint64_t)r->p.seed_create_types, r->dcum, r->dflag, nt, st))
+  if (akmc_batch_init_sites_launch(r->sites, r->d, (uint64_t)r->p.seed_create_types, r->dcum, r->dflag, nt, st))
     return fail(AKMC_ERR_HIP, "init_sites launch failed");
   // Vacancy sites: va_count uniform draws (repeats collapse, as in the
   // reference's hash insertions), in site order.
@@ -264,7 +269,7 @@ int init_device_async(HostRep* r, hipStream_t st) {
   for (size_t i = 0; i < idx.size(); i++) keys[i] = key_of(r->d, idx[i]);
   if (!idx.empty()) {
     HIP_TRY(hipMemcpyAsync(r->didx, idx.data(), idx.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
-    if (akmc_batch_place_vacancies_launch(r->sites, r->didx, (int64_t)idx.size(), st))
+    if (akmc_batch_place_vacancies_launch(r->sites, r->d, r->didx, (int64_t)idx.size(), st))
       return fail(AKMC_ERR_HIP, "place_vacancies launch failed");
     HIP_TRY(hipMemcpyAsync(r->vac, keys.data(), keys.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
   }
@@ -275,11 +280,43 @@ int init_device_async(HostRep* r, hipStream_t st) {
   return AKMC_OK;
 }
 
+// A replica's sites between the logical order x + X (y + Y z) and the
+// device's brick layout (akmc_batch.h), eight x-consecutive bytes at a time.
+void to_bricks(const akmc::BoxDims& d, const uint8_t* logical, uint8_t* bricks) {
+  const akmc::BrickDims b = akmc::brick_dims(d);
+  memset(bricks, 0, (size_t)akmc::brick_bytes(b));
+  for (int z = 0; z < d.Z; z++)
+    for (int y = 0; y < d.Y; y++) {
+      const uint8_t* row = logical + (int64_t)d.X * (y + (int64_t)d.Y * z);
+      for (int x = 0; x < d.X; x += 8)
+        memcpy(bricks + akmc::brick_index(b.nbx, b.nby, x, y, z), row + x, (size_t)std::min(8, d.X - x));
+    }
+}
+void from_bricks(const akmc::BoxDims& d, const uint8_t* bricks, uint8_t* logical) {
+  const akmc::BrickDims b = akmc::brick_dims(d);
+  for (int z = 0; z < d.Z; z++)
+    for (int y = 0; y < d.Y; y++) {
+      uint8_t* row = logical + (int64_t)d.X * (y + (int64_t)d.Y * z);
+      for (int x = 0; x < d.X; x += 8)
+        memcpy(row + x, bricks + akmc::brick_index(b.nbx, b.nby, x, y, z), (size_t)std::min(8, d.X - x));
+    }
+}
+
+// Copies a replica's sites to the host in logical order (synchronous).
+int download_sites(const HostRep& r, uint8_t* out) {
+  std::vector<uint8_t> tmp((size_t)r.sbytes);
+  HIP_TRY(hipMemcpy(tmp.data(), r.sites, (size_t)r.sbytes, hipMemcpyDeviceToHost));
+  from_bricks(r.d, tmp.data(), out);
+  return AKMC_OK;
+}
+
 int upload_rep(HostRep* r, const uint8_t* sites, hipStream_t st) {
   const std::vector<uint32_t> keys = vacancy_keys(r->d, sites, r->nsites);
   if ((int64_t)keys.size() > r->cap)
     return fail(AKMC_ERR_CAPACITY, "lattice holds %zu vacancies, replica capacity %d", keys.size(), r->cap);
-  HIP_TRY(hipMemcpyAsync(r->sites, sites, r->nsites, hipMemcpyHostToDevice, st));
+  std::vector<uint8_t> bricks((size_t)r->sbytes);
+  to_bricks(r->d, sites, bricks.data());
+  HIP_TRY(hipMemcpyAsync(r->sites, bricks.data(), (size_t)r->sbytes, hipMemcpyHostToDevice, st));
   if (!keys.empty())
     HIP_TRY(hipMemcpyAsync(r->vac, keys.data(), keys.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
   r->nvac = (int32_t)keys.size();
@@ -291,7 +328,8 @@ int upload_rep(HostRep* r, const uint8_t* sites, hipStream_t st) {
 // ---------------------------------------------------------------- launches
 struct Tuning {
   int32_t vcap = 256;   // LDS class capacity (-1: no LDS class)
-  int32_t waves = 2;
+  int32_t il = 1;       // interleave width: replica waves per SIMD (1, 2, 4)
+  int32_t waves = 2;    // waves per workgroup (4 * il when il > 1)
   int32_t order = 1;
   int64_t slice = 1024;
 };
@@ -375,13 +413,13 @@ struct Runner {
     grid_lds = grid_glob = 0;
     if (n_lds) {
       int per_cu = 0;
-      if (akmc_batch_occupancy(fp32, 1, tun.waves, region_lds, &per_cu) || per_cu <= 0)
+      if (akmc_batch_occupancy(fp32, 1, tun.il, tun.waves, region_lds, &per_cu) || per_cu <= 0)
         return fail(AKMC_ERR_CAPACITY, "LDS class does not fit (vcap %d, %d waves)", tun.vcap, tun.waves);
       grid_lds = std::min(per_cu * n_cu, (n_lds + tun.waves - 1) / tun.waves);
     }
     if (n_glob) {
       int per_cu = 0;
-      if (akmc_batch_occupancy(fp32, 0, tun.waves, region_glob, &per_cu) || per_cu <= 0)
+      if (akmc_batch_occupancy(fp32, 0, tun.il, tun.waves, region_glob, &per_cu) || per_cu <= 0)
         return fail(AKMC_ERR_CAPACITY, "global class does not fit");
       grid_glob = std::min(per_cu * n_cu, (n_glob + tun.waves - 1) / tun.waves);
     }
@@ -418,13 +456,17 @@ struct Runner {
     if (n_glob) {
       hipStream_t s = both ? s1 : s0;
       const akmc::BatchArgs a = args(false);
-      if (akmc_batch_reset_launch(d_queue + 2, s)) return fail(AKMC_ERR_HIP, "reset launch failed");
-      if (akmc_batch_launch(&a, fp32, 0, tun.waves, grid_glob, s)) return fail(AKMC_ERR_HIP, "launch failed");
+      if (const int e = akmc_batch_reset_launch(d_queue + 2, s))
+        return fail(AKMC_ERR_HIP, "reset launch failed: %s", hipGetErrorString((hipError_t)e));
+      if (const int e = akmc_batch_launch(&a, fp32, 0, tun.il, tun.waves, grid_glob, s))
+        return fail(AKMC_ERR_HIP, "launch failed: %s", hipGetErrorString((hipError_t)e));
     }
     if (n_lds) {
       const akmc::BatchArgs a = args(true);
-      if (akmc_batch_reset_launch(d_queue, s0)) return fail(AKMC_ERR_HIP, "reset launch failed");
-      if (akmc_batch_launch(&a, fp32, 1, tun.waves, grid_lds, s0)) return fail(AKMC_ERR_HIP, "launch failed");
+      if (const int e = akmc_batch_reset_launch(d_queue, s0))
+        return fail(AKMC_ERR_HIP, "reset launch failed: %s", hipGetErrorString((hipError_t)e));
+      if (const int e = akmc_batch_launch(&a, fp32, 1, tun.il, tun.waves, grid_lds, s0))
+        return fail(AKMC_ERR_HIP, "launch failed: %s", hipGetErrorString((hipError_t)e));
     }
     if (both) {
       HIP_TRY(hipEventRecord(join, s1));
@@ -489,6 +531,7 @@ struct akmc_batch {
   uint8_t* pool = nullptr;
   std::vector<akmc::ReplicaState> h_state;
   akmc_batch_tuning last_tune{};
+  std::vector<akmc_batch_tuning> trials;  // every setting of the last akmc_batch_tune, with its rate
 };
 
 namespace {
@@ -660,10 +703,13 @@ int akmc_batch_create(const akmc_params* reps, int32_t n, const akmc_batch_optio
   b->R.use_graph = opt.use_graph != 0;
   b->R.flags = opt.error_stats ? AKMC_BF_ERROR_STATS : 0;
   b->R.tun.vcap = opt.lds_vcap > 0 ? opt.lds_vcap : (opt.lds_vcap < 0 ? -1 : 256);
-  b->R.tun.waves = opt.waves_per_group > 0 ? opt.waves_per_group : 2;
+  b->R.tun.il = opt.interleave > 0 ? opt.interleave : 1;
+  b->R.tun.waves = b->R.tun.il > 1 ? 4 * b->R.tun.il : (opt.waves_per_group > 0 ? opt.waves_per_group : 2);
   b->R.tun.order = opt.order >= 0 ? opt.order : 1;
   b->R.tun.slice = opt.slice_events > 0 ? opt.slice_events : 1024;
-  if (b->R.tun.waves != 1 && b->R.tun.waves != 2 && b->R.tun.waves != 4)
+  if (b->R.tun.il != 1 && b->R.tun.il != 2 && b->R.tun.il != 4)
+    return fail(AKMC_ERR_INVALID, "interleave must be 1, 2 or 4");
+  if (b->R.tun.il == 1 && b->R.tun.waves != 1 && b->R.tun.waves != 2 && b->R.tun.waves != 4)
     return fail(AKMC_ERR_INVALID, "waves_per_group must be 1, 2 or 4");
   rc = b->R.set_items(b->live, b->cap_of_slot);
   if (rc) return rc;
@@ -736,6 +782,7 @@ int akmc_batch_get_tuning(akmc_batch* b, akmc_batch_tuning* t) {
   memset(t, 0, sizeof(*t));
   t->lds_vcap = b->R.tun.vcap;
   t->waves_per_group = b->R.tun.waves;
+  t->interleave = b->R.tun.il;
   t->order = b->R.tun.order;
   t->slice_events = b->R.tun.slice;
   t->events_per_second = b->last_tune.events_per_second;
@@ -757,13 +804,18 @@ int akmc_batch_tune(akmc_batch* b, double trial_seconds, akmc_batch_tuning* chos
   Tuning best = keep;
   double best_rate = -1;
   int trials = 0;
+  b->trials.clear();
+  // Interleave width il > 1: workgroups of 4 * il waves, il per SIMD.
+  const std::pair<int32_t, int32_t> shapes[] = {{1, 1}, {1, 2}, {1, 4}, {2, 8}, {4, 16}};
   for (int32_t vc : vcaps)
-    for (int32_t w : {1, 2, 4})
+    for (const auto& sh : shapes)
       for (int32_t ord : {0, 1}) {
         Tuning t = keep;
         t.vcap = vc;
-        t.waves = w;
+        t.il = sh.first;
+        t.waves = sh.second;
         t.order = ord;
+        const int w = sh.second;
         if (vc > 0 && (int64_t)w * akmc_batch_region_bytes(b->R.fp32, vc) > 160 * 1024) continue;
         b->R.tun = t;
         if (b->R.set_items(b->live, b->cap_of_slot)) continue;  // does not fit: skip
@@ -772,6 +824,15 @@ int akmc_batch_tune(akmc_batch* b, double trial_seconds, akmc_batch_tuning* chos
         if (rc) return rc;
         trials++;
         const double rate = st.kernel_seconds > 0 ? st.events / st.kernel_seconds : 0;
+        akmc_batch_tuning rec{};
+        rec.lds_vcap = t.vcap;
+        rec.waves_per_group = t.waves;
+        rec.interleave = t.il;
+        rec.order = t.order;
+        rec.slice_events = t.slice;
+        rec.events_per_second = rate;
+        rec.trials = 1;
+        b->trials.push_back(rec);
         if (rate > best_rate) {
           best_rate = rate;
           best = t;
@@ -810,6 +871,13 @@ int akmc_batch_tune(akmc_batch* b, double trial_seconds, akmc_batch_tuning* chos
   return chosen ? akmc_batch_get_tuning(b, chosen) : AKMC_OK;
 }
 
+int akmc_batch_tune_trials(akmc_batch* b, akmc_batch_tuning* out, int32_t cap, int32_t* n) {
+  if (!b || !n || cap < 0 || (cap > 0 && !out)) return fail(AKMC_ERR_INVALID, "bad arguments");
+  *n = (int32_t)b->trials.size();
+  for (int32_t i = 0; i < std::min(cap, *n); i++) out[i] = b->trials[i];
+  return AKMC_OK;
+}
+
 int32_t akmc_batch_size(const akmc_batch* b) { return b ? (int32_t)b->reps.size() : 0; }
 
 int64_t akmc_batch_num_sites(const akmc_batch* b, int32_t i) {
@@ -833,8 +901,7 @@ int akmc_batch_download(akmc_batch* b, int32_t i, uint8_t* sites, int64_t n) {
                                           (long long)b->reps[i].nsites);
   HIP_TRY(hipSetDevice(b->R.device));
   HIP_TRY(hipStreamSynchronize(b->R.s0));
-  HIP_TRY(hipMemcpy(sites, b->reps[i].sites, n, hipMemcpyDeviceToHost));
-  return AKMC_OK;
+  return download_sites(b->reps[i], sites);
 }
 
 int akmc_batch_upload(akmc_batch* b, int32_t i, const uint8_t* sites, int64_t n) {
@@ -897,7 +964,7 @@ int akmc_batch_sector_rates(akmc_batch* b, int32_t i, int sector, int64_t cap, i
     a.queue = dq;
     a.flags = AKMC_BF_RATES_ONLY;
     a.rates_sector = sector;
-    if (akmc_batch_launch(&a, fp32, 0, 1, 1, b->R.s0)) {
+    if (akmc_batch_launch(&a, fp32, 0, 1, 1, 1, b->R.s0)) {
       rc = fail(AKMC_ERR_HIP, "rates launch failed");
       break;
     }
@@ -1174,7 +1241,8 @@ int retire(akmc_server* s, Job* j, int32_t rep_status) {
   } else {
     if (s->opt.keep_results) {
       j->result.resize(j->rep.nsites);
-      HIP_TRY(hipMemcpy(j->result.data(), j->rep.sites, j->rep.nsites, hipMemcpyDeviceToHost));
+      const int rc = download_sites(j->rep, j->result.data());
+      if (rc) return rc;
     }
     s->arena.release(j->rep.mem, j->rep.bytes);
     j->rep.mem = nullptr;
@@ -1401,6 +1469,7 @@ int akmc_server_create(int device, const akmc_server_options* o, akmc_server** o
   s->mix_job.assign(opt.max_live, nullptr);
   for (int32_t k = opt.max_live - 1; k >= 0; k--) s->mix_free.push_back(k);
   s->R.tun.vcap = opt.lds_vcap > 0 ? opt.lds_vcap : (opt.lds_vcap < 0 ? -1 : 256);
+  s->R.tun.il = 1;
   s->R.tun.waves = opt.waves_per_group > 0 ? opt.waves_per_group : 2;
   s->R.tun.order = 1;
   s->R.tun.slice = opt.slice_events > 0 ? opt.slice_events : 4096;

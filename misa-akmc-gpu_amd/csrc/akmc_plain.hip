@@ -71,27 +71,7 @@ __device__ int collect_sector(SecT& S, const LatticeView& L, int s, int cap) {
   }
   if (n > cap) return -1;
   wave_sync();
-  // Bitonic network in its all-ascending form: each stage first compares i
-  // with its mirror i ^ (k - 1), then with i ^ d for d = k/4 .. 1.  Every
-  // comparator puts the smaller key at the lower index, so positions >= n act
-  // as +infinity and are never read or written: no padding beyond n.
-  int n2 = 1;
-  while (n2 < n) n2 <<= 1;
-  for (int k = 2; k <= n2; k <<= 1) {
-    for (int d = k - 1; d > 0; d = (d == k - 1) ? (k >> 2) : (d >> 1)) {
-      for (int i = l; i < n; i += 64) {
-        const int p = i ^ d;
-        if (p > i && p < n) {
-          const uint32_t a = S.key[i], b = S.key[p];
-          if (a > b) {
-            S.key[i] = b; S.key[p] = a;
-            const int32_t t = S.slot[i]; S.slot[i] = S.slot[p]; S.slot[p] = t;
-          }
-        }
-      }
-      wave_sync();
-    }
-  }
+  sort_by_key(S.key, S.slot, n);
   return n;
 }
 
@@ -266,10 +246,8 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
     // used when the hopped vacancy is re-evaluated below, after the global
     // latency has been hidden behind the list update.
     uint8_t pf = 0;
-    {
-      const uint64_t q = stencil_site(L.d, T.st, LS, (uint32_t)uni_i((int)to));
-      if (l < T.st.n_sites) pf = L.sites[q];
-    }
+    const uint64_t pq = stencil_site(L.d, T.st, LS, (uint32_t)uni_i((int)to));
+    if (l < T.st.n_sites) pf = L.sites[pq];
     const uint8_t ff = S.own[vsel];
     const uint8_t gt = S.nb1[vsel * 8 + bsel];
     const uint8_t xb = (gt & BIT_CU) ? BIT_CU : (gt & BIT_MN) ? BIT_MN : (gt & BIT_NI) ? BIT_NI : 0;
@@ -351,7 +329,13 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
         hit = (v < ihi) & (in_stencil(L.d, T.st, w, from) | in_stencil(L.d, T.st, w, to));
       }
       const uint64_t m = ballot(hit);
-      if (m) {
+      if (stays && knew >= b0 && knew < b0 + 64 && m == 1ull << (knew - b0)) {
+        // Only the vacancy that hopped: from the stencil gathered at the hop.
+        amin = min(amin, knew);
+        amax = max(amax, knew);
+        eval_hopped(S, T, LS, L, knew, to, pt, pf, pq);
+        st.n_eval += 1;
+      } else if (m) {
         const int r = __popcll(m & ((1ull << l) - 1));
         if (hit) S.list[r] = v;
         amin = min(amin, b0 + first_set(m));

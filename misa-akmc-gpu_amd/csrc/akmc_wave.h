@@ -199,6 +199,33 @@ struct Patch {
   uint64_t i0, i1;  // site indices of k0 and k1
 };
 
+// ---------------------------------------------------------------- sort
+// Sorts key[0..n) ascending (sweep order) and slot[] with it, by a bitonic
+// network in its all-ascending form: each stage first compares i with its
+// mirror i ^ (k - 1), then with i ^ d for d = k/4 .. 1.  Every comparator
+// puts the smaller key at the lower index, so positions >= n act as
+// +infinity and are never read or written: no padding beyond n.
+__device__ __forceinline__ void sort_by_key(uint32_t* key, int32_t* slot, int n) {
+  const int l = lane_id();
+  int n2 = 1;
+  while (n2 < n) n2 <<= 1;
+  for (int k = 2; k <= n2; k <<= 1) {
+    for (int d = k - 1; d > 0; d = (d == k - 1) ? (k >> 2) : (d >> 1)) {
+      for (int i = l; i < n; i += 64) {
+        const int p = i ^ d;
+        if (p > i && p < n) {
+          const uint32_t a = key[i], b = key[p];
+          if (a > b) {
+            key[i] = b; key[p] = a;
+            const int32_t t = slot[i]; slot[i] = slot[p]; slot[p] = t;
+          }
+        }
+      }
+      wave_sync();
+    }
+  }
+}
+
 // ---------------------------------------------------------------- searches
 // First index i in [0, n) with pred(a[i]), n if none, for a sorted so that
 // pred is false and then true.  Each round splits the remaining range into
@@ -665,6 +692,50 @@ __device__ void eval_list(SecT& S, const SharedTables& T, const LaneStencil& LS,
     }
     wave_sync();
   }
+}
+
+// eval_list for one vacancy: the one that just hopped (sorted index vi, key
+// `to`, own flags pt.f1), which is the only vacancy re-evaluated after most
+// hops.  Its stencil bytes were gathered by the caller right after the hop
+// (pf, lane = stencil site, site index pq = stencil_site(to)), so there are
+// no index loads and no per-slot loops; the operations on the rates and the
+// side effect are eval_list's.
+template <class SecT>
+__device__ __forceinline__ void eval_hopped(SecT& S, const SharedTables& T, const LaneStencil& LS,
+                                            const LatticeView& L, int vi, uint32_t to, const Patch& pt, uint8_t pf,
+                                            uint64_t pq) {
+  const int l = lane_id();
+  uint32_t f = pf;
+  if (pq == pt.i0) f = pt.f0;
+  if (pq == pt.i1) f = pt.f1;
+  if (l < T.st.n_sites) {
+    S.sten[0][l] = (uint8_t)readout((uint8_t)f);
+    if (l >= 1 && l <= 8) S.nb1[vi * 8 + (l - 1)] = (uint8_t)f;
+  }
+  wave_sync();
+  const int b = l & 7;
+  const bool act = l < 8;
+  double r = 0.0;
+  int X = SP_V;
+  if (act) {
+    r = rate_fp64_staged(S.sten[0], (int)(key_x(to) & 1), LS, T.e1, T.e2, T.e0, L.nu, L.kT, T.exp_tab, X);
+    S.rate[vi * 8 + b] = (typename SecT::rate_t)r;
+  }
+  const int ti = act ? tie_index((double)(typename SecT::rate_t)r) : -1;
+  const uint64_t tmv = or8_u64(ti >= 0 ? 1ull << ti : 0ull);
+  const uint64_t mcu = ballot(act & (X == SP_CU));
+  const uint64_t mmn = ballot(act & (X == SP_MN));
+  const uint64_t mni = ballot(act & (X == SP_NI));
+  wave_sync();
+  if (l == 0) {
+    const uint8_t clr = (uint8_t)((mcu ? BIT_CU : 0) | (mmn ? BIT_MN : 0) | (mni ? BIT_NI : 0));
+    const uint8_t f0 = pt.f1;
+    const uint8_t nf = (uint8_t)(f0 & ~clr);
+    if (nf != f0) L.sites[key_index(L.d, to)] = nf;
+    S.own[vi] = nf;
+    S.tmask[vi] = tmv;
+  }
+  wave_sync();
 }
 
 }  // namespace akmc

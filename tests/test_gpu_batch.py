@@ -154,6 +154,8 @@ def test_results_do_not_depend_on_batch_slicing_or_tuning():
         dict(lds_vcap=-1, waves_per_group=1),            # global kind only
         dict(lds_vcap=64, waves_per_group=4, order=0),
         dict(use_graph=0, slice_events=333),
+        dict(interleave=2),                               # two replica waves per SIMD
+        dict(interleave=4, lds_vcap=-1),
     ]
     for v in variants:
         runs = v.pop("runs", 1)
@@ -244,7 +246,14 @@ def test_autotuner_picks_a_valid_setting():
     cfgs = _mixed_configs(32, seed=41)
     b = akmc.Batch(cfgs, init=akmc.AKMC_INIT_DEVICE)
     t = b.tune(trial_seconds=0.001)
-    assert t["trials"] >= 6 and t["waves_per_group"] in (1, 2, 4) and t["events_per_second"] > 0
+    assert t["trials"] >= 6 and t["events_per_second"] > 0
+    assert t["interleave"] in (1, 2, 4)
+    assert t["waves_per_group"] == (4 * t["interleave"] if t["interleave"] > 1 else t["waves_per_group"])
+    assert t["waves_per_group"] in (1, 2, 4, 8, 16)
+    trials = b.tune_trials()
+    assert len(trials) == t["trials"]
+    assert {x["interleave"] for x in trials} == {1, 2, 4}          # the interleave widths are swept
+    assert max(x["events_per_second"] for x in trials) == t["events_per_second"]
     done = [b.replica_stats(i)["events"] for i in range(32)]
     assert len(set(done)) == 1 and done[0] > 0    # every replica brought to the same count
     b.run(500)
@@ -294,3 +303,33 @@ def test_server_alone_vs_mix_and_budget():
         srv.submit(make_cfg(n=200, va=10), 10)   # 16 MB of sites alone exceeds the budget
     assert e.value.code == -6
     srv.close()
+
+
+def test_brick_layout_round_trips_and_device_creation():
+    # Replica sites live in 8 x 4 x 4 bricks on the device; the host sees the
+    # logical order.  An odd box (padding on every axis): the vacancy list of a
+    # device-created lattice sits exactly on the V-tagged sites of its
+    # download, events conserve the vacancies, and upload . download is the
+    # identity for an evolved lattice and for arbitrary tag bytes.
+    say("test_brick_layout_round_trips_and_device_creation")
+    cfg = make_cfg(box=(17, 13, 11), va=40, T=600.0)
+    b = akmc.Batch([cfg, cfg], precision=akmc.AKMC_PREC_FP64, init=akmc.AKMC_INIT_DEVICE)
+    s0 = b.download(0)
+    assert s0.size == 2 * 17 * 13 * 11
+    X, Y = 2 * 17, 13
+    keys = set()
+    for sec in range(8):
+        gs, _, _ = b.sector_rates(0, sec)
+        keys.update(int(k) for k in gs)
+    assert keys == set(int(i) for i in np.flatnonzero(s0 & 1))
+    b.run(3000)
+    s1 = b.download(0)
+    assert int((s1 & 1).sum()) == int((s0 & 1).sum())
+    b.upload(1, s1)
+    assert np.array_equal(b.download(1), s1)
+    rng = np.random.default_rng(5)
+    junk = rng.integers(0, 16, size=s1.size, dtype=np.uint8) & np.uint8(0x0E)
+    junk[np.flatnonzero(s1 & 1)] |= 1
+    b.upload(1, junk)
+    assert np.array_equal(b.download(1), junk)
+    b.close()
