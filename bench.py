@@ -45,7 +45,7 @@ HBM_PEAK_GBS = 8000.0
 # 1 GiB read of the library's copy kernel; WRITE_SIZE reads it exactly):
 # profiles/plain_fp64_256cube_hbm_counters.md, profiles/replica_fp32_1024_hbm_counters.md.
 PLAIN_TRAFFIC_BYTES_PER_EVENT = 75.0
-REPLICA_TRAFFIC_BYTES_PER_EVENT = 248.9
+REPLICA_TRAFFIC_BYTES_PER_EVENT = 60.1
 
 WORKLOAD = dict(box=(256, 256, 256), alloy=[("Fe", 9666), ("Cu", 134), ("Mn", 130), ("Ni", 70)], va_count=6711,
                 seeds=dict(create_types=100565, create_vacancy=16650, event_selection=563264, time_inc=594689),
