@@ -628,6 +628,9 @@ int akmc_run(akmc_engine* e, int64_t max_events, akmc_stats* stats) {
     }
     if (st == AKMC_RUN_ERR_CAPACITY)
       return fail(AKMC_ERR_CAPACITY, "a sector holds more than %d vacancies", e->sector_cap);
+    if (st == AKMC_RUN_ERR_HELPER)
+      return fail(AKMC_ERR_HIP, "event loop: the helper wavefront did not answer (event %lld)",
+                  (long long)e->h_state.events);
     return fail(AKMC_ERR_HIP, "event loop returned status %d", st);
   }
   const auto t1 = std::chrono::steady_clock::now();

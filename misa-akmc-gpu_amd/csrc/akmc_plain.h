@@ -13,7 +13,8 @@ enum {
   AKMC_RUN_NEED_RNG = 3,       // the uploaded random-number chunk is used up
   AKMC_RUN_YIELD = 4,          // sector-visit budget of one launch used up; relaunch
   AKMC_RUN_ERR_NO_EVENT = -10, // u * sum >= sum: no event above the target
-  AKMC_RUN_ERR_CAPACITY = -11  // a sector holds more vacancies than the kernel's capacity
+  AKMC_RUN_ERR_CAPACITY = -11, // a sector holds more vacancies than the kernel's capacity
+  AKMC_RUN_ERR_HELPER = -12    // the helper wavefront did not answer within its bound
 };
 
 namespace akmc {

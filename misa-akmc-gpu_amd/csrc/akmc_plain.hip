@@ -112,6 +112,182 @@ __device__ void move_entry(SecT& S, int nv, int from_i, int to_i) {
   }
 }
 
+// The list edit after a hop of entry vsel to key `to`: the rank of `to`
+// among the sector's other keys (the hop moves at most one z-plane, so a
+// window around the old position first), the entries between moved by one,
+// and the key written at its new position; an entry that left the sector is
+// deleted.  Returns the new position (-1 if it left) and updates nv.
+template <class SecT>
+__device__ __forceinline__ int rank_and_move(SecT& S, int& nv, int vsel, uint32_t to, bool stays, int32_t vslot) {
+  const int l = lane_id();
+  int knew = -1;
+  if (stays) {
+    const int w0 = max(0, min(vsel - 32, nv - 64));
+    const int wv = w0 + l;
+    const uint32_t kw = S.key[min(wv, nv - 1)];
+    const uint64_t below = ballot((wv < nv) & (wv != vsel) & (kw < to));
+    // Every key before the window is below `to` and every key after it above.
+    const bool inside = (w0 == 0 || S.key[w0 - 1] < to) && (w0 + 64 >= nv || S.key[w0 + 64] > to);
+    if (inside) {
+      knew = w0 + __popcll(below) - (vsel < w0 ? 1 : 0);
+    } else {
+      const int lb = lower_bound_key(S.key, nv, to);
+      knew = lb > vsel ? lb - 1 : lb;
+    }
+  }
+  move_entry(S, nv, vsel, stays ? knew : -1);
+  if (stays) {
+    if (l == 0) { S.key[knew] = to; S.slot[knew] = vslot; }
+  } else {
+    nv -= 1;
+  }
+  wave_sync();
+  return knew;
+}
+
+// Candidates of re-evaluation: sorted positions whose keys lie within 2
+// z-planes of the hop (the stencil's reach; `to` is one plane from `from`),
+// unwrapped since a sector never wraps around the box.  A 64-entry window
+// around position `pos` usually holds them all.
+template <class SecT>
+__device__ __forceinline__ void cand_range(const SecT& S, int nv, int pos, int zlo, int zhi, int Z, int& ilo,
+                                           int& ihi) {
+  const int c0 = max(0, min(pos - 32, nv - 64));
+  const bool covers = (c0 == 0 || (int)key_z(S.key[c0 - 1]) < zlo) &&
+                      (c0 + 64 >= nv || (int)key_z(S.key[c0 + 64]) > zhi);
+  if (covers) {
+    ilo = c0;
+    ihi = min(nv, c0 + 64);
+  } else {
+    ilo = zlo <= 0 ? 0 : lower_bound_key(S.key, nv, make_key(0, 0, (uint32_t)zlo));
+    ihi = zhi + 1 >= Z ? nv : lower_bound_key(S.key, nv, make_key(0, 0, (uint32_t)(zhi + 1)));
+  }
+}
+
+// True if no position of [ilo, ihi) holds a vacancy whose stencil contains
+// `from` or `to`, apart from the one that hopped (position knew, when it
+// stayed in the sector): then it is the only one to re-evaluate.
+template <class SecT>
+__device__ __forceinline__ bool only_hopped(const SecT& S, const BoxDims& d, const StencilTables& st, int ilo,
+                                            int ihi, int knew, uint32_t from, uint32_t to) {
+  const int l = lane_id();
+  bool only = true;
+  for (int b0 = ilo; b0 < ihi; b0 += 64) {
+    const int v = b0 + l;
+    const uint32_t w = S.key[min(v, ihi - 1)];
+    const bool hit = (v < ihi) & (v != knew) & (in_stencil(d, st, w, from) | in_stencil(d, st, w, to));
+    only = only & (ballot(hit) == 0);
+  }
+  return only;
+}
+
+// ---------------------------------------------------------------- helper wave
+// The LDS kernel runs a second wavefront beside the event loop.  After a hop
+// the event wave hands it the list edit and the candidate search (rank of the
+// new key, entries moved, candidate range, whether any vacancy besides the
+// hopped one needs re-evaluation) and meanwhile evaluates the hopped
+// vacancy's rates into registers; it then waits for the answer and stores
+// the rates at the new position.  Commands go through a ring in LDS whose
+// counters lane 0 writes after an LDS release fence and every lane reads
+// before an LDS acquire fence (the handed-over state is all in LDS; a fence
+// over global memory would also wait for the hop's stores to the lattice).
+// Every wait is bounded: one that runs out ends the launch with
+// AKMC_RUN_ERR_HELPER instead of hanging.  Handing over the tail's region
+// shifts as well was measured and not kept (posting and waiting cost more
+// than the stores).
+enum : int32_t { CMD_MOVE = 1, CMD_EXIT = 3 };
+constexpr int MB_RING = 16;
+constexpr int64_t MB_SPIN_LIMIT = (int64_t)1 << 26;  // polls of ~100 cycles: seconds
+struct MbCmd {
+  int32_t op;
+  int32_t vsel, nv, stays;               // MOVE arguments
+  uint32_t from, to;
+  int32_t vslot, zlo, zhi;
+  int32_t knew, nv_out, ilo, ihi, only;  // MOVE results
+};
+struct Mailbox {
+  int32_t posted;  // commands posted by the event wave
+  int32_t done;    // commands completed by the helper wave
+  int32_t pad[2];
+  MbCmd q[MB_RING];
+};
+
+__device__ __forceinline__ void lds_acquire() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local"); }
+__device__ __forceinline__ int mb_load(int32_t* p) {
+  const int v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  lds_acquire();
+  return uni_i(v);
+}
+__device__ __forceinline__ void mb_store(int32_t* p, int32_t v) {
+  // The wave's earlier LDS writes (every lane's) complete before the count.
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  if (lane_id() == 0) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Event wave: the slot of the next command (seq = commands posted so far),
+// once the ring has room; nullptr if it stayed full past the spin limit.
+// Lane 0 fills the slot, then mb_commit posts it.
+__device__ __forceinline__ MbCmd* mb_reserve(Mailbox& mb, int seq) {
+  int64_t spins = 0;
+  while (seq - mb_load(&mb.done) >= MB_RING) {
+    if (++spins > MB_SPIN_LIMIT) return nullptr;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return &mb.q[seq & (MB_RING - 1)];
+}
+__device__ __forceinline__ int mb_commit(Mailbox& mb, int& seq) {
+  const int k = seq;
+  mb_store(&mb.posted, k + 1);
+  seq = k + 1;
+  return k;
+}
+// Event wave: waits until commands 0 .. k are done.
+__device__ __forceinline__ bool mb_wait(Mailbox& mb, int k) {
+  int64_t spins = 0;
+  while (mb_load(&mb.done) <= k) {
+    if (++spins > MB_SPIN_LIMIT) return false;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return true;
+}
+
+template <class SecT>
+__device__ void helper_loop(SecT& S, const SharedTables& T, const BoxDims d, Mailbox& mb) {
+  const int l = lane_id();
+  for (int k = 0;; k++) {
+    int64_t spins = 0;
+    bool got = true;
+    while (mb_load(&mb.posted) <= k) {
+      // The event wave always posts CMD_EXIT before it ends; this bound only
+      // keeps a broken protocol from running forever.
+      if (++spins > 4 * MB_SPIN_LIMIT) { got = false; break; }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    if (!got) return;
+    MbCmd& c = mb.q[k & (MB_RING - 1)];
+    const int op = uni_i(c.op);
+    if (op == CMD_EXIT) return;
+    if (op == CMD_MOVE) {
+      int nv = uni_i(c.nv);
+      const int vsel = uni_i(c.vsel);
+      const bool stays = uni_i(c.stays) != 0;
+      const uint32_t from = (uint32_t)uni_i((int)c.from), to = (uint32_t)uni_i((int)c.to);
+      const int knew = rank_and_move(S, nv, vsel, to, stays, uni_i(c.vslot));
+      int ilo, ihi;
+      cand_range(S, nv, stays ? knew : vsel, uni_i(c.zlo), uni_i(c.zhi), d.Z, ilo, ihi);
+      const bool only = only_hopped(S, d, T.st, ilo, ihi, stays ? knew : -1, from, to);
+      if (l == 0) {
+        c.knew = knew;
+        c.nv_out = nv;
+        c.ilo = ilo;
+        c.ihi = ihi;
+        c.only = only ? 1 : 0;
+      }
+    }
+    mb_store(&mb.done, k + 1);
+  }
+}
+
 // Self-check: the running sums and the total against a serial refold of the
 // sector's rates in sweep order (every lane folds the same values, so the
 // result is wave-uniform).
@@ -130,8 +306,11 @@ __device__ bool sums_agree(const SecT& S, int nv, double sum) {
 
 // The event loop over a sector state S of capacity `cap` (LDS or device
 // memory); T and evo[8] are LDS.
-template <class SecT>
-__device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap, SharedTables& T, double* evo) {
+// HELPER: a helper wavefront runs helper_loop on the same LDS state and mb
+// is its mailbox (the LDS kernel); otherwise mb is unused.
+template <class SecT, bool HELPER>
+__device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap, SharedTables& T, double* evo,
+                                           Mailbox* mb) {
   const int l = lane_id();
   stage_tables(T);
   LaneStencil LS = lane_stencil(T.st);
@@ -147,6 +326,7 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
   int s = ring_sector(st.sect);
   int32_t status = AKMC_RUN_OK;
   int64_t visits = 0;  // sector visits this launch (bounded so a launch always returns)
+  int mb_seq = 0;      // commands posted to the helper wave
   // Random numbers of the next event, loaded one event ahead.
   double u_next = 0.0, l_next = 0.0;
   if (a.rates_sector < 0 && a.u_stream && a.l_stream && st.events >= a.rng_base && st.events < a.rng_end) {
@@ -275,74 +455,78 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
 
     // ---- the next event's calcRates, incrementally.
     const bool stays = key_sector(L.d, to) == s;
-    int knew = -1;
-    if (stays) {
-      // Rank of `to` among the other keys; the hop moves at most one z-plane,
-      // so look in a window around the old position first.
-      const int w0 = max(0, min(vsel - 32, nv - 64));
-      const int wv = w0 + l;
-      const uint32_t kw = S.key[min(wv, nv - 1)];
-      const uint64_t below = ballot((wv < nv) & (wv != vsel) & (kw < to));
-      // Every key before the window is below `to` and every key after it above.
-      const bool inside = (w0 == 0 || S.key[w0 - 1] < to) && (w0 + 64 >= nv || S.key[w0 + 64] > to);
-      if (inside) {
-        knew = w0 + __popcll(below) - (vsel < w0 ? 1 : 0);
-      } else {
-        const int lb = lower_bound_key(S.key, nv, to);
-        knew = lb > vsel ? lb - 1 : lb;
-      }
-    }
-    move_entry(S, nv, vsel, stays ? knew : -1);
-    if (stays) {
-      if (l == 0) { S.key[knew] = to; S.slot[knew] = vslot; }
-    } else {
-      nv -= 1;
-    }
-    wave_sync();
-    // Candidates: sorted keys within 3 z-planes of the hop (the stencil reaches
-    // 2 planes; `to` is one plane from `from`), unwrapped since a sector never
-    // wraps around the box.
     const int zf = (int)key_z(from);
     const int zlo = min(zf, zf + oz) - 2, zhi = max(zf, zf + oz) + 2;
-    int ilo, ihi;
-    {
-      // A 64-entry window around the hop usually holds every candidate.
-      const int pos = stays ? knew : vsel;
-      const int c0 = max(0, min(pos - 32, nv - 64));
-      const bool covers = (c0 == 0 || (int)key_z(S.key[c0 - 1]) < zlo) &&
-                          (c0 + 64 >= nv || (int)key_z(S.key[c0 + 64]) > zhi);
-      if (covers) {
-        ilo = c0;
-        ihi = min(nv, c0 + 64);
-      } else {
-        ilo = zlo <= 0 ? 0 : lower_bound_key(S.key, nv, make_key(0, 0, (uint32_t)zlo));
-        ihi = zhi + 1 >= L.d.Z ? nv : lower_bound_key(S.key, nv, make_key(0, 0, (uint32_t)(zhi + 1)));
+    int knew = -1, ilo = 0, ihi = 0;
+    bool only = false;
+    HoppedEval hop{};
+    if (HELPER) {
+      // The helper wave edits the list and finds the candidates while this
+      // wave evaluates the hopped vacancy's rates (they do not depend on its
+      // new position).
+      // Every earlier command is done (this wave waited for it), so the slot
+      // is free.
+      MbCmd* c = &mb->q[mb_seq & (MB_RING - 1)];
+      if (l == 0) {
+        c->op = CMD_MOVE;
+        c->vsel = vsel;
+        c->nv = nv;
+        c->stays = stays ? 1 : 0;
+        c->from = from;
+        c->to = to;
+        c->vslot = vslot;
+        c->zlo = zlo;
+        c->zhi = zhi;
       }
+      const int km = mb_commit(*mb, mb_seq);
+      if (stays) {
+        const Patch pt0{from, to, nf_from, nf_to, -1, key_index(L.d, from), key_index(L.d, to)};
+        hop = eval_hopped_rates(S, T, LS, L, to, pt0, pf, pq);
+      }
+      if (!mb_wait(*mb, km)) { status = AKMC_RUN_ERR_HELPER; break; }
+      const MbCmd& r = mb->q[km & (MB_RING - 1)];
+      knew = uni_i(r.knew);
+      nv = uni_i(r.nv_out);
+      ilo = uni_i(r.ilo);
+      ihi = uni_i(r.ihi);
+      only = uni_i(r.only) != 0;
+    } else {
+      knew = rank_and_move(S, nv, vsel, to, stays, vslot);
+      cand_range(S, nv, stays ? knew : vsel, zlo, zhi, L.d.Z, ilo, ihi);
     }
     int amin = 0x7FFFFFFF, amax = -1;
     const Patch pt{from, to, nf_from, nf_to, stays ? knew : -1, key_index(L.d, from), key_index(L.d, to)};
-    for (int b0 = ilo; b0 < ihi; b0 += 64) {
-      const int v = b0 + l;
-      bool hit = false;
-      {
-        const uint32_t w = S.key[min(v, ihi - 1)];
-        hit = (v < ihi) & (in_stencil(L.d, T.st, w, from) | in_stencil(L.d, T.st, w, to));
-      }
-      const uint64_t m = ballot(hit);
-      if (stays && knew >= b0 && knew < b0 + 64 && m == 1ull << (knew - b0)) {
-        // Only the vacancy that hopped: from the stencil gathered at the hop.
-        amin = min(amin, knew);
-        amax = max(amax, knew);
-        eval_hopped(S, T, LS, L, knew, to, pt, pf, pq);
+    if (HELPER && only) {
+      // Only the vacancy that hopped changed its stencil (or none stayed).
+      if (stays) {
+        eval_hopped_store(S, L, knew, to, pt, hop);
+        amin = amax = knew;
         st.n_eval += 1;
-      } else if (m) {
-        const int r = __popcll(m & ((1ull << l) - 1));
-        if (hit) S.list[r] = v;
-        amin = min(amin, b0 + first_set(m));
-        amax = max(amax, b0 + 63 - (int)__clzll(m));
-        wave_sync();
-        eval_list(S, T, LS, L, S.list, 0, __popcll(m), pt, true, pf, stays ? knew : -1);
-        st.n_eval += __popcll(m);
+      }
+    } else {
+      for (int b0 = ilo; b0 < ihi; b0 += 64) {
+        const int v = b0 + l;
+        bool hit = false;
+        {
+          const uint32_t w = S.key[min(v, ihi - 1)];
+          hit = (v < ihi) & (in_stencil(L.d, T.st, w, from) | in_stencil(L.d, T.st, w, to));
+        }
+        const uint64_t m = ballot(hit);
+        if (stays && knew >= b0 && knew < b0 + 64 && m == 1ull << (knew - b0)) {
+          // Only the vacancy that hopped: from the stencil gathered at the hop.
+          amin = min(amin, knew);
+          amax = max(amax, knew);
+          eval_hopped(S, T, LS, L, knew, to, pt, pf, pq);
+          st.n_eval += 1;
+        } else if (m) {
+          const int r = __popcll(m & ((1ull << l) - 1));
+          if (hit) S.list[r] = v;
+          amin = min(amin, b0 + first_set(m));
+          amax = max(amax, b0 + 63 - (int)__clzll(m));
+          wave_sync();
+          eval_list(S, T, LS, L, S.list, 0, __popcll(m), pt, true, pf, stays ? knew : -1);
+          st.n_eval += __popcll(m);
+        }
       }
     }
     // ---- running sums.
@@ -364,6 +548,13 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
     }
     if (a.check_sums && !sums_agree(S, nv, sum)) st.sum_errors++;
   }
+  if (HELPER) {
+    MbCmd* c = mb_reserve(*mb, mb_seq);
+    if (c) {
+      if (l == 0) c->op = CMD_EXIT;
+      mb_commit(*mb, mb_seq);
+    }
+  }
   if (l == 0 && a.rates_sector < 0) {
     st.status = status;
     for (int i = 0; i < 8; i++) st.evo[i] = evo[i];
@@ -372,11 +563,21 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
   if (l == 0 && a.rates_sector >= 0 && status != AKMC_RUN_OK) *a.out_n = status;
 }
 
-__global__ __launch_bounds__(64) void plain_kernel(PlainArgs a) {
+// Two wavefronts: the event loop (wave 0) and its helper (wave 1).
+__global__ __launch_bounds__(128) void plain_kernel(PlainArgs a) {
   __shared__ PlainSector S;
   __shared__ SharedTables T;
   __shared__ double evo[8];  // sector clocks (indexed by the running sector id)
-  plain_body(a, S, PLAIN_VCAP, T, evo);
+  __shared__ Mailbox mb;
+  if (threadIdx.x == 0) {
+    mb.posted = 0;
+    mb.done = 0;
+  }
+  __syncthreads();
+  if (threadIdx.x < 64)
+    plain_body<PlainSector, true>(a, S, PLAIN_VCAP, T, evo, &mb);
+  else
+    helper_loop(S, T, a.dims, mb);
 }
 
 // The same loop with the sector arrays in device memory, for sectors of more
@@ -393,7 +594,7 @@ __global__ __launch_bounds__(64) void plain_kernel_hbm(PlainArgs a) {
   SectorRef S{reinterpret_cast<uint32_t*>(w + off[0]), reinterpret_cast<int32_t*>(w + off[1]),
               reinterpret_cast<double*>(w + off[2]),   reinterpret_cast<double*>(w + off[3]),
               reinterpret_cast<uint64_t*>(w + off[4]), w + off[5], w + off[6], w + off[7], sten, cflag, list};
-  plain_body(a, S, a.sector_cap, T, evo);
+  plain_body<SectorRef, false>(a, S, a.sector_cap, T, evo, nullptr);
 }
 
 }  // namespace akmc
@@ -402,7 +603,7 @@ extern "C" int akmc_plain_launch(const akmc::PlainArgs* args, hipStream_t stream
   if (args->sector_ws)
     hipLaunchKernelGGL(akmc::plain_kernel_hbm, dim3(1), dim3(64), 0, stream, *args);
   else
-    hipLaunchKernelGGL(akmc::plain_kernel, dim3(1), dim3(64), 0, stream, *args);
+    hipLaunchKernelGGL(akmc::plain_kernel, dim3(1), dim3(128), 0, stream, *args);
   return (int)hipGetLastError();
 }
 

@@ -694,48 +694,65 @@ __device__ void eval_list(SecT& S, const SharedTables& T, const LaneStencil& LS,
   }
 }
 
-// eval_list for one vacancy: the one that just hopped (sorted index vi, key
-// `to`, own flags pt.f1), which is the only vacancy re-evaluated after most
-// hops.  Its stencil bytes were gathered by the caller right after the hop
-// (pf, lane = stencil site, site index pq = stencil_site(to)), so there are
-// no index loads and no per-slot loops; the operations on the rates and the
-// side effect are eval_list's.
+// eval_list for one vacancy: the one that just hopped (key `to`, own flags
+// pt.f1), which is the only vacancy re-evaluated after most hops.  Its
+// stencil bytes were gathered by the caller right after the hop (pf, lane =
+// stencil site, site index pq = stencil_site(to)), so there are no index
+// loads and no per-slot loops; the operations on the rates and the side
+// effect are eval_list's.  Split in two: the rates into registers (nothing
+// depends on the vacancy's sorted index yet) and the stores at its index vi.
+struct HoppedEval {
+  double r;            // lane b < 8: rate of direction b
+  uint64_t tmv;        // tie mask of the eight rates
+  uint8_t clr;         // species tags to drop from the vacancy's own byte
+  uint8_t f;           // lane 1..8: the 1nn flag bytes
+};
 template <class SecT>
-__device__ __forceinline__ void eval_hopped(SecT& S, const SharedTables& T, const LaneStencil& LS,
-                                            const LatticeView& L, int vi, uint32_t to, const Patch& pt, uint8_t pf,
-                                            uint64_t pq) {
+__device__ __forceinline__ HoppedEval eval_hopped_rates(SecT& S, const SharedTables& T, const LaneStencil& LS,
+                                                        const LatticeView& L, uint32_t to, const Patch& pt,
+                                                        uint8_t pf, uint64_t pq) {
   const int l = lane_id();
   uint32_t f = pf;
   if (pq == pt.i0) f = pt.f0;
   if (pq == pt.i1) f = pt.f1;
-  if (l < T.st.n_sites) {
-    S.sten[0][l] = (uint8_t)readout((uint8_t)f);
-    if (l >= 1 && l <= 8) S.nb1[vi * 8 + (l - 1)] = (uint8_t)f;
-  }
+  if (l < T.st.n_sites) S.sten[0][l] = (uint8_t)readout((uint8_t)f);
   wave_sync();
-  const int b = l & 7;
   const bool act = l < 8;
   double r = 0.0;
   int X = SP_V;
-  if (act) {
-    r = rate_fp64_staged(S.sten[0], (int)(key_x(to) & 1), LS, T.e1, T.e2, T.e0, L.nu, L.kT, T.exp_tab, X);
-    S.rate[vi * 8 + b] = (typename SecT::rate_t)r;
-  }
+  if (act) r = rate_fp64_staged(S.sten[0], (int)(key_x(to) & 1), LS, T.e1, T.e2, T.e0, L.nu, L.kT, T.exp_tab, X);
   const int ti = act ? tie_index((double)(typename SecT::rate_t)r) : -1;
-  const uint64_t tmv = or8_u64(ti >= 0 ? 1ull << ti : 0ull);
+  HoppedEval h;
+  h.r = r;
+  h.tmv = or8_u64(ti >= 0 ? 1ull << ti : 0ull);
   const uint64_t mcu = ballot(act & (X == SP_CU));
   const uint64_t mmn = ballot(act & (X == SP_MN));
   const uint64_t mni = ballot(act & (X == SP_NI));
-  wave_sync();
+  h.clr = (uint8_t)((mcu ? BIT_CU : 0) | (mmn ? BIT_MN : 0) | (mni ? BIT_NI : 0));
+  h.f = (uint8_t)f;
+  return h;
+}
+template <class SecT>
+__device__ __forceinline__ void eval_hopped_store(SecT& S, const LatticeView& L, int vi, uint32_t to, const Patch& pt,
+                                                  const HoppedEval& h) {
+  const int l = lane_id();
+  if (l < 8) S.rate[vi * 8 + l] = (typename SecT::rate_t)h.r;
+  if (l >= 1 && l <= 8) S.nb1[vi * 8 + (l - 1)] = h.f;
   if (l == 0) {
-    const uint8_t clr = (uint8_t)((mcu ? BIT_CU : 0) | (mmn ? BIT_MN : 0) | (mni ? BIT_NI : 0));
     const uint8_t f0 = pt.f1;
-    const uint8_t nf = (uint8_t)(f0 & ~clr);
+    const uint8_t nf = (uint8_t)(f0 & ~h.clr);
     if (nf != f0) L.sites[key_index(L.d, to)] = nf;
     S.own[vi] = nf;
-    S.tmask[vi] = tmv;
+    S.tmask[vi] = h.tmv;
   }
   wave_sync();
+}
+template <class SecT>
+__device__ __forceinline__ void eval_hopped(SecT& S, const SharedTables& T, const LaneStencil& LS,
+                                            const LatticeView& L, int vi, uint32_t to, const Patch& pt, uint8_t pf,
+                                            uint64_t pq) {
+  const HoppedEval h = eval_hopped_rates(S, T, LS, L, to, pt, pf, pq);
+  eval_hopped_store(S, L, vi, to, pt, h);
 }
 
 }  // namespace akmc
