@@ -198,12 +198,14 @@ __device__ __forceinline__ bool only_hopped(const SecT& S, const BoxDims& d, con
 enum : int32_t { CMD_MOVE = 1, CMD_EXIT = 3 };
 constexpr int MB_RING = 16;
 constexpr int64_t MB_SPIN_LIMIT = (int64_t)1 << 26;  // polls of ~100 cycles: seconds
-struct MbCmd {
-  int32_t op;
-  int32_t vsel, nv, stays;               // MOVE arguments
-  uint32_t from, to;
-  int32_t vslot, zlo, zhi;
-  int32_t knew, nv_out, ilo, ihi, only;  // MOVE results
+// A command: arguments in three 16-byte words, results in two, so that each
+// side writes and reads them with a few wide LDS accesses.
+struct __align__(16) MbCmd {
+  int4 a0;  // op, vsel, nv, stays
+  int4 a1;  // from, to, vslot, zlo
+  int4 a2;  // zhi, -, -, -
+  int4 r0;  // knew, nv_out, ilo, ihi
+  int4 r1;  // only, -, -, -
 };
 struct Mailbox {
   int32_t posted;  // commands posted by the event wave
@@ -265,23 +267,21 @@ __device__ void helper_loop(SecT& S, const SharedTables& T, const BoxDims d, Mai
     }
     if (!got) return;
     MbCmd& c = mb.q[k & (MB_RING - 1)];
-    const int op = uni_i(c.op);
+    const int4 a0 = c.a0, a1 = c.a1, a2 = c.a2;
+    const int op = uni_i(a0.x);
     if (op == CMD_EXIT) return;
     if (op == CMD_MOVE) {
-      int nv = uni_i(c.nv);
-      const int vsel = uni_i(c.vsel);
-      const bool stays = uni_i(c.stays) != 0;
-      const uint32_t from = (uint32_t)uni_i((int)c.from), to = (uint32_t)uni_i((int)c.to);
-      const int knew = rank_and_move(S, nv, vsel, to, stays, uni_i(c.vslot));
+      int nv = uni_i(a0.z);
+      const int vsel = uni_i(a0.y);
+      const bool stays = uni_i(a0.w) != 0;
+      const uint32_t from = (uint32_t)uni_i(a1.x), to = (uint32_t)uni_i(a1.y);
+      const int knew = rank_and_move(S, nv, vsel, to, stays, uni_i(a1.z));
       int ilo, ihi;
-      cand_range(S, nv, stays ? knew : vsel, uni_i(c.zlo), uni_i(c.zhi), d.Z, ilo, ihi);
+      cand_range(S, nv, stays ? knew : vsel, uni_i(a1.w), uni_i(a2.x), d.Z, ilo, ihi);
       const bool only = only_hopped(S, d, T.st, ilo, ihi, stays ? knew : -1, from, to);
       if (l == 0) {
-        c.knew = knew;
-        c.nv_out = nv;
-        c.ilo = ilo;
-        c.ihi = ihi;
-        c.only = only ? 1 : 0;
+        c.r0 = make_int4(knew, nv, ilo, ihi);
+        c.r1 = make_int4(only ? 1 : 0, 0, 0, 0);
       }
     }
     mb_store(&mb.done, k + 1);
@@ -468,15 +468,9 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
       // is free.
       MbCmd* c = &mb->q[mb_seq & (MB_RING - 1)];
       if (l == 0) {
-        c->op = CMD_MOVE;
-        c->vsel = vsel;
-        c->nv = nv;
-        c->stays = stays ? 1 : 0;
-        c->from = from;
-        c->to = to;
-        c->vslot = vslot;
-        c->zlo = zlo;
-        c->zhi = zhi;
+        c->a0 = make_int4(CMD_MOVE, vsel, nv, stays ? 1 : 0);
+        c->a1 = make_int4((int)from, (int)to, vslot, zlo);
+        c->a2 = make_int4(zhi, 0, 0, 0);
       }
       const int km = mb_commit(*mb, mb_seq);
       if (stays) {
@@ -485,11 +479,13 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
       }
       if (!mb_wait(*mb, km)) { status = AKMC_RUN_ERR_HELPER; break; }
       const MbCmd& r = mb->q[km & (MB_RING - 1)];
-      knew = uni_i(r.knew);
-      nv = uni_i(r.nv_out);
-      ilo = uni_i(r.ilo);
-      ihi = uni_i(r.ihi);
-      only = uni_i(r.only) != 0;
+      const int4 r0 = r.r0;
+      const int4 r1 = r.r1;
+      knew = uni_i(r0.x);
+      nv = uni_i(r0.y);
+      ilo = uni_i(r0.z);
+      ihi = uni_i(r0.w);
+      only = uni_i(r1.x) != 0;
     } else {
       knew = rank_and_move(S, nv, vsel, to, stays, vslot);
       cand_range(S, nv, stays ? knew : vsel, zlo, zhi, L.d.Z, ilo, ihi);
@@ -551,7 +547,7 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
   if (HELPER) {
     MbCmd* c = mb_reserve(*mb, mb_seq);
     if (c) {
-      if (l == 0) c->op = CMD_EXIT;
+      if (l == 0) c->a0 = make_int4(CMD_EXIT, 0, 0, 0);
       mb_commit(*mb, mb_seq);
     }
   }
