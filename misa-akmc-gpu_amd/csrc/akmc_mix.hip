@@ -1004,7 +1004,7 @@ __global__ __launch_bounds__(64) void mix_kernel(const MixDesc* descs) {
         prefix_from(c, 0, nitl);
       }
       eval_near(c, S, 0, nr, cs, 0, false, true, nitl, list, st.n_eval);
-      sum = nr > 0 ? fold_range(S, 0, nr, 0.0) : 0.0;
+      sum = nr > 0 ? fold_range(S, 0, nr, 0.0, S.pvac) : 0.0;
       rebuild = false;
       if (D.rates_sector >= 0) {
         for (int i = l; i < nr && i < D.out_cap; i += 64) {
@@ -1304,7 +1304,7 @@ __global__ __launch_bounds__(64) void mix_kernel(const MixDesc* descs) {
     // Records before c_lo kept their sums; when nothing at or after it is
     // left (the last record deleted), the total is the last kept sum.
     if (nr == 0) sum = 0.0;
-    else if (c_lo < nr) sum = fold_range(S, c_lo, nr, c_lo > 0 ? D.pvac[c_lo - 1] : 0.0);
+    else if (c_lo < nr) sum = fold_range(S, c_lo, nr, c_lo > 0 ? D.pvac[c_lo - 1] : 0.0, S.pvac);
     else sum = uni_d(D.pvac[nr - 1]);
     st.nitl = nitl;  // Edumb terms are current again
   }

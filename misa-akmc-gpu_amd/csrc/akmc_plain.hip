@@ -8,7 +8,12 @@
 // (akmc_wave.h): the sector's vacancy list, rates and running sums stay in
 // LDS; after a hop only the vacancies whose 55-site stencil holds one of the
 // two changed sites are re-evaluated, and the running sums are repaired
-// exactly (block fold + tail shift) instead of re-folded.
+// exactly (block fold + tail shift) instead of re-folded.  In the LDS kernel
+// a second wavefront takes the list edit and the tail shift off the event
+// wave, which meanwhile selects the next event from sums known to within a
+// proven bound (and falls back to waiting for the exact ones when that
+// bound cannot decide), evaluates the hopped vacancy and advances the clock
+// only with the exact total.
 //
 // Evaluation points match the reference.  The reference re-evaluates every
 // vacancy of the sector at the start of every event (ABVIModel::calcRates,
@@ -36,7 +41,7 @@ __constant__ double c_e1[25] = AKMC_E1_TABLE;
 __constant__ double c_e2[25] = AKMC_E2_TABLE;
 __constant__ double c_e0[5] = AKMC_E0_TABLE;
 
-static constexpr int PLAIN_VCAP = 1536;
+static constexpr int PLAIN_VCAP = 1472;
 // SectorMeta ring 0,7,2,5,3,4,1,6 (sublattice.h:21-30), one nibble per
 // position: arithmetic instead of a table load in the event loop.
 __device__ __forceinline__ int ring_sector(int pos) { return (int)((0x61435270u >> (4 * pos)) & 7u); }
@@ -187,26 +192,48 @@ __device__ __forceinline__ bool only_hopped(const SecT& S, const BoxDims& d, con
 // new key, entries moved, candidate range, whether any vacancy besides the
 // hopped one needs re-evaluation) and meanwhile evaluates the hopped
 // vacancy's rates into registers; it then waits for the answer and stores
-// the rates at the new position.  Commands go through a ring in LDS whose
-// counters lane 0 writes after an LDS release fence and every lane reads
-// before an LDS acquire fence (the handed-over state is all in LDS; a fence
-// over global memory would also wait for the hop's stores to the lattice).
-// Every wait is bounded: one that runs out ends the launch with
-// AKMC_RUN_ERR_HELPER instead of hanging.  Handing over the tail's region
-// shifts as well was measured and not kept (posting and waiting cost more
-// than the stores).
-enum : int32_t { CMD_MOVE = 1, CMD_EXIT = 3 };
+// the rates at the new position.  After the block fold it hands over the
+// tail shift (TAIL) and goes on to the next event: it selects from the
+// exact prefix and the old tail sums plus the block's change, each within a
+// bound it checks, and collects the exact total before the clock advances.
+// Commands go through a ring in LDS whose counters lane 0 writes after an
+// LDS release fence and every lane reads before an LDS acquire fence (the
+// handed-over state is all in LDS; a fence over global memory would also
+// wait for the hop's stores to the lattice).  The helper runs commands in
+// order.  Every wait is bounded: one that runs out ends the launch with
+// AKMC_RUN_ERR_HELPER instead of hanging.  Handing over only the tail's
+// region shifts, with the event wave waiting for them, was measured and not
+// kept (posting and waiting cost more than the stores).
+enum : int32_t { CMD_MOVE = 1, CMD_TAIL = 2, CMD_EXIT = 3, CMD_COPY = 4 };
 constexpr int MB_RING = 16;
 constexpr int64_t MB_SPIN_LIMIT = (int64_t)1 << 26;  // polls of ~100 cycles: seconds
 // A command: arguments in three 16-byte words, results in two, so that each
 // side writes and reads them with a few wide LDS accesses.
+//   MOVE  a0 = op, vsel, nv, stays; a1 = from, to, vslot, zlo; a2 = zhi
+//         r0 = knew, nv_out, ilo, ihi; r1 = only
+//   TAIL  a0 = op, v0, n; a1 = Pn, Po (the bits of two doubles)
+//         r0 = total (bits), hi
+//   COPY  a0 = op
+// A tail repair writes the new sums into a separate array (pnew) and leaves
+// the old ones in pvac, where the event wave still reads them while it
+// selects the next event.  The COPY posted after that event's MOVE moves
+// pnew[v0, hi) into pvac: after the list edit, because the positions the
+// edit shifts are refolded anyway (a vacancy that left the sector shifts
+// every position from vsel on, which the fold then covers).
 struct __align__(16) MbCmd {
-  int4 a0;  // op, vsel, nv, stays
-  int4 a1;  // from, to, vslot, zlo
-  int4 a2;  // zhi, -, -, -
-  int4 r0;  // knew, nv_out, ilo, ihi
-  int4 r1;  // only, -, -, -
+  int4 a0;
+  int4 a1;
+  int4 a2;
+  int4 r0;
+  int4 r1;
 };
+__device__ __forceinline__ int4 pack_dd(double x, double y) {
+  const uint64_t a = as_u64(x), b = as_u64(y);
+  return make_int4((int)(uint32_t)a, (int)(uint32_t)(a >> 32), (int)(uint32_t)b, (int)(uint32_t)(b >> 32));
+}
+__device__ __forceinline__ double unpack_d(int lo, int hi) {
+  return as_f64(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
 struct Mailbox {
   int32_t posted;  // commands posted by the event wave
   int32_t done;    // commands completed by the helper wave
@@ -253,9 +280,27 @@ __device__ __forceinline__ bool mb_wait(Mailbox& mb, int k) {
   return true;
 }
 
+// Copies the new running sums [lo, hi) from pnew into pvac.
 template <class SecT>
-__device__ void helper_loop(SecT& S, const SharedTables& T, const BoxDims d, Mailbox& mb) {
+__device__ __forceinline__ void copy_sums(SecT& S, const double* pnew, int lo, int hi) {
   const int l = lane_id();
+  for (int c = lo; c < hi; c += 256) {
+    double x[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) x[k] = pnew[min(c + 64 * k + l, hi - 1)];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (c + 64 * k + l < hi) S.pvac[c + 64 * k + l] = x[k];
+  }
+  wave_sync();
+}
+
+template <class SecT>
+__device__ void helper_loop(SecT& S, const SharedTables& T, const BoxDims d, Mailbox& mb, double* pnew) {
+  const int l = lane_id();
+  int pend_lo = 0, pend_hi = 0;  // new sums of the last tail repair, not yet in pvac
+  int last_vsel = 0;             // the last MOVE's vacancy position, and
+  bool last_stays = true;        // whether it stayed in the sector
   for (int k = 0;; k++) {
     int64_t spins = 0;
     bool got = true;
@@ -274,6 +319,8 @@ __device__ void helper_loop(SecT& S, const SharedTables& T, const BoxDims d, Mai
       int nv = uni_i(a0.z);
       const int vsel = uni_i(a0.y);
       const bool stays = uni_i(a0.w) != 0;
+      last_vsel = vsel;
+      last_stays = stays;
       const uint32_t from = (uint32_t)uni_i(a1.x), to = (uint32_t)uni_i(a1.y);
       const int knew = rank_and_move(S, nv, vsel, to, stays, uni_i(a1.z));
       int ilo, ihi;
@@ -283,6 +330,18 @@ __device__ void helper_loop(SecT& S, const SharedTables& T, const BoxDims d, Mai
         c.r0 = make_int4(knew, nv, ilo, ihi);
         c.r1 = make_int4(only ? 1 : 0, 0, 0, 0);
       }
+    } else if (op == CMD_TAIL) {
+      // The exact running-sum repair behind the changed block.
+      const double Pn = uni_d(unpack_d(a1.x, a1.y)), Po = uni_d(unpack_d(a1.z, a1.w));
+      int hi = 0;
+      const double tot = tail_update(S, uni_i(a0.y), uni_i(a0.z), Pn, Po, pnew, hi);
+      pend_lo = uni_i(a0.y);
+      pend_hi = hi;
+      const uint64_t tb = as_u64(tot);
+      if (l == 0) c.r0 = make_int4((int)(uint32_t)tb, (int)(uint32_t)(tb >> 32), hi, 0);
+    } else if (op == CMD_COPY) {
+      copy_sums(S, pnew, pend_lo, last_stays ? pend_hi : min(pend_hi, last_vsel));
+      pend_lo = pend_hi = 0;
     }
     mb_store(&mb.done, k + 1);
   }
@@ -304,13 +363,48 @@ __device__ bool sums_agree(const SecT& S, int nv, double sum) {
   return ok & (p == (nv > 0 ? sum : 0.0));
 }
 
+// First index i in [0, n) whose estimated running sum exceeds t, n if none:
+// pvac[i] itself below pre_n, the old sum pvac[i] plus D from there on (a
+// tail whose repair is still running).  The search assumes sorted sums; the
+// caller checks the margins around whatever it returns.
+__device__ __forceinline__ int first_above_est(const double* pvac, int n, int pre_n, double D, double t) {
+  if (n <= 0) return 0;
+  const int l = lane_id();
+  auto pred = [=](int i) {
+    const double p = pvac[i];
+    return (i < pre_n ? p : p + D) > t;
+  };
+  int lo = 0, len = n;
+  while (len > 64) {
+    const int S = (len + 63) >> 6;
+    const int ci = lo + min(len - 1, l * S + S - 1);
+    const int f = first_set(ballot((l * S < len) & pred(ci)));
+    if (f == 64) return n;
+    lo += f * S;
+    len = min(S, len - f * S);
+  }
+  const int g = first_set(ballot((l < len) & pred(lo + min(l, len - 1))));
+  return g == 64 ? n : lo + g;
+}
+
+// Event wave: waits for the tail repair posted as command k; its total and
+// the end of the range it wrote into pnew.
+__device__ __forceinline__ bool tail_collect(Mailbox& mb, int k, double& tot, int& hi) {
+  if (!mb_wait(mb, k)) return false;
+  const int4 r0 = mb.q[k & (MB_RING - 1)].r0;
+  tot = uni_d(unpack_d(r0.x, r0.y));
+  hi = uni_i(r0.z);
+  return true;
+}
+
 // The event loop over a sector state S of capacity `cap` (LDS or device
 // memory); T and evo[8] are LDS.
-// HELPER: a helper wavefront runs helper_loop on the same LDS state and mb
-// is its mailbox (the LDS kernel); otherwise mb is unused.
+// HELPER: a helper wavefront runs helper_loop on the same LDS state, mb is
+// its mailbox and pnew its tail-repair array (the LDS kernel); otherwise both
+// are unused.
 template <class SecT, bool HELPER>
 __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap, SharedTables& T, double* evo,
-                                           Mailbox* mb) {
+                                           Mailbox* mb, double* pnew) {
   const int l = lane_id();
   stage_tables(T);
   LaneStencil LS = lane_stencil(T.st);
@@ -327,6 +421,14 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
   int32_t status = AKMC_RUN_OK;
   int64_t visits = 0;  // sector visits this launch (bounded so a launch always returns)
   int mb_seq = 0;      // commands posted to the helper wave
+  // A running-sum tail handed to the helper wave and not yet collected:
+  // its command index (-1: none), the total before the event and its
+  // estimate after it, the shift d_est = fl(Pn - Po) of the old tail sums,
+  // and the exact prefix (positions < pre_n, sum pre_sum).
+  int tail_k = -1;
+  double t_old = 0.0, t_apx = 0.0, d_est = 0.0, pre_sum = 0.0;
+  int pre_n = 0;
+  int copy_k = -1;  // a COPY posted and not yet waited for
   // Random numbers of the next event, loaded one event ahead.
   double u_next = 0.0, l_next = 0.0;
   if (a.rates_sector < 0 && a.u_stream && a.l_stream && st.events >= a.rng_base && st.events < a.rng_end) {
@@ -375,7 +477,7 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
       eval_list(S, T, LS, L, nullptr, 0, nv, nopatch, false, 0, -1);
       st.n_eval += nv;
       st.n_fold += nv;
-      sum = fold_range(S, 0, nv, 0.0);
+      sum = fold_range(S, 0, nv, 0.0, S.pvac);
       if (a.check_sums && !sums_agree(S, nv, sum)) st.sum_errors++;
       rebuild = false;
       if (a.rates_sector >= 0) {
@@ -388,32 +490,89 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
         break;
       }
     }
+    // A tail repair still with the helper: its total exceeds the exact
+    // prefix sum, so the zero test below is decided unless that is tiny.
+    if (HELPER && tail_k >= 0 && !(pre_sum >= 1e-10)) {
+      int hi = 0;
+      if (!tail_collect(*mb, tail_k, sum, hi)) { status = AKMC_RUN_ERR_HELPER; break; }
+      copy_sums(S, pnew, pre_n, hi);
+      tail_k = -1;
+    }
     // sum_rates == 0 or below epsilon: jump to the threshold (sublattice.inl:108-112).
-    if (sum == 0.0 || fabs(sum) < 2.220446049250313e-16) {
+    if (tail_k < 0 && (sum == 0.0 || fabs(sum) < 2.220446049250313e-16)) {
       st.t = st.thr;
       continue;
     }
 
     // ---- select: first (vacancy, dir) whose running sum exceeds u * sum
     // (ModelAdapter::selectAndPerform, model_adapter.h:95-101; ABVIModel::select, kmc.cpp:242-309).
-    const double target = uni_d(u_next) * sum;
     const double lg = uni_d(l_next);
-    const int vsel = first_above(S.pvac, nv, target);
-    if (vsel >= nv) { status = AKMC_RUN_ERR_NO_EVENT; break; }
-    int bsel = -1;
-    {
-      double r8[8];
+    int vsel = -1, bsel = -1;
+    if (HELPER && tail_k >= 0) {
+      // While the helper repairs the tail, every running sum is known to
+      // within eps of its exact value: exact below pre_n, and from there on
+      // old + d_est, off by at most one ulp of the running sum (<= ulp(tm))
+      // per add of the tail (old and new folds round differently) plus the
+      // rounding of d_est and of the sum; the slot's own adds and the two
+      // products u * total add a few more.  If u * estimate lies farther
+      // than eps from the running sums on either side of the slot it
+      // selects, the exact target selects the same slot.
+      const double u = uni_d(u_next);
+      const double tm = 2.0 * fmax(t_old, t_apx);
+      if (tm > 0x1p-969) {
+        const double ulp_t = __builtin_ldexp(1.0, exponent_of(tm) - 52);
+        const double eps = (16.0 * (double)(nv - pre_n) + 32.0) * ulp_t;
+        const double tt = u * t_apx;
+        const int v = first_above_est(S.pvac, nv, pre_n, d_est, tt);
+        if (v < nv) {
+          double r8[8];
 #pragma unroll
-      for (int b = 0; b < 8; b++) r8[b] = S.rate[vsel * 8 + b];
-      double p = vsel == 0 ? 0.0 : S.pvac[vsel - 1];
+          for (int b = 0; b < 8; b++) r8[b] = S.rate[v * 8 + b];
+          const double pv = v == 0 ? 0.0 : S.pvac[v - 1];
+          const double pb = v == 0 ? 0.0 : (v - 1 < pre_n ? pv : pv + d_est);
+          double p = pb, prev = pb;
+          int b_apx = -1;
+          bool ok = true;
 #pragma unroll
-      for (int b = 0; b < 8; b++) {
-        p += r8[b];
-        if (bsel < 0 && p > target) bsel = b;
+          for (int b = 0; b < 8; b++) {
+            p += r8[b];
+            if (b_apx < 0 && p > tt) {
+              b_apx = b;
+              ok = (p > tt + eps) & (prev <= tt - eps);
+            }
+            prev = p;
+          }
+          if (uni_i(ok && b_apx >= 0 ? 1 : 0)) {
+            vsel = uni_i(v);
+            bsel = uni_i(b_apx);
+          }
+        }
+      }
+      if (vsel < 0) {
+        int hi = 0;
+        if (!tail_collect(*mb, tail_k, sum, hi)) { status = AKMC_RUN_ERR_HELPER; break; }
+        copy_sums(S, pnew, pre_n, hi);
+        tail_k = -1;
       }
     }
-    bsel = uni_i(bsel);
-    if (bsel < 0) { status = AKMC_RUN_ERR_NO_EVENT; break; }
+    if (vsel < 0) {
+      const double target = uni_d(u_next) * sum;
+      vsel = first_above(S.pvac, nv, target);
+      if (vsel >= nv) { status = AKMC_RUN_ERR_NO_EVENT; break; }
+      {
+        double r8[8];
+#pragma unroll
+        for (int b = 0; b < 8; b++) r8[b] = S.rate[vsel * 8 + b];
+        double p = vsel == 0 ? 0.0 : S.pvac[vsel - 1];
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+          p += r8[b];
+          if (bsel < 0 && p > target) bsel = b;
+        }
+      }
+      bsel = uni_i(bsel);
+      if (bsel < 0) { status = AKMC_RUN_ERR_NO_EVENT; break; }
+    }
 
     // ---- perform (ABVIModel::perform, kmc.cpp:358-396): the vacancy moves to
     // the target; the target's first species tag of Cu, Mn, Ni moves back.
@@ -441,16 +600,27 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
       const int64_t li = st.events - a.log_base;
       if (a.log && li >= 0 && li < a.log_cap) { a.log[2 * li] = from; a.log[2 * li + 1] = to; }
     }
-    // ---- residence time (sublattice.inl:118-120): dt = -log(r) / sum.
-    st.t += lg / sum;
     st.events++;
     if (st.events < a.rng_end) {
       u_next = a.u_stream[st.events - a.rng_base];
       l_next = a.l_stream[st.events - a.rng_base];
     }
-    if (!(st.t < st.thr) || st.events >= a.max_events || st.events >= a.rng_end) {
-      rebuild = true;
-      continue;
+    const bool last = st.events >= a.max_events || st.events >= a.rng_end;
+    // ---- residence time (sublattice.inl:118-120): dt = -log(r) / sum, with
+    // the exact total.  While the helper still repairs the tail, the clock
+    // advances once the helper has also done the list edit posted below.
+    if (!HELPER || tail_k < 0 || last) {
+      if (HELPER && tail_k >= 0) {
+        int hi = 0;
+        if (!tail_collect(*mb, tail_k, sum, hi)) { status = AKMC_RUN_ERR_HELPER; break; }
+        copy_sums(S, pnew, pre_n, hi);
+        tail_k = -1;
+      }
+      st.t += lg / sum;
+      if (!(st.t < st.thr) || last) {
+        rebuild = true;
+        continue;
+      }
     }
 
     // ---- the next event's calcRates, incrementally.
@@ -464,8 +634,7 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
       // The helper wave edits the list and finds the candidates while this
       // wave evaluates the hopped vacancy's rates (they do not depend on its
       // new position).
-      // Every earlier command is done (this wave waited for it), so the slot
-      // is free.
+      // At most the tail repair is still running, so the slot is free.
       MbCmd* c = &mb->q[mb_seq & (MB_RING - 1)];
       if (l == 0) {
         c->a0 = make_int4(CMD_MOVE, vsel, nv, stays ? 1 : 0);
@@ -473,11 +642,32 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
         c->a2 = make_int4(zhi, 0, 0, 0);
       }
       const int km = mb_commit(*mb, mb_seq);
+      if (tail_k >= 0) {
+        MbCmd* cc = &mb->q[mb_seq & (MB_RING - 1)];
+        if (l == 0) cc->a0 = make_int4(CMD_COPY, 0, 0, 0);
+        copy_k = mb_commit(*mb, mb_seq);
+      }
       if (stays) {
         const Patch pt0{from, to, nf_from, nf_to, -1, key_index(L.d, from), key_index(L.d, to)};
         hop = eval_hopped_rates(S, T, LS, L, to, pt0, pf, pq);
       }
       if (!mb_wait(*mb, km)) { status = AKMC_RUN_ERR_HELPER; break; }
+      if (tail_k >= 0) {
+        // The helper runs commands in order: the tail's total is ready, and
+        // its new sums are in pvac.
+        const int4 t0 = mb->q[tail_k & (MB_RING - 1)].r0;
+        sum = uni_d(unpack_d(t0.x, t0.y));
+        tail_k = -1;
+        st.t += lg / sum;
+        if (!(st.t < st.thr)) {
+          // The sector ends: its list (edited ahead) is collected afresh
+          // at its next visit, once the helper has stopped writing pvac.
+          if (copy_k >= 0 && !mb_wait(*mb, copy_k)) { status = AKMC_RUN_ERR_HELPER; break; }
+          copy_k = -1;
+          rebuild = true;
+          continue;
+        }
+      }
       const MbCmd& r = mb->q[km & (MB_RING - 1)];
       const int4 r0 = r.r0;
       const int4 r1 = r.r1;
@@ -526,21 +716,42 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
       }
     }
     // ---- running sums.
+    if (HELPER && copy_k >= 0) {
+      if (!mb_wait(*mb, copy_k)) { status = AKMC_RUN_ERR_HELPER; break; }
+      copy_k = -1;
+    }
     if (nv == 0) {
       sum = 0.0;
     } else if (!stays) {
       const int c_lo = min(min(vsel, amin), nv);
       st.n_fold += nv - c_lo;
-      sum = c_lo < nv ? fold_range(S, c_lo, nv, c_lo > 0 ? S.pvac[c_lo - 1] : 0.0)
+      sum = c_lo < nv ? fold_range(S, c_lo, nv, c_lo > 0 ? S.pvac[c_lo - 1] : 0.0, S.pvac)
                       : S.pvac[nv - 1];
     } else {
       const int c_lo = min(min(vsel, knew), amin);
       const int c_hi = max(max(vsel, knew), amax);
       const double Po = S.pvac[c_hi];
-      const double Pn = fold_range(S, c_lo, c_hi + 1, c_lo > 0 ? S.pvac[c_lo - 1] : 0.0);
+      const double Pn = fold_range(S, c_lo, c_hi + 1, c_lo > 0 ? S.pvac[c_lo - 1] : 0.0, S.pvac);
       st.n_fold += c_hi + 1 - c_lo;
       st.n_shift += nv - (c_hi + 1);
-      sum = c_hi + 1 < nv ? tail_update(S, c_hi + 1, nv, Pn, Po) : Pn;
+      if (HELPER && !a.check_sums && c_hi + 1 < nv) {
+        // The helper repairs the tail while the next event is selected
+        // (from the exact prefix when it can be) and performed.
+        MbCmd* c = &mb->q[mb_seq & (MB_RING - 1)];
+        if (l == 0) {
+          c->a0 = make_int4(CMD_TAIL, c_hi + 1, nv, 0);
+          c->a1 = pack_dd(Pn, Po);
+        }
+        tail_k = mb_commit(*mb, mb_seq);
+        t_old = sum;
+        d_est = uni_d(Pn - Po);
+        t_apx = uni_d(sum + d_est);
+        pre_n = c_hi + 1;
+        pre_sum = Pn;
+      } else {
+        int hi = 0;
+        sum = c_hi + 1 < nv ? tail_update(S, c_hi + 1, nv, Pn, Po, S.pvac, hi) : Pn;
+      }
     }
     if (a.check_sums && !sums_agree(S, nv, sum)) st.sum_errors++;
   }
@@ -565,15 +776,16 @@ __global__ __launch_bounds__(128) void plain_kernel(PlainArgs a) {
   __shared__ SharedTables T;
   __shared__ double evo[8];  // sector clocks (indexed by the running sector id)
   __shared__ Mailbox mb;
+  __shared__ double pnew[PLAIN_VCAP];
   if (threadIdx.x == 0) {
     mb.posted = 0;
     mb.done = 0;
   }
   __syncthreads();
   if (threadIdx.x < 64)
-    plain_body<PlainSector, true>(a, S, PLAIN_VCAP, T, evo, &mb);
+    plain_body<PlainSector, true>(a, S, PLAIN_VCAP, T, evo, &mb, pnew);
   else
-    helper_loop(S, T, a.dims, mb);
+    helper_loop(S, T, a.dims, mb, pnew);
 }
 
 // The same loop with the sector arrays in device memory, for sectors of more
@@ -590,7 +802,7 @@ __global__ __launch_bounds__(64) void plain_kernel_hbm(PlainArgs a) {
   SectorRef S{reinterpret_cast<uint32_t*>(w + off[0]), reinterpret_cast<int32_t*>(w + off[1]),
               reinterpret_cast<double*>(w + off[2]),   reinterpret_cast<double*>(w + off[3]),
               reinterpret_cast<uint64_t*>(w + off[4]), w + off[5], w + off[6], w + off[7], sten, cflag, list};
-  plain_body<SectorRef, false>(a, S, a.sector_cap, T, evo, nullptr);
+  plain_body<SectorRef, false>(a, S, a.sector_cap, T, evo, nullptr, nullptr);
 }
 
 }  // namespace akmc

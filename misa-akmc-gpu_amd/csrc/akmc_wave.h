@@ -263,9 +263,9 @@ __device__ __forceinline__ int first_above(const double* pvac, int n, double tar
 // Serial fold of the cnt <= 8 vacancies v .. v + cnt - 1 from P, in the
 // reference order.  Every lane reads the same rates (broadcast LDS reads, all
 // issued before the adds) and folds them in registers; lane j writes vacancy
-// v + j's running sum and tie risk.  Returns the sum after the last.
+// v + j's running sum (into out) and tie risk.  Returns the sum after the last.
 template <class SecT>
-__device__ __forceinline__ double fold_serial(SecT& S, int v, int cnt, double P) {
+__device__ __forceinline__ double fold_serial(SecT& S, int v, int cnt, double P, double* out) {
   const int l = lane_id();
   double pw = 0.0, pw0 = 0.0;
 #pragma unroll
@@ -281,16 +281,17 @@ __device__ __forceinline__ double fold_serial(SecT& S, int v, int cnt, double P)
     }
   }
   if (l < cnt) {
-    S.pvac[v + l] = pw;
+    out[v + l] = pw;
     S.risk[v + l] = fold_risk(pw0, pw, S.tmask[v + l]);
   }
   return uni_d(P);
 }
 
 // Exact running sums for vacancies [v0, v1) starting from P (the sum before
-// v0).  Writes pvac and returns the sum after v1 - 1.
+// v0).  Writes them into out (pvac unless a caller keeps the old sums) with
+// their tie risks, and returns the sum after v1 - 1.
 template <class SecT>
-__device__ __forceinline__ double fold_range(SecT& S, int v0, int v1, double P) {
+__device__ __forceinline__ double fold_range(SecT& S, int v0, int v1, double P, double* out) {
   const int l = lane_id();
   int v = uni_i(v0);
   v1 = uni_i(v1);
@@ -298,7 +299,7 @@ __device__ __forceinline__ double fold_range(SecT& S, int v0, int v1, double P) 
   while (v < v1) {
     if (v1 - v <= 6) {
       // A few vacancies: serially (cheaper than a scan pass).
-      P = fold_serial(S, v, v1 - v, P);
+      P = fold_serial(S, v, v1 - v, P, out);
       break;
     }
     if (!(P >= 0x1p-969)) {
@@ -312,7 +313,7 @@ __device__ __forceinline__ double fold_range(SecT& S, int v0, int v1, double P) 
       for (int b = 0; b < 8; b++) p += a[b];
       p = uni_d(p);
       if (l == 0) {
-        S.pvac[v] = p;
+        out[v] = p;
         S.risk[v] = fold_risk(P, p, tm);
       }
       P = p;
@@ -352,7 +353,7 @@ __device__ __forceinline__ double fold_range(SecT& S, int v0, int v1, double P) 
     const int fb = first_set(ballot(active & bad));
     if (active & (l < fb)) {
       // All of this vacancy's sums lie in P's binade e: only ties matter.
-      S.pvac[vi] = P + incl * u;
+      out[vi] = P + incl * u;
       S.risk[vi] = tie_free(S.tmask[vi], e) ? (uint8_t)0 : (uint8_t)RISK_TIE;
     }
     if (fb == 64) {
@@ -375,7 +376,7 @@ __device__ __forceinline__ double fold_range(SecT& S, int v0, int v1, double P) 
       for (int b = 0; b < 8; b++) p += ab[b];
       p = uni_d(p);
       if (l == 0) {
-        S.pvac[vb] = p;
+        out[vb] = p;
         S.risk[vb] = fold_risk(p0, p, tm);
       }
       P = p;
@@ -387,8 +388,9 @@ __device__ __forceinline__ double fold_range(SecT& S, int v0, int v1, double P) 
 }
 
 // Shifts the running sums of vacancies [v0, n) whose rates did not change
-// after the sum before them moved from Po to Pn.  pvac[v] still holds old
-// values on entry for v >= v0.  Returns the new total (pvac[n - 1]).
+// after the sum before them moved from Po to Pn.  pvac[v] holds the old sums
+// for v >= v0 on entry; the new ones go to out[v0, hi), and from hi on they
+// equal the old ones (out may be pvac itself).  Returns the new total.
 //
 // Region by region: with the old sum before v in binade e (S_old) and the
 // new one (P_new) in the same binade, D = P_new - S_old is a multiple of
@@ -401,22 +403,27 @@ __device__ __forceinline__ double fold_range(SecT& S, int v0, int v1, double P) 
 // round trip, until old and new sums share a binade again, and the next
 // region starts with the new exact difference.
 template <class SecT>
-__device__ __forceinline__ double tail_update(SecT& S, int v0, int n, double Pn, double Po) {
+__device__ __forceinline__ double tail_update(SecT& S, int v0, int n, double Pn, double Po, double* out, int& hi) {
   const int l = lane_id();
   int v = uni_i(v0);
   n = uni_i(n);
   double Sold = uni_d(Po);   // old running sum before v
   double Pprev = uni_d(Pn);  // new running sum before v
   bool capped = false;
+  hi = v;
   while (v < n) {
     const double D = Pprev - Sold;
     if (D == 0.0) {
       wave_sync();
-      return S.pvac[n - 1];  // the rest is unchanged
+      hi = v;
+      return uni_d(S.pvac[n - 1]);  // the rest is unchanged
     }
     const int e = exponent_of(Sold);
     const bool start_ok = (Sold >= 0x1p-969) & (exponent_of(Pprev) == e) & (Sold + D == Pprev) & (Pprev - D == Sold);
-    if (!start_ok && capped) return fold_range(S, v, n, Pprev);
+    if (!start_ok && capped) {
+      hi = n;
+      return fold_range(S, v, n, Pprev, out);
+    }
     int stop = v;
     if (start_ok) {
       // D < 2^e here, so top - D is exact, and D / u is an integer below 2^52.
@@ -445,21 +452,24 @@ __device__ __forceinline__ double tail_update(SecT& S, int v0, int n, double Pn,
         const int f = fl == 64 ? n : c + 4 * fl + __builtin_amdgcn_readlane(kf, fl);
 #pragma unroll
         for (int k = 0; k < 4; k++)
-          if (i0 + k < f) S.pvac[i0 + k] = x[k] + D;
+          if (i0 + k < f) out[i0 + k] = x[k] + D;
         if (f < n) {
           stop = f;
           break;
         }
       }
       wave_sync();
-      if (stop >= n) return uni_d(S.pvac[n - 1]);
+      if (stop >= n) {
+        hi = n;
+        return uni_d(out[n - 1]);
+      }
     }
     // Serial exact fold from `stop` until a folded vacancy's old and new sums
     // share a binade again (normally one to three vacancies next to a power
     // of two, or the risky vacancy alone), at most 64.  One vacancy per step:
     // every lane reads its eight rates, old sum and tie mask (broadcast LDS
     // reads) and folds in registers; lane 0 writes its new sum and risk.
-    double p = uni_d(stop == v ? Pprev : S.pvac[stop - 1]);
+    double p = uni_d(stop == v ? Pprev : out[stop - 1]);
     double old_w = 0.0;
     int w = stop;
     bool same = false;
@@ -475,14 +485,17 @@ __device__ __forceinline__ double tail_update(SecT& S, int v0, int n, double Pn,
       old_w = ow;
       same = uni_i((exponent_of(old_w) == exponent_of(p)) & (old_w >= 0x1p-969)) != 0;
       if (l == 0) {
-        S.pvac[w] = p;
+        out[w] = p;
         S.risk[w] = fold_risk(p0, p, tw);
       }
       w++;
       if (w >= n || same || w - stop >= 64) break;
     }
     wave_sync();
-    if (w >= n) return p;
+    if (w >= n) {
+      hi = n;
+      return p;
+    }
     capped = !same;
     Sold = old_w;
     Pprev = p;

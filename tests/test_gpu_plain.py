@@ -266,8 +266,9 @@ def test_time_loop_end_and_empty_box():
 
 
 def test_sectors_beyond_lds_capacity():
-    """Sectors of more than the LDS form's 1,536 vacancies run with the sector
-    state in device memory: rates and trajectory bit-exact vs the oracle."""
+    """Sectors of more vacancies than the LDS form holds (akmc_plain_vcap)
+    run with the sector state in device memory: rates and trajectory
+    bit-exact vs the oracle."""
     say("test_sectors_beyond_lds_capacity")
     cfg = make_cfg(n=32, va=16000, T=600.0)  # ~2,000 vacancies per sector
     eng = akmc.Engine(cfg)
@@ -275,7 +276,7 @@ def test_sectors_beyond_lds_capacity():
     for s in (0, 5):
         gs, gr, gt = eng.sector_rates(s, cap=4096)
         rs, rr, rt = ref.sector_rates(s, cap=4096)
-        assert len(gs) > 1536
+        assert len(gs) > akmc.lib().akmc_plain_vcap()
         assert np.array_equal(gs, rs) and np.array_equal(_bits(gr), _bits(rr)) and gt == rt
     eng.close()
     st = compare_run(cfg, 400)
