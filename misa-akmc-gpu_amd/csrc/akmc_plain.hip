@@ -186,40 +186,78 @@ __device__ __forceinline__ bool only_hopped(const SecT& S, const BoxDims& d, con
   return only;
 }
 
-// ---------------------------------------------------------------- helper wave
-// The LDS kernel runs a second wavefront beside the event loop.  After a hop
-// the event wave hands it the list edit and the candidate search (rank of the
-// new key, entries moved, candidate range, whether any vacancy besides the
-// hopped one needs re-evaluation) and meanwhile evaluates the hopped
-// vacancy's rates into registers; it then waits for the answer and stores
-// the rates at the new position.  After the block fold it hands over the
-// tail shift (TAIL) and goes on to the next event: it selects from the
-// exact prefix and the old tail sums plus the block's change, each within a
-// bound it checks, and collects the exact total before the clock advances.
-// Commands go through a ring in LDS whose counters lane 0 writes after an
-// LDS release fence and every lane reads before an LDS acquire fence (the
-// handed-over state is all in LDS; a fence over global memory would also
-// wait for the hop's stores to the lattice).  The helper runs commands in
-// order.  Every wait is bounded: one that runs out ends the launch with
-// AKMC_RUN_ERR_HELPER instead of hanging.  Handing over only the tail's
-// region shifts, with the event wave waiting for them, was measured and not
-// kept (posting and waiting cost more than the stores).
-enum : int32_t { CMD_MOVE = 1, CMD_TAIL = 2, CMD_EXIT = 3, CMD_COPY = 4 };
+// The read-only half of the list edit after a hop of entry vsel from key
+// `from` to key `to`, done on the list as it is before the edit (nv
+// entries): the new position knew of the hopped vacancy (-1 if it left the
+// sector) and the candidate range [ilo, ihi) and only_hopped flag as they
+// are after it.  Positions before min(vsel, knew) and after max(vsel, knew)
+// keep their entries; the hopped key lies inside the candidates' z-planes,
+// so a range that covers them before the edit covers them after it, one
+// shorter when the entry was deleted.
+template <class SecT>
+__device__ __forceinline__ void analyze_hop(const SecT& S, const BoxDims& d, const StencilTables& st, int nv,
+                                            int vsel, uint32_t from, uint32_t to, bool stays, int zlo, int zhi,
+                                            int& knew, int& ilo, int& ihi, bool& only) {
+  const int l = lane_id();
+  knew = -1;
+  if (stays) {
+    const int w0 = max(0, min(vsel - 32, nv - 64));
+    const int wv = w0 + l;
+    const uint32_t kw = S.key[min(wv, nv - 1)];
+    const uint64_t below = ballot((wv < nv) & (wv != vsel) & (kw < to));
+    // Every key before the window is below `to` and every key after it above.
+    const bool inside = (w0 == 0 || S.key[w0 - 1] < to) && (w0 + 64 >= nv || S.key[w0 + 64] > to);
+    if (inside) {
+      knew = w0 + __popcll(below) - (vsel < w0 ? 1 : 0);
+    } else {
+      const int lb = lower_bound_key(S.key, nv, to);
+      knew = lb > vsel ? lb - 1 : lb;
+    }
+  }
+  cand_range(S, nv, vsel, zlo, zhi, d.Z, ilo, ihi);
+  only = only_hopped(S, d, st, ilo, ihi, vsel, from, to);
+  if (!stays) ihi -= 1;
+}
+
+// ---------------------------------------------------------------- helper waves
+// The LDS kernel runs two helper wavefronts beside the event loop, each with
+// its own command ring in LDS.
+//  * The sum helper repairs the running-sum tail.  After the block fold the
+//    event wave hands it the tail shift (TAIL) and goes on to the next event:
+//    it selects from the exact prefix and the old tail sums plus the block's
+//    change, each within a bound it checks, and collects the exact total
+//    before the clock advances.
+//  * The list helper does the read-only half of the list edit (analyze_hop:
+//    rank of the new key, candidate range, whether any vacancy besides the
+//    hopped one needs re-evaluation) while the event wave evaluates the
+//    hopped vacancy's rates into registers and the tail repair may still be
+//    running.  The event wave then moves the entries itself, once the tail
+//    repair (which reads the rates and tie masks that move) is done.
+// Counters are written by lane 0 after an LDS release fence and read by
+// every lane before an LDS acquire fence (the handed-over state is all in
+// LDS; a fence over global memory would also wait for the hop's stores to
+// the lattice).  A helper runs its commands in order.  Every wait is
+// bounded: one that runs out ends the launch with AKMC_RUN_ERR_HELPER
+// instead of hanging.  Handing over only the tail's region shifts, with the
+// event wave waiting for them, was measured and not kept (posting and
+// waiting cost more than the stores).
+enum : int32_t { CMD_ANALYZE = 1, CMD_TAIL = 2, CMD_EXIT = 3, CMD_COPY = 4 };
 constexpr int MB_RING = 16;
 constexpr int64_t MB_SPIN_LIMIT = (int64_t)1 << 26;  // polls of ~100 cycles: seconds
 // A command: arguments in three 16-byte words, results in two, so that each
 // side writes and reads them with a few wide LDS accesses.
-//   MOVE  a0 = op, vsel, nv, stays; a1 = from, to, vslot, zlo; a2 = zhi
-//         r0 = knew, nv_out, ilo, ihi; r1 = only
-//   TAIL  a0 = op, v0, n; a1 = Pn, Po (the bits of two doubles)
-//         r0 = total (bits), hi
-//   COPY  a0 = op
+//   ANALYZE  a0 = op, vsel, nv, stays; a1 = from, to, zlo, zhi
+//            r0 = knew, ilo, ihi, only
+//   TAIL     a0 = op, v0, n; a1 = Pn, Po (the bits of two doubles)
+//            r0 = total (bits), hi
+//   COPY     a0 = op, vsel, stays
 // A tail repair writes the new sums into a separate array (pnew) and leaves
 // the old ones in pvac, where the event wave still reads them while it
-// selects the next event.  The COPY posted after that event's MOVE moves
-// pnew[v0, hi) into pvac: after the list edit, because the positions the
-// edit shifts are refolded anyway (a vacancy that left the sector shifts
-// every position from vsel on, which the fold then covers).
+// selects the next event.  The COPY posted once that selection is done
+// moves pnew[v0, hi) into pvac; the list edit moves no running sums, and the
+// positions it shifts are refolded anyway (a vacancy that left the sector
+// shifts every position from vsel on, which the fold then covers, so those
+// are not copied).
 struct __align__(16) MbCmd {
   int4 a0;
   int4 a1;
@@ -299,8 +337,6 @@ template <class SecT>
 __device__ void helper_loop(SecT& S, const SharedTables& T, const BoxDims d, Mailbox& mb, double* pnew) {
   const int l = lane_id();
   int pend_lo = 0, pend_hi = 0;  // new sums of the last tail repair, not yet in pvac
-  int last_vsel = 0;             // the last MOVE's vacancy position, and
-  bool last_stays = true;        // whether it stayed in the sector
   for (int k = 0;; k++) {
     int64_t spins = 0;
     bool got = true;
@@ -312,24 +348,15 @@ __device__ void helper_loop(SecT& S, const SharedTables& T, const BoxDims d, Mai
     }
     if (!got) return;
     MbCmd& c = mb.q[k & (MB_RING - 1)];
-    const int4 a0 = c.a0, a1 = c.a1, a2 = c.a2;
+    const int4 a0 = c.a0, a1 = c.a1;
     const int op = uni_i(a0.x);
     if (op == CMD_EXIT) return;
-    if (op == CMD_MOVE) {
-      int nv = uni_i(a0.z);
-      const int vsel = uni_i(a0.y);
-      const bool stays = uni_i(a0.w) != 0;
-      last_vsel = vsel;
-      last_stays = stays;
-      const uint32_t from = (uint32_t)uni_i(a1.x), to = (uint32_t)uni_i(a1.y);
-      const int knew = rank_and_move(S, nv, vsel, to, stays, uni_i(a1.z));
-      int ilo, ihi;
-      cand_range(S, nv, stays ? knew : vsel, uni_i(a1.w), uni_i(a2.x), d.Z, ilo, ihi);
-      const bool only = only_hopped(S, d, T.st, ilo, ihi, stays ? knew : -1, from, to);
-      if (l == 0) {
-        c.r0 = make_int4(knew, nv, ilo, ihi);
-        c.r1 = make_int4(only ? 1 : 0, 0, 0, 0);
-      }
+    if (op == CMD_ANALYZE) {
+      int knew, ilo, ihi;
+      bool only;
+      analyze_hop(S, d, T.st, uni_i(a0.z), uni_i(a0.y), (uint32_t)uni_i(a1.x), (uint32_t)uni_i(a1.y),
+                  uni_i(a0.w) != 0, uni_i(a1.z), uni_i(a1.w), knew, ilo, ihi, only);
+      if (l == 0) c.r0 = make_int4(knew, ilo, ihi, only ? 1 : 0);
     } else if (op == CMD_TAIL) {
       // The exact running-sum repair behind the changed block.
       const double Pn = uni_d(unpack_d(a1.x, a1.y)), Po = uni_d(unpack_d(a1.z, a1.w));
@@ -340,7 +367,7 @@ __device__ void helper_loop(SecT& S, const SharedTables& T, const BoxDims d, Mai
       const uint64_t tb = as_u64(tot);
       if (l == 0) c.r0 = make_int4((int)(uint32_t)tb, (int)(uint32_t)(tb >> 32), hi, 0);
     } else if (op == CMD_COPY) {
-      copy_sums(S, pnew, pend_lo, last_stays ? pend_hi : min(pend_hi, last_vsel));
+      copy_sums(S, pnew, pend_lo, uni_i(a0.z) ? pend_hi : min(pend_hi, uni_i(a0.y)));
       pend_lo = pend_hi = 0;
     }
     mb_store(&mb.done, k + 1);
@@ -399,12 +426,12 @@ __device__ __forceinline__ bool tail_collect(Mailbox& mb, int k, double& tot, in
 
 // The event loop over a sector state S of capacity `cap` (LDS or device
 // memory); T and evo[8] are LDS.
-// HELPER: a helper wavefront runs helper_loop on the same LDS state, mb is
-// its mailbox and pnew its tail-repair array (the LDS kernel); otherwise both
-// are unused.
+// HELPER: two helper wavefronts run helper_loop on the same LDS state, mb
+// and mbc are the mailboxes of the sum helper and the list helper and pnew
+// the tail-repair array (the LDS kernel); otherwise all three are unused.
 template <class SecT, bool HELPER>
 __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap, SharedTables& T, double* evo,
-                                           Mailbox* mb, double* pnew) {
+                                           Mailbox* mb, Mailbox* mbc, double* pnew) {
   const int l = lane_id();
   stage_tables(T);
   LaneStencil LS = lane_stencil(T.st);
@@ -420,7 +447,8 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
   int s = ring_sector(st.sect);
   int32_t status = AKMC_RUN_OK;
   int64_t visits = 0;  // sector visits this launch (bounded so a launch always returns)
-  int mb_seq = 0;      // commands posted to the helper wave
+  int mb_seq = 0;      // commands posted to the sum helper
+  int mc_seq = 0;      // commands posted to the list helper
   // A running-sum tail handed to the helper wave and not yet collected:
   // its command index (-1: none), the total before the event and its
   // estimate after it, the shift d_est = fl(Pn - Po) of the old tail sums,
@@ -607,8 +635,9 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
     }
     const bool last = st.events >= a.max_events || st.events >= a.rng_end;
     // ---- residence time (sublattice.inl:118-120): dt = -log(r) / sum, with
-    // the exact total.  While the helper still repairs the tail, the clock
-    // advances once the helper has also done the list edit posted below.
+    // the exact total.  While the sum helper still repairs the tail, the
+    // clock advances after the list analysis posted below, with the
+    // repair's total.
     if (!HELPER || tail_k < 0 || last) {
       if (HELPER && tail_k >= 0) {
         int hi = 0;
@@ -631,51 +660,56 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
     bool only = false;
     HoppedEval hop{};
     if (HELPER) {
-      // The helper wave edits the list and finds the candidates while this
-      // wave evaluates the hopped vacancy's rates (they do not depend on its
-      // new position).
-      // At most the tail repair is still running, so the slot is free.
-      MbCmd* c = &mb->q[mb_seq & (MB_RING - 1)];
+      // The list helper analyses the edit while this wave evaluates the
+      // hopped vacancy's rates (they do not depend on its new position) and
+      // the sum helper may still be repairing the tail.  Every earlier
+      // analysis is done, so the slot is free.
+      MbCmd* c = &mbc->q[mc_seq & (MB_RING - 1)];
       if (l == 0) {
-        c->a0 = make_int4(CMD_MOVE, vsel, nv, stays ? 1 : 0);
-        c->a1 = make_int4((int)from, (int)to, vslot, zlo);
-        c->a2 = make_int4(zhi, 0, 0, 0);
+        c->a0 = make_int4(CMD_ANALYZE, vsel, nv, stays ? 1 : 0);
+        c->a1 = make_int4((int)from, (int)to, zlo, zhi);
       }
-      const int km = mb_commit(*mb, mb_seq);
+      const int ka = mb_commit(*mbc, mc_seq);
       if (tail_k >= 0) {
+        // This event's selection is done with the old tail sums: the sum
+        // helper moves the new ones into pvac after its repair.
         MbCmd* cc = &mb->q[mb_seq & (MB_RING - 1)];
-        if (l == 0) cc->a0 = make_int4(CMD_COPY, 0, 0, 0);
+        if (l == 0) cc->a0 = make_int4(CMD_COPY, vsel, stays ? 1 : 0, 0);
         copy_k = mb_commit(*mb, mb_seq);
       }
       if (stays) {
         const Patch pt0{from, to, nf_from, nf_to, -1, key_index(L.d, from), key_index(L.d, to)};
         hop = eval_hopped_rates(S, T, LS, L, to, pt0, pf, pq);
       }
-      if (!mb_wait(*mb, km)) { status = AKMC_RUN_ERR_HELPER; break; }
+      if (!mb_wait(*mbc, ka)) { status = AKMC_RUN_ERR_HELPER; break; }
+      const int4 r0 = mbc->q[ka & (MB_RING - 1)].r0;
+      knew = uni_i(r0.x);
+      ilo = uni_i(r0.y);
+      ihi = uni_i(r0.z);
+      only = uni_i(r0.w) != 0;
       if (tail_k >= 0) {
-        // The helper runs commands in order: the tail's total is ready, and
-        // its new sums are in pvac.
-        const int4 t0 = mb->q[tail_k & (MB_RING - 1)].r0;
-        sum = uni_d(unpack_d(t0.x, t0.y));
+        int hi = 0;
+        if (!tail_collect(*mb, tail_k, sum, hi)) { status = AKMC_RUN_ERR_HELPER; break; }
         tail_k = -1;
         st.t += lg / sum;
         if (!(st.t < st.thr)) {
-          // The sector ends: its list (edited ahead) is collected afresh
-          // at its next visit, once the helper has stopped writing pvac.
+          // The sector ends: it is collected afresh at its next visit, once
+          // the sum helper has stopped writing pvac.
           if (copy_k >= 0 && !mb_wait(*mb, copy_k)) { status = AKMC_RUN_ERR_HELPER; break; }
           copy_k = -1;
           rebuild = true;
           continue;
         }
       }
-      const MbCmd& r = mb->q[km & (MB_RING - 1)];
-      const int4 r0 = r.r0;
-      const int4 r1 = r.r1;
-      knew = uni_i(r0.x);
-      nv = uni_i(r0.y);
-      ilo = uni_i(r0.z);
-      ihi = uni_i(r0.w);
-      only = uni_i(r1.x) != 0;
+      // The list edit proper, now that the tail repair has stopped reading
+      // the rates and tie masks it moves (running sums do not move).
+      move_entry(S, nv, vsel, stays ? knew : -1);
+      if (stays) {
+        if (l == 0) { S.key[knew] = to; S.slot[knew] = vslot; }
+      } else {
+        nv -= 1;
+      }
+      wave_sync();
     } else {
       knew = rank_and_move(S, nv, vsel, to, stays, vslot);
       cand_range(S, nv, stays ? knew : vsel, zlo, zhi, L.d.Z, ilo, ihi);
@@ -761,6 +795,11 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
       if (l == 0) c->a0 = make_int4(CMD_EXIT, 0, 0, 0);
       mb_commit(*mb, mb_seq);
     }
+    c = mb_reserve(*mbc, mc_seq);
+    if (c) {
+      if (l == 0) c->a0 = make_int4(CMD_EXIT, 0, 0, 0);
+      mb_commit(*mbc, mc_seq);
+    }
   }
   if (l == 0 && a.rates_sector < 0) {
     st.status = status;
@@ -770,22 +809,27 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
   if (l == 0 && a.rates_sector >= 0 && status != AKMC_RUN_OK) *a.out_n = status;
 }
 
-// Two wavefronts: the event loop (wave 0) and its helper (wave 1).
-__global__ __launch_bounds__(128) void plain_kernel(PlainArgs a) {
+// Three wavefronts: the event loop (wave 0), the sum helper (wave 1) and the
+// list helper (wave 2).
+__global__ __launch_bounds__(192) void plain_kernel(PlainArgs a) {
   __shared__ PlainSector S;
   __shared__ SharedTables T;
   __shared__ double evo[8];  // sector clocks (indexed by the running sector id)
-  __shared__ Mailbox mb;
+  __shared__ Mailbox mb, mbc;
   __shared__ double pnew[PLAIN_VCAP];
   if (threadIdx.x == 0) {
     mb.posted = 0;
     mb.done = 0;
+    mbc.posted = 0;
+    mbc.done = 0;
   }
   __syncthreads();
   if (threadIdx.x < 64)
-    plain_body<PlainSector, true>(a, S, PLAIN_VCAP, T, evo, &mb, pnew);
-  else
+    plain_body<PlainSector, true>(a, S, PLAIN_VCAP, T, evo, &mb, &mbc, pnew);
+  else if (threadIdx.x < 128)
     helper_loop(S, T, a.dims, mb, pnew);
+  else
+    helper_loop(S, T, a.dims, mbc, pnew);
 }
 
 // The same loop with the sector arrays in device memory, for sectors of more
@@ -802,7 +846,7 @@ __global__ __launch_bounds__(64) void plain_kernel_hbm(PlainArgs a) {
   SectorRef S{reinterpret_cast<uint32_t*>(w + off[0]), reinterpret_cast<int32_t*>(w + off[1]),
               reinterpret_cast<double*>(w + off[2]),   reinterpret_cast<double*>(w + off[3]),
               reinterpret_cast<uint64_t*>(w + off[4]), w + off[5], w + off[6], w + off[7], sten, cflag, list};
-  plain_body<SectorRef, false>(a, S, a.sector_cap, T, evo, nullptr, nullptr);
+  plain_body<SectorRef, false>(a, S, a.sector_cap, T, evo, nullptr, nullptr, nullptr);
 }
 
 }  // namespace akmc
@@ -811,7 +855,7 @@ extern "C" int akmc_plain_launch(const akmc::PlainArgs* args, hipStream_t stream
   if (args->sector_ws)
     hipLaunchKernelGGL(akmc::plain_kernel_hbm, dim3(1), dim3(64), 0, stream, *args);
   else
-    hipLaunchKernelGGL(akmc::plain_kernel, dim3(1), dim3(128), 0, stream, *args);
+    hipLaunchKernelGGL(akmc::plain_kernel, dim3(1), dim3(192), 0, stream, *args);
   return (int)hipGetLastError();
 }
 

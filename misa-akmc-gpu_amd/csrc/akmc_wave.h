@@ -464,16 +464,20 @@ __device__ __forceinline__ double tail_update(SecT& S, int v0, int n, double Pn,
         return uni_d(out[n - 1]);
       }
     }
-    // Serial exact fold from `stop` until a folded vacancy's old and new sums
-    // share a binade again (normally one to three vacancies next to a power
-    // of two, or the risky vacancy alone), at most 64.  One vacancy per step:
-    // every lane reads its eight rates, old sum and tie mask (broadcast LDS
-    // reads) and folds in registers; lane 0 writes its new sum and risk.
+    // Exact fold from `stop` until a folded vacancy's old and new sums share
+    // a binade again (normally one to three vacancies next to a power of
+    // two, or the risky vacancy alone; a large change of the block moves
+    // more), at least 64 unless that is reached.  The first vacancy goes
+    // serially: every lane reads its eight rates, old sum and tie mask
+    // (broadcast LDS reads) and folds in registers; lane 0 writes its new sum
+    // and risk.  If the zone goes on, its end is estimated as the first
+    // vacancy whose old sum and old sum + D share a binade, and everything up
+    // to it is folded in one fold_range; the check after it decides.
     double p = uni_d(stop == v ? Pprev : out[stop - 1]);
     double old_w = 0.0;
     int w = stop;
     bool same = false;
-    for (;;) {
+    {
       double r[8];
 #pragma unroll
       for (int b = 0; b < 8; b++) r[b] = (double)S.rate[8 * w + b];
@@ -489,7 +493,19 @@ __device__ __forceinline__ double tail_update(SecT& S, int v0, int n, double Pn,
         S.risk[w] = fold_risk(p0, p, tw);
       }
       w++;
-      if (w >= n || same || w - stop >= 64) break;
+    }
+    const double Dz = Pprev - Sold;
+    while (!same && w < n && w - stop < 64) {
+      const int i = w + l;
+      const int ic = min(i, n - 1);
+      const double xo = S.pvac[ic];  // old sums (fold_range below writes [w, w + cnt) only)
+      const bool es = (xo >= 0x1p-969) & (exponent_of(xo) == exponent_of(xo + Dz));
+      const int j = first_set(ballot((i < n) & (es | (i == n - 1))));
+      const int cnt = j == 64 ? 64 : j + 1;
+      old_w = bcast_d(xo, cnt - 1);
+      p = fold_range(S, w, w + cnt, p, out);
+      w += cnt;
+      same = (exponent_of(old_w) == exponent_of(p)) & (old_w >= 0x1p-969);
     }
     wave_sync();
     if (w >= n) {
