@@ -227,12 +227,12 @@ __device__ __forceinline__ void analyze_hop(const SecT& S, const BoxDims& d, con
 //    it selects from the exact prefix and the old tail sums plus the block's
 //    change, each within a bound it checks, and collects the exact total
 //    before the clock advances.
-//  * The list helper does the read-only half of the list edit (analyze_hop:
-//    rank of the new key, candidate range, whether any vacancy besides the
-//    hopped one needs re-evaluation) while the event wave evaluates the
-//    hopped vacancy's rates into registers and the tail repair may still be
-//    running.  The event wave then moves the entries itself, once the tail
-//    repair (which reads the rates and tie masks that move) is done.
+//  * The list helper does the list edit while the event wave evaluates the
+//    hopped vacancy's rates into registers: first its read-only half
+//    (analyze_hop: rank of the new key, candidate range, whether any vacancy
+//    besides the hopped one needs re-evaluation), which may overlap the tail
+//    repair, then, once the repair (which reads the rates and tie masks that
+//    move) is done, the move of the entries.
 // Counters are written by lane 0 after an LDS release fence and read by
 // every lane before an LDS acquire fence (the handed-over state is all in
 // LDS; a fence over global memory would also wait for the hop's stores to
@@ -246,8 +246,9 @@ constexpr int MB_RING = 16;
 constexpr int64_t MB_SPIN_LIMIT = (int64_t)1 << 26;  // polls of ~100 cycles: seconds
 // A command: arguments in three 16-byte words, results in two, so that each
 // side writes and reads them with a few wide LDS accesses.
-//   ANALYZE  a0 = op, vsel, nv, stays; a1 = from, to, zlo, zhi
-//            r0 = knew, ilo, ihi, only
+//   ANALYZE  a0 = op, vsel, nv, stays; a1 = from, to, zlo, zhi;
+//            a2 = tail repair to wait for before the move (-1: none), vslot
+//            r0 = knew, ilo, ihi, only; r1 = status (nonzero: that wait ran out)
 //   TAIL     a0 = op, v0, n; a1 = Pn, Po (the bits of two doubles)
 //            r0 = total (bits), hi
 //   COPY     a0 = op, vsel, stays
@@ -333,8 +334,10 @@ __device__ __forceinline__ void copy_sums(SecT& S, const double* pnew, int lo, i
   wave_sync();
 }
 
+// tmb: the sum helper's mailbox (for the list helper's wait on a tail repair).
 template <class SecT>
-__device__ void helper_loop(SecT& S, const SharedTables& T, const BoxDims d, Mailbox& mb, double* pnew) {
+__device__ void helper_loop(SecT& S, const SharedTables& T, const BoxDims d, Mailbox& mb, double* pnew,
+                            Mailbox* tmb) {
   const int l = lane_id();
   int pend_lo = 0, pend_hi = 0;  // new sums of the last tail repair, not yet in pvac
   for (int k = 0;; k++) {
@@ -348,15 +351,33 @@ __device__ void helper_loop(SecT& S, const SharedTables& T, const BoxDims d, Mai
     }
     if (!got) return;
     MbCmd& c = mb.q[k & (MB_RING - 1)];
-    const int4 a0 = c.a0, a1 = c.a1;
+    const int4 a0 = c.a0, a1 = c.a1, a2 = c.a2;
     const int op = uni_i(a0.x);
     if (op == CMD_EXIT) return;
     if (op == CMD_ANALYZE) {
       int knew, ilo, ihi;
       bool only;
-      analyze_hop(S, d, T.st, uni_i(a0.z), uni_i(a0.y), (uint32_t)uni_i(a1.x), (uint32_t)uni_i(a1.y),
-                  uni_i(a0.w) != 0, uni_i(a1.z), uni_i(a1.w), knew, ilo, ihi, only);
-      if (l == 0) c.r0 = make_int4(knew, ilo, ihi, only ? 1 : 0);
+      const int nv = uni_i(a0.z), vsel = uni_i(a0.y);
+      const bool stays = uni_i(a0.w) != 0;
+      const uint32_t to = (uint32_t)uni_i(a1.y);
+      analyze_hop(S, d, T.st, nv, vsel, (uint32_t)uni_i(a1.x), to, stays, uni_i(a1.z), uni_i(a1.w), knew, ilo,
+                  ihi, only);
+      const int wait_k = uni_i(a2.x);
+      int status = 0;
+      if (wait_k >= 0 && !mb_wait(*tmb, wait_k)) {
+        status = 1;
+      } else {
+        move_entry(S, nv, vsel, stays ? knew : -1);
+        if (stays && l == 0) {
+          S.key[knew] = to;
+          S.slot[knew] = uni_i(a2.y);
+        }
+        wave_sync();
+      }
+      if (l == 0) {
+        c.r0 = make_int4(knew, ilo, ihi, only ? 1 : 0);
+        c.r1 = make_int4(status, 0, 0, 0);
+      }
     } else if (op == CMD_TAIL) {
       // The exact running-sum repair behind the changed block.
       const double Pn = uni_d(unpack_d(a1.x, a1.y)), Po = uni_d(unpack_d(a1.z, a1.w));
@@ -668,6 +689,7 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
       if (l == 0) {
         c->a0 = make_int4(CMD_ANALYZE, vsel, nv, stays ? 1 : 0);
         c->a1 = make_int4((int)from, (int)to, zlo, zhi);
+        c->a2 = make_int4(tail_k, vslot, 0, 0);
       }
       const int ka = mb_commit(*mbc, mc_seq);
       if (tail_k >= 0) {
@@ -687,7 +709,10 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
       ilo = uni_i(r0.y);
       ihi = uni_i(r0.z);
       only = uni_i(r0.w) != 0;
+      if (uni_i(mbc->q[ka & (MB_RING - 1)].r1.x) != 0) { status = AKMC_RUN_ERR_HELPER; break; }
+      if (!stays) nv -= 1;
       if (tail_k >= 0) {
+        // The list helper waited for the repair before its move.
         int hi = 0;
         if (!tail_collect(*mb, tail_k, sum, hi)) { status = AKMC_RUN_ERR_HELPER; break; }
         tail_k = -1;
@@ -701,15 +726,6 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
           continue;
         }
       }
-      // The list edit proper, now that the tail repair has stopped reading
-      // the rates and tie masks it moves (running sums do not move).
-      move_entry(S, nv, vsel, stays ? knew : -1);
-      if (stays) {
-        if (l == 0) { S.key[knew] = to; S.slot[knew] = vslot; }
-      } else {
-        nv -= 1;
-      }
-      wave_sync();
     } else {
       knew = rank_and_move(S, nv, vsel, to, stays, vslot);
       cand_range(S, nv, stays ? knew : vsel, zlo, zhi, L.d.Z, ilo, ihi);
@@ -827,9 +843,9 @@ __global__ __launch_bounds__(192) void plain_kernel(PlainArgs a) {
   if (threadIdx.x < 64)
     plain_body<PlainSector, true>(a, S, PLAIN_VCAP, T, evo, &mb, &mbc, pnew);
   else if (threadIdx.x < 128)
-    helper_loop(S, T, a.dims, mb, pnew);
+    helper_loop(S, T, a.dims, mb, pnew, nullptr);
   else
-    helper_loop(S, T, a.dims, mbc, pnew);
+    helper_loop(S, T, a.dims, mbc, pnew, &mb);
 }
 
 // The same loop with the sector arrays in device memory, for sectors of more
