@@ -382,7 +382,7 @@ __device__ void helper_loop(SecT& S, const SharedTables& T, const BoxDims d, Mai
       // The exact running-sum repair behind the changed block.
       const double Pn = uni_d(unpack_d(a1.x, a1.y)), Po = uni_d(unpack_d(a1.z, a1.w));
       int hi = 0;
-      const double tot = tail_update(S, uni_i(a0.y), uni_i(a0.z), Pn, Po, pnew, hi);
+      const double tot = tail_update<true>(S, uni_i(a0.y), uni_i(a0.z), Pn, Po, pnew, hi);
       pend_lo = uni_i(a0.y);
       pend_hi = hi;
       const uint64_t tb = as_u64(tot);
@@ -800,7 +800,7 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
         pre_sum = Pn;
       } else {
         int hi = 0;
-        sum = c_hi + 1 < nv ? tail_update(S, c_hi + 1, nv, Pn, Po, S.pvac, hi) : Pn;
+        sum = c_hi + 1 < nv ? tail_update<false>(S, c_hi + 1, nv, Pn, Po, S.pvac, hi) : Pn;
       }
     }
     if (a.check_sums && !sums_agree(S, nv, sum)) st.sum_errors++;

@@ -390,7 +390,11 @@ __device__ __forceinline__ double fold_range(SecT& S, int v0, int v1, double P, 
 // Shifts the running sums of vacancies [v0, n) whose rates did not change
 // after the sum before them moved from Po to Pn.  pvac[v] holds the old sums
 // for v >= v0 on entry; the new ones go to out[v0, hi), and from hi on they
-// equal the old ones (out may be pvac itself).  Returns the new total.
+// equal the old ones.  Returns the new total.  OOP: out is a separate array,
+// not pvac; then a region round stores all its shifted sums before it looks
+// for the region's end (a store past the end is overwritten later in the
+// repair, which writes every position of [v0, hi) in increasing order, or
+// lies at or past hi), so the stores overlap the search.
 //
 // Region by region: with the old sum before v in binade e (S_old) and the
 // new one (P_new) in the same binade, D = P_new - S_old is a multiple of
@@ -402,7 +406,7 @@ __device__ __forceinline__ double fold_range(SecT& S, int v0, int v1, double P, 
 // round.  The vacancies where it ends are folded serially, four per LDS
 // round trip, until old and new sums share a binade again, and the next
 // region starts with the new exact difference.
-template <class SecT>
+template <bool OOP, class SecT>
 __device__ __forceinline__ double tail_update(SecT& S, int v0, int n, double Pn, double Po, double* out, int& hi) {
   const int l = lane_id();
   int v = uni_i(v0);
@@ -443,6 +447,10 @@ __device__ __forceinline__ double tail_update(SecT& S, int v0, int n, double Pn,
           x[k] = S.pvac[ic];
           rk[k] = S.risk[ic];
         }
+        if (OOP) {
+#pragma unroll
+          for (int k = 0; k < 4; k++) out[min(i0 + k, n - 1)] = x[k] + D;
+        }
         int kf = 4;  // first of this lane's four that ends the region
 #pragma unroll
         for (int k = 3; k >= 0; k--)
@@ -450,9 +458,11 @@ __device__ __forceinline__ double tail_update(SecT& S, int v0, int n, double Pn,
         const uint64_t m = ballot(kf < 4);
         const int fl = first_set(m);
         const int f = fl == 64 ? n : c + 4 * fl + __builtin_amdgcn_readlane(kf, fl);
+        if (!OOP) {
 #pragma unroll
-        for (int k = 0; k < 4; k++)
-          if (i0 + k < f) out[i0 + k] = x[k] + D;
+          for (int k = 0; k < 4; k++)
+            if (i0 + k < f) out[i0 + k] = x[k] + D;
+        }
         if (f < n) {
           stop = f;
           break;
