@@ -14,7 +14,7 @@ enum {
   AKMC_RUN_YIELD = 4,          // sector-visit budget of one launch used up; relaunch
   AKMC_RUN_ERR_NO_EVENT = -10, // u * sum >= sum: no event above the target
   AKMC_RUN_ERR_CAPACITY = -11, // a sector holds more vacancies than the kernel's capacity
-  AKMC_RUN_ERR_HELPER = -12    // the helper wavefront did not answer within its bound
+  AKMC_RUN_ERR_HELPER = -12    // a helper wavefront did not answer within its bound
 };
 
 namespace akmc {

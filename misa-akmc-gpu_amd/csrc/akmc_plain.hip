@@ -9,7 +9,7 @@
 // LDS; after a hop only the vacancies whose 55-site stencil holds one of the
 // two changed sites are re-evaluated, and the running sums are repaired
 // exactly (block fold + tail shift) instead of re-folded.  In the LDS kernel
-// a second wavefront takes the list edit and the tail shift off the event
+// two more wavefronts take the tail shift and the list edit off the event
 // wave, which meanwhile selects the next event from sums known to within a
 // proven bound (and falls back to waiting for the exact ones when that
 // bound cannot decide), evaluates the hopped vacancy and advances the clock
