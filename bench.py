@@ -44,7 +44,7 @@ HBM_PEAK_GBS = 8000.0
 # same workloads (FETCH_SIZE scaled by 2.0: it reads 524,306 KiB for the
 # 1 GiB read of the library's copy kernel; WRITE_SIZE reads it exactly):
 # profiles/plain_fp64_256cube_hbm_counters.md, profiles/replica_fp32_1024_hbm_counters.md.
-PLAIN_TRAFFIC_BYTES_PER_EVENT = 75.0
+PLAIN_TRAFFIC_BYTES_PER_EVENT = 75.4
 REPLICA_TRAFFIC_BYTES_PER_EVENT = 60.1
 
 WORKLOAD = dict(box=(256, 256, 256), alloy=[("Fe", 9666), ("Cu", 134), ("Mn", 130), ("Ni", 70)], va_count=6711,
@@ -576,8 +576,9 @@ def main():
                          "hbm_counter_bytes_per_event": PLAIN_TRAFFIC_BYTES_PER_EVENT,
                          "hbm_counter_frac": PLAIN_TRAFFIC_BYTES_PER_EVENT * d_events / max(ksec, 1e-12)
                                              / 1e9 / HBM_PEAK_GBS,
-                         "note": "one wavefront runs the serial event chain: bound by dependent latency "
-                                 "(LDS, L2/HBM gathers, FP64 folds), not by HBM bandwidth"},
+                         "note": "one workgroup (an event wavefront and two helpers) runs the serial "
+                                 "event chain: bound by dependent latency (LDS, L2/HBM gathers, FP64 "
+                                 "folds), not by HBM bandwidth"},
             "per_step": per_step})
     if args.copy_calibration:
         akmc.copy_bench(1 << 30, iters=5)
