@@ -164,6 +164,16 @@ int akmc_set_rng_chunk(akmc_engine* e, int64_t events);
  * reference: it costs a full sector fold per event. */
 int akmc_set_check_sums(akmc_engine* e, int on);
 
+/* How the event loop selects while its helper wavefront still repairs the
+ * running sums behind the last changed block: 0 (the default) from the
+ * exact prefix and the old sums shifted by the block's change when their
+ * error bound decides the slot, waiting for the exact sums otherwise; 1
+ * always waits; 2 uses the estimate on events of odd index only.  Every mode
+ * selects the same events (the estimate is used only where it provably
+ * agrees); a test aid with no counterpart in the reference, which selects
+ * from sums it refolds in full at every event. */
+int akmc_set_select_mode(akmc_engine* e, int mode);
+
 /* Events [first, first + n) of the two random streams akmc_run consumes, one
  * pair per event: u = r::random() of a default-seeded mt19937
  * (src/utils/random/random.cpp:35-41) and l = -log(r), r drawn by

@@ -102,6 +102,7 @@ def lib():
     L.akmc_get_event_log.argtypes = [vp, vp, vp, i64, P(i64)]
     L.akmc_set_rng_chunk.argtypes = [vp, i64]
     L.akmc_set_check_sums.argtypes = [vp, ctypes.c_int]
+    L.akmc_set_select_mode.argtypes = [vp, ctypes.c_int]
     L.akmc_run_steps.argtypes = [vp, i64, i64, P(Stats)]
     L.akmc_checkpoint_save.argtypes = [vp, ctypes.c_char_p]
     L.akmc_checkpoint_load.argtypes = [ctypes.c_char_p, i32, P(vp)]
@@ -335,6 +336,12 @@ class Engine:
     def set_check_sums(self, on=True):
         """Self-check of the running sums after every event (a test aid)."""
         _check(lib().akmc_set_check_sums(self.h, int(bool(on))))
+
+    def set_select_mode(self, mode):
+        """0: select from the bounded estimate while the tail repair runs
+        (default); 1: always wait for the exact sums; 2: estimate on odd
+        events only (a test aid: every mode selects the same events)."""
+        _check(lib().akmc_set_select_mode(self.h, int(mode)))
 
     def set_event_log(self, capacity):
         _check(lib().akmc_set_event_log(self.h, int(capacity)))

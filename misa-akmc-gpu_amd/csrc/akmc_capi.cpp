@@ -175,6 +175,7 @@ struct akmc_engine {
   int64_t log_base = 0;
   bool lattice_ready = false;
   int32_t check_sums = 0;
+  int32_t select_mode = 0;
   int64_t step_limit = INT64_MAX;  // akmc_run_steps: the time loop ends at this step
   uint8_t* d_sector = nullptr;     // device-memory sector state (sectors beyond the LDS capacity)
   int32_t sector_cap = 0;
@@ -267,6 +268,7 @@ akmc::PlainArgs base_args(akmc_engine* e) {
   a.kT = e->kT;
   a.rates_sector = -1;
   a.check_sums = e->check_sums;
+  a.select_mode = e->select_mode;
   a.sector_ws = e->d_sector;
   a.sector_cap = e->sector_cap;
   return a;
@@ -629,7 +631,7 @@ int akmc_run(akmc_engine* e, int64_t max_events, akmc_stats* stats) {
     if (st == AKMC_RUN_ERR_CAPACITY)
       return fail(AKMC_ERR_CAPACITY, "a sector holds more than %d vacancies", e->sector_cap);
     if (st == AKMC_RUN_ERR_HELPER)
-      return fail(AKMC_ERR_HIP, "event loop: the helper wavefront did not answer (event %lld)",
+      return fail(AKMC_ERR_HIP, "event loop: a helper wavefront did not answer (event %lld)",
                   (long long)e->h_state.events);
     return fail(AKMC_ERR_HIP, "event loop returned status %d", st);
   }
@@ -756,6 +758,13 @@ int akmc_run_steps(akmc_engine* e, int64_t steps, int64_t max_events, akmc_stats
 int akmc_set_check_sums(akmc_engine* e, int on) {
   if (!e) return fail(AKMC_ERR_INVALID, "null engine");
   e->check_sums = on ? 1 : 0;
+  return AKMC_OK;
+}
+
+int akmc_set_select_mode(akmc_engine* e, int mode) {
+  if (!e) return fail(AKMC_ERR_INVALID, "null engine");
+  if (mode < 0 || mode > 2) return fail(AKMC_ERR_INVALID, "select mode must be 0, 1 or 2");
+  e->select_mode = mode;
   return AKMC_OK;
 }
 

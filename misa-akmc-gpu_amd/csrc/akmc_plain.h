@@ -62,6 +62,11 @@ struct PlainArgs {
   // Self-check: after every update of the running sums, every lane refolds
   // the sector serially in sweep order and compares (counted in sum_errors).
   int32_t check_sums;
+  // Selection while the tail repair runs (LDS kernel): 0 from the estimated
+  // sums when their bound decides, 1 never (always wait for the exact sums),
+  // 2 only on events of odd index.  A test aid: every mode selects the same
+  // events.
+  int32_t select_mode;
   uint32_t* out_keys;
   double* out_rates;
   double* out_total;

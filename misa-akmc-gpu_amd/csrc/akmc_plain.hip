@@ -336,8 +336,8 @@ __device__ __forceinline__ void copy_sums(SecT& S, const double* pnew, int lo, i
 
 // tmb: the sum helper's mailbox (for the list helper's wait on a tail repair).
 template <class SecT>
-__device__ void helper_loop(SecT& S, const SharedTables& T, const BoxDims d, Mailbox& mb, double* pnew,
-                            Mailbox* tmb) {
+__device__ __forceinline__ void helper_loop(SecT& S, const SharedTables& T, const BoxDims d, Mailbox& mb,
+                                            double* pnew, Mailbox* tmb) {
   const int l = lane_id();
   int pend_lo = 0, pend_hi = 0;  // new sums of the last tail repair, not yet in pvac
   for (int k = 0;; k++) {
@@ -450,7 +450,9 @@ __device__ __forceinline__ bool tail_collect(Mailbox& mb, int k, double& tot, in
 // HELPER: two helper wavefronts run helper_loop on the same LDS state, mb
 // and mbc are the mailboxes of the sum helper and the list helper and pnew
 // the tail-repair array (the LDS kernel); otherwise all three are unused.
-template <class SecT, bool HELPER>
+// AIDS: the test aids of the arguments (check_sums, select_mode) are read;
+// the instance without them is the one launched when both are off.
+template <class SecT, bool HELPER, bool AIDS>
 __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap, SharedTables& T, double* evo,
                                            Mailbox* mb, Mailbox* mbc, double* pnew) {
   const int l = lane_id();
@@ -527,7 +529,7 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
       st.n_eval += nv;
       st.n_fold += nv;
       sum = fold_range(S, 0, nv, 0.0, S.pvac);
-      if (a.check_sums && !sums_agree(S, nv, sum)) st.sum_errors++;
+      if (AIDS && a.check_sums && !sums_agree(S, nv, sum)) st.sum_errors++;
       rebuild = false;
       if (a.rates_sector >= 0) {
         for (int i = l; i < nv && i < a.out_cap; i += 64) {
@@ -568,7 +570,8 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
       // selects, the exact target selects the same slot.
       const double u = uni_d(u_next);
       const double tm = 2.0 * fmax(t_old, t_apx);
-      if (tm > 0x1p-969) {
+      const bool try_est = !AIDS || a.select_mode == 0 || (a.select_mode == 2 && (st.events & 1));
+      if (try_est && tm > 0x1p-969) {
         const double ulp_t = __builtin_ldexp(1.0, exponent_of(tm) - 52);
         const double eps = (16.0 * (double)(nv - pre_n) + 32.0) * ulp_t;
         const double tt = u * t_apx;
@@ -784,9 +787,9 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
       const double Pn = fold_range(S, c_lo, c_hi + 1, c_lo > 0 ? S.pvac[c_lo - 1] : 0.0, S.pvac);
       st.n_fold += c_hi + 1 - c_lo;
       st.n_shift += nv - (c_hi + 1);
-      if (HELPER && !a.check_sums && c_hi + 1 < nv) {
+      if (HELPER && c_hi + 1 < nv) {
         // The helper repairs the tail while the next event is selected
-        // (from the exact prefix when it can be) and performed.
+        // (from the estimated sums when their bound decides) and performed.
         MbCmd* c = &mb->q[mb_seq & (MB_RING - 1)];
         if (l == 0) {
           c->a0 = make_int4(CMD_TAIL, c_hi + 1, nv, 0);
@@ -798,12 +801,19 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
         t_apx = uni_d(sum + d_est);
         pre_n = c_hi + 1;
         pre_sum = Pn;
+        if (AIDS && a.check_sums) {
+          // The self-check compares the repaired sums: wait for them.
+          int hi = 0;
+          if (!tail_collect(*mb, tail_k, sum, hi)) { status = AKMC_RUN_ERR_HELPER; break; }
+          copy_sums(S, pnew, pre_n, hi);
+          tail_k = -1;
+        }
       } else {
         int hi = 0;
         sum = c_hi + 1 < nv ? tail_update<false>(S, c_hi + 1, nv, Pn, Po, S.pvac, hi) : Pn;
       }
     }
-    if (a.check_sums && !sums_agree(S, nv, sum)) st.sum_errors++;
+    if (AIDS && a.check_sums && !sums_agree(S, nv, sum)) st.sum_errors++;
   }
   if (HELPER) {
     MbCmd* c = mb_reserve(*mb, mb_seq);
@@ -827,7 +837,8 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
 
 // Three wavefronts: the event loop (wave 0), the sum helper (wave 1) and the
 // list helper (wave 2).
-__global__ __launch_bounds__(192) void plain_kernel(PlainArgs a) {
+template <bool AIDS>
+__device__ __forceinline__ void plain_lds(const PlainArgs& a) {
   __shared__ PlainSector S;
   __shared__ SharedTables T;
   __shared__ double evo[8];  // sector clocks (indexed by the running sector id)
@@ -841,12 +852,15 @@ __global__ __launch_bounds__(192) void plain_kernel(PlainArgs a) {
   }
   __syncthreads();
   if (threadIdx.x < 64)
-    plain_body<PlainSector, true>(a, S, PLAIN_VCAP, T, evo, &mb, &mbc, pnew);
+    plain_body<PlainSector, true, AIDS>(a, S, PLAIN_VCAP, T, evo, &mb, &mbc, pnew);
   else if (threadIdx.x < 128)
     helper_loop(S, T, a.dims, mb, pnew, nullptr);
   else
     helper_loop(S, T, a.dims, mbc, pnew, &mb);
 }
+__global__ __launch_bounds__(192) void plain_kernel(PlainArgs a) { plain_lds<false>(a); }
+// The same with the test aids (akmc_set_check_sums, akmc_set_select_mode).
+__global__ __launch_bounds__(192) void plain_kernel_aids(PlainArgs a) { plain_lds<true>(a); }
 
 // The same loop with the sector arrays in device memory, for sectors of more
 // than PLAIN_VCAP vacancies.
@@ -862,7 +876,7 @@ __global__ __launch_bounds__(64) void plain_kernel_hbm(PlainArgs a) {
   SectorRef S{reinterpret_cast<uint32_t*>(w + off[0]), reinterpret_cast<int32_t*>(w + off[1]),
               reinterpret_cast<double*>(w + off[2]),   reinterpret_cast<double*>(w + off[3]),
               reinterpret_cast<uint64_t*>(w + off[4]), w + off[5], w + off[6], w + off[7], sten, cflag, list};
-  plain_body<SectorRef, false>(a, S, a.sector_cap, T, evo, nullptr, nullptr, nullptr);
+  plain_body<SectorRef, false, true>(a, S, a.sector_cap, T, evo, nullptr, nullptr, nullptr);
 }
 
 }  // namespace akmc
@@ -870,6 +884,8 @@ __global__ __launch_bounds__(64) void plain_kernel_hbm(PlainArgs a) {
 extern "C" int akmc_plain_launch(const akmc::PlainArgs* args, hipStream_t stream) {
   if (args->sector_ws)
     hipLaunchKernelGGL(akmc::plain_kernel_hbm, dim3(1), dim3(64), 0, stream, *args);
+  else if (args->check_sums || args->select_mode)
+    hipLaunchKernelGGL(akmc::plain_kernel_aids, dim3(1), dim3(192), 0, stream, *args);
   else
     hipLaunchKernelGGL(akmc::plain_kernel, dim3(1), dim3(192), 0, stream, *args);
   return (int)hipGetLastError();

@@ -30,10 +30,12 @@ def _bits(a):
     return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
 
 
-def compare_run(cfg, n, chunk=None, pieces=1):
+def compare_run(cfg, n, chunk=None, pieces=1, select_mode=None):
     eng = akmc.Engine(cfg)
     if chunk:
         eng.set_rng_chunk(chunk)
+    if select_mode is not None:
+        eng.set_select_mode(select_mode)
     eng.set_event_log(n)
     ref = refcpu.RefCpu(cfg)
     assert np.array_equal(eng.download(), ref.sites())
@@ -110,6 +112,19 @@ def test_trajectory_bit_exact(name, cfg, n):
     compare_run(cfg, n)
 
 
+@pytest.mark.parametrize("mode", [1, 2])
+def test_select_modes_bit_exact(mode):
+    """While the helper wavefront repairs the running sums, the event loop
+    selects from their bounded estimate; on real trajectories the bound
+    almost always decides, so its exact fallback (wait for the repair, copy
+    the repaired sums, select exactly) is forced here: always (mode 1) or on
+    alternate events (mode 2, mixing both paths and the copies each posts).
+    Trajectories, clocks and lattices bit-exact vs the oracle."""
+    for name, cfg, n in (TRAJ_CASES[1], TRAJ_CASES[3], TRAJ_CASES[5], TRAJ_CASES[6]):
+        say("test_select_modes_bit_exact", mode, name, n)
+        compare_run(cfg, n, select_mode=mode)
+
+
 def test_resume_and_small_chunks():
     say("test_resume_and_small_chunks")
     cfg = make_cfg(n=24, va=120, T=590.0)
@@ -145,8 +160,9 @@ SUM_CASES = [c[1] for c in TRAJ_CASES] + [
 
 
 def test_running_sums_self_check():
-    """The incrementally kept running sums (block folds, exact tail shifts,
-    serial folds at binade crossings and ties) against a serial refold of the
+    """The incrementally kept running sums (block folds, exact tail shifts
+    made by the helper wavefront into its own array and copied back, serial
+    folds at binade crossings and ties) against a serial refold of the
     sector after every event, on the trajectory cases, the golden cases and
     the 256^3 benchmark box."""
     say("test_running_sums_self_check")
