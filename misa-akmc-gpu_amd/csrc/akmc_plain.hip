@@ -496,51 +496,58 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
     for (int i = 0; i < 8; i++)
       if (ring_sector(i) == a.rates_sector) st.sect = i;
   }
+  // After an event that went the whole way through the loop body the checks
+  // below all pass (its residence step has tested the clock against the
+  // threshold and the event counts against their limits): skip them.
+  bool cont = false;
   for (;;) {
-    if (!st.in_sector) {
-      if (st.step >= a.time_steps) { status = AKMC_RUN_TIME_DONE; break; }
-      if (++visits > (int64_t)1 << 16) { status = AKMC_RUN_YIELD; break; }
+    if (!cont) {
+      if (!st.in_sector) {
+        if (st.step >= a.time_steps) { status = AKMC_RUN_TIME_DONE; break; }
+        if (++visits > (int64_t)1 << 16) { status = AKMC_RUN_YIELD; break; }
+        s = ring_sector(st.sect);
+        // step_threshold_time (sublattice.inl:72)
+        st.thr = static_cast<double>(st.step + 1) * a.sector_T - evo[s];
+        st.t = 0.0;
+        st.in_sector = 1;
+        rebuild = true;
+      }
       s = ring_sector(st.sect);
-      // step_threshold_time (sublattice.inl:72)
-      st.thr = static_cast<double>(st.step + 1) * a.sector_T - evo[s];
-      st.t = 0.0;
-      st.in_sector = 1;
-      rebuild = true;
-    }
-    s = ring_sector(st.sect);
-    if (!(st.t < st.thr)) {
-      if (l == 0) evo[s] += st.t;  // sublattice.inl:140
-      wave_sync();
-      st.in_sector = 0;
-      if (++st.sect == 8) { st.sect = 0; st.step++; }
-      continue;
-    }
-    if (st.events >= a.max_events) { status = AKMC_RUN_EVENT_CAP; break; }
-    if (st.events >= a.rng_end) { status = AKMC_RUN_NEED_RNG; break; }
+      if (!(st.t < st.thr)) {
+        if (l == 0) evo[s] += st.t;  // sublattice.inl:140
+        wave_sync();
+        st.in_sector = 0;
+        if (++st.sect == 8) { st.sect = 0; st.step++; }
+        continue;
+      }
+      if (st.events >= a.max_events) { status = AKMC_RUN_EVENT_CAP; break; }
+      if (st.events >= a.rng_end) { status = AKMC_RUN_NEED_RNG; break; }
 
-    if (rebuild) {
-      // The whole sector is re-read from HBM: make this wave's earlier
-      // writes visible first.  (Within a sector the engine never reads back
-      // a byte it wrote in the same event; those are patched from registers.)
-      __threadfence_block();
-      nv = collect_sector(S, L, s, cap);
-      if (nv < 0) { status = AKMC_RUN_ERR_CAPACITY; break; }
-      eval_list(S, T, LS, L, nullptr, 0, nv, nopatch, false, 0, -1);
-      st.n_eval += nv;
-      st.n_fold += nv;
-      sum = fold_range(S, 0, nv, 0.0, S.pvac);
-      if (AIDS && a.check_sums && !sums_agree(S, nv, sum)) st.sum_errors++;
-      rebuild = false;
-      if (a.rates_sector >= 0) {
-        for (int i = l; i < nv && i < a.out_cap; i += 64) {
-          a.out_keys[i] = S.key[i];
-          for (int b = 0; b < 8; b++) a.out_rates[8 * i + b] = S.rate[8 * i + b];
+      if (rebuild) {
+        // The whole sector is re-read from HBM: make this wave's earlier
+        // writes visible first.  (Within a sector the engine never reads back
+        // a byte it wrote in the same event; those are patched from registers.)
+        __threadfence_block();
+        nv = collect_sector(S, L, s, cap);
+        if (nv < 0) { status = AKMC_RUN_ERR_CAPACITY; break; }
+        eval_list(S, T, LS, L, nullptr, 0, nv, nopatch, false, 0, -1);
+        st.n_eval += nv;
+        st.n_fold += nv;
+        sum = fold_range(S, 0, nv, 0.0, S.pvac);
+        if (AIDS && a.check_sums && !sums_agree(S, nv, sum)) st.sum_errors++;
+        rebuild = false;
+        if (a.rates_sector >= 0) {
+          for (int i = l; i < nv && i < a.out_cap; i += 64) {
+            a.out_keys[i] = S.key[i];
+            for (int b = 0; b < 8; b++) a.out_rates[8 * i + b] = S.rate[8 * i + b];
+          }
+          if (l == 0) { *a.out_total = sum; *a.out_n = nv; }
+          status = AKMC_RUN_OK;
+          break;
         }
-        if (l == 0) { *a.out_total = sum; *a.out_n = nv; }
-        status = AKMC_RUN_OK;
-        break;
       }
     }
+    cont = false;
     // A tail repair still with the helper: its total exceeds the exact
     // prefix sum, so the zero test below is decided unless that is tiny.
     if (HELPER && tail_k >= 0 && !(pre_sum >= 1e-10)) {
@@ -814,6 +821,7 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
       }
     }
     if (AIDS && a.check_sums && !sums_agree(S, nv, sum)) st.sum_errors++;
+    cont = true;
   }
   if (HELPER) {
     MbCmd* c = mb_reserve(*mb, mb_seq);
