@@ -450,8 +450,9 @@ __device__ __forceinline__ bool tail_collect(Mailbox& mb, int k, double& tot, in
 // HELPER: two helper wavefronts run helper_loop on the same LDS state, mb
 // and mbc are the mailboxes of the sum helper and the list helper and pnew
 // the tail-repair array (the LDS kernel); otherwise all three are unused.
-// AIDS: the test aids of the arguments (check_sums, select_mode) are read;
-// the instance without them is the one launched when both are off.
+// AIDS: the test aids and hooks of the arguments (check_sums, select_mode,
+// the event log, the parity hook rates_sector) are read; the instance
+// without them is the one launched when all are off.
 template <class SecT, bool HELPER, bool AIDS>
 __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap, SharedTables& T, double* evo,
                                            Mailbox* mb, Mailbox* mbc, double* pnew) {
@@ -462,6 +463,7 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
   const LatticeView L{a.sites, a.vac_key, a.nvac, a.dims, a.nu, a.kT};
   const Patch nopatch{0xFFFFFFFFu, 0xFFFFFFFFu, 0, 0, -1, ~0ull, ~0ull};
   PlainState st = *a.state;  // every lane keeps an identical copy
+  const int rsec = AIDS ? a.rates_sector : -1;  // the parity hook's sector (-1: none)
   if (l < 8) evo[l] = a.state->evo[l];
   wave_sync();
   bool rebuild = true;
@@ -482,19 +484,19 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
   int copy_k = -1;  // a COPY posted and not yet waited for
   // Random numbers of the next event, loaded one event ahead.
   double u_next = 0.0, l_next = 0.0;
-  if (a.rates_sector < 0 && a.u_stream && a.l_stream && st.events >= a.rng_base && st.events < a.rng_end) {
+  if (rsec < 0 && a.u_stream && a.l_stream && st.events >= a.rng_base && st.events < a.rng_end) {
     u_next = a.u_stream[st.events - a.rng_base];
     l_next = a.l_stream[st.events - a.rng_base];
   }
 
-  if (a.rates_sector >= 0) {
+  if (rsec >= 0) {
     // Parity hook: evaluate one sector as that sector's first event would.
     st.in_sector = 1;
     st.t = 0.0;
     st.thr = 1.0;
     st.events = 0;
     for (int i = 0; i < 8; i++)
-      if (ring_sector(i) == a.rates_sector) st.sect = i;
+      if (ring_sector(i) == rsec) st.sect = i;
   }
   // After an event that went the whole way through the loop body the checks
   // below all pass (its residence step has tested the clock against the
@@ -536,7 +538,7 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
         sum = fold_range(S, 0, nv, 0.0, S.pvac);
         if (AIDS && a.check_sums && !sums_agree(S, nv, sum)) st.sum_errors++;
         rebuild = false;
-        if (a.rates_sector >= 0) {
+        if (rsec >= 0) {
           for (int i = l; i < nv && i < a.out_cap; i += 64) {
             a.out_keys[i] = S.key[i];
             for (int b = 0; b < 8; b++) a.out_rates[8 * i + b] = S.rate[8 * i + b];
@@ -657,7 +659,7 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
       L.sites[key_index(L.d, to)] = nf_to;
       L.vac_key[vslot] = to;
       const int64_t li = st.events - a.log_base;
-      if (a.log && li >= 0 && li < a.log_cap) { a.log[2 * li] = from; a.log[2 * li + 1] = to; }
+      if (AIDS && a.log && li >= 0 && li < a.log_cap) { a.log[2 * li] = from; a.log[2 * li + 1] = to; }
     }
     st.events++;
     if (st.events < a.rng_end) {
@@ -835,12 +837,12 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
       mb_commit(*mbc, mc_seq);
     }
   }
-  if (l == 0 && a.rates_sector < 0) {
+  if (l == 0 && rsec < 0) {
     st.status = status;
     for (int i = 0; i < 8; i++) st.evo[i] = evo[i];
     *a.state = st;
   }
-  if (l == 0 && a.rates_sector >= 0 && status != AKMC_RUN_OK) *a.out_n = status;
+  if (l == 0 && rsec >= 0 && status != AKMC_RUN_OK) *a.out_n = status;
 }
 
 // Three wavefronts: the event loop (wave 0), the sum helper (wave 1) and the
@@ -867,7 +869,8 @@ __device__ __forceinline__ void plain_lds(const PlainArgs& a) {
     helper_loop(S, T, a.dims, mbc, pnew, &mb);
 }
 __global__ __launch_bounds__(192) void plain_kernel(PlainArgs a) { plain_lds<false>(a); }
-// The same with the test aids (akmc_set_check_sums, akmc_set_select_mode).
+// The same with the test aids and hooks (akmc_set_check_sums,
+// akmc_set_select_mode, the event log, the parity hook).
 __global__ __launch_bounds__(192) void plain_kernel_aids(PlainArgs a) { plain_lds<true>(a); }
 
 // The same loop with the sector arrays in device memory, for sectors of more
@@ -892,7 +895,7 @@ __global__ __launch_bounds__(64) void plain_kernel_hbm(PlainArgs a) {
 extern "C" int akmc_plain_launch(const akmc::PlainArgs* args, hipStream_t stream) {
   if (args->sector_ws)
     hipLaunchKernelGGL(akmc::plain_kernel_hbm, dim3(1), dim3(64), 0, stream, *args);
-  else if (args->check_sums || args->select_mode)
+  else if (args->check_sums || args->select_mode || args->log || args->rates_sector >= 0)
     hipLaunchKernelGGL(akmc::plain_kernel_aids, dim3(1), dim3(192), 0, stream, *args);
   else
     hipLaunchKernelGGL(akmc::plain_kernel, dim3(1), dim3(192), 0, stream, *args);

@@ -30,13 +30,14 @@ def _bits(a):
     return np.ascontiguousarray(a, dtype=np.float64).view(np.uint64)
 
 
-def compare_run(cfg, n, chunk=None, pieces=1, select_mode=None):
+def compare_run(cfg, n, chunk=None, pieces=1, select_mode=None, log=True):
     eng = akmc.Engine(cfg)
     if chunk:
         eng.set_rng_chunk(chunk)
     if select_mode is not None:
         eng.set_select_mode(select_mode)
-    eng.set_event_log(n)
+    if log:
+        eng.set_event_log(n)
     ref = refcpu.RefCpu(cfg)
     assert np.array_equal(eng.download(), ref.sites())
     done = 0
@@ -46,11 +47,12 @@ def compare_run(cfg, n, chunk=None, pieces=1, select_mode=None):
         done = st["events"]
     rc, lf, lt = ref.run(n, log=True)
     rs = ref.state()
-    gf, gt = eng.event_log(n)
     assert st["events"] == rs["events"]
     m = rs["events"]
-    assert np.array_equal(gf[:m], lf[:m]) and np.array_equal(gt[:m], lt[:m]), \
-        "first divergence at %s" % np.nonzero((gf[:m] != lf[:m]) | (gt[:m] != lt[:m]))[0][:3]
+    if log:
+        gf, gt = eng.event_log(n)
+        assert np.array_equal(gf[:m], lf[:m]) and np.array_equal(gt[:m], lt[:m]), \
+            "first divergence at %s" % np.nonzero((gf[:m] != lf[:m]) | (gt[:m] != lt[:m]))[0][:3]
     assert np.array_equal(eng.download(), ref.sites())
     assert np.array_equal(_bits(st["evolution"]), _bits(rs["evolution"]))
     assert st["sector_time"] == rs["sector_time"] and st["step"] == rs["step"]
@@ -110,6 +112,16 @@ TRAJ_CASES = [
 def test_trajectory_bit_exact(name, cfg, n):
     say("test_trajectory_bit_exact", name, n)
     compare_run(cfg, n)
+
+
+def test_default_kernel_bit_exact():
+    """The event log, the self-check, the selection modes and the parity hook
+    run in a second instance of the kernel; with none of them set the
+    default instance runs: final lattices, sector clocks and counters
+    bit-exact vs the oracle on the trajectory cases."""
+    for name, cfg, n in TRAJ_CASES:
+        say("test_default_kernel_bit_exact", name, n)
+        compare_run(cfg, n, log=False)
 
 
 @pytest.mark.parametrize("mode", [1, 2])
