@@ -1,11 +1,12 @@
-// One-wavefront sector engine: the vacancy-hop event loop of one lattice's
-// current sector, run by a single 64-lane wavefront on LDS-resident state.
+// Wavefront-level sector engine: the vacancy-hop event loop of one lattice's
+// current sector, run by one 64-lane wavefront on LDS-resident state (in the
+// plain LDS kernel with two helper wavefronts beside it, akmc_plain.hip).
 //
 // The event chain of one lattice is serial (every hop changes the next
-// rates), so the engine is written for latency, not throughput: one wave, no
-// workgroup barriers, cross-lane work through ballots, DPP and readlane,
-// loads issued together and consumed late.  The plain path runs one engine on
-// the whole GPU; the batched path runs one engine per replica, many per CU.
+// rates), so the engine is written for latency, not throughput: no workgroup
+// barriers, cross-lane work through ballots, DPP and readlane, loads issued
+// together and consumed late.  The plain path runs one engine on the whole
+// GPU; the batched path runs one engine per replica, many per CU.
 //
 // Exactness of the running sums.  The reference folds the sector's rates in
 // sweep order, sum_rates += rate (models/abvi/kmc.cpp:103), and select()
