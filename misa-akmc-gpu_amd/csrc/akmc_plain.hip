@@ -243,7 +243,7 @@ __device__ __forceinline__ void analyze_hop(const SecT& S, const BoxDims& d, con
 // waiting cost more than the stores).
 enum : int32_t { CMD_ANALYZE = 1, CMD_TAIL = 2, CMD_EXIT = 3, CMD_COPY = 4 };
 constexpr int MB_RING = 16;
-constexpr int64_t MB_SPIN_LIMIT = (int64_t)1 << 26;  // polls of ~100 cycles: seconds
+constexpr int64_t MB_SPIN_LIMIT = (int64_t)1 << 26;  // x4: the bound of a wait, seconds
 // A command: arguments in three 16-byte words, results in two, so that each
 // side writes and reads them with a few wide LDS accesses.
 //   ANALYZE  a0 = op, vsel, nv, stays; a1 = from, to, zlo, zhi;
@@ -309,12 +309,13 @@ __device__ __forceinline__ int mb_commit(Mailbox& mb, int& seq) {
   seq = k + 1;
   return k;
 }
-// Event wave: waits until commands 0 .. k are done.
+// Waits until commands 0 .. k are done, polling without a sleep (a hand-over
+// on the event chain costs the poll's LDS round trip, not a sleep quantum):
+// ~2^28 polls, seconds, before it reports failure.
 __device__ __forceinline__ bool mb_wait(Mailbox& mb, int k) {
   int64_t spins = 0;
   while (mb_load(&mb.done) <= k) {
-    if (++spins > MB_SPIN_LIMIT) return false;
-    __builtin_amdgcn_s_sleep(1);
+    if (++spins > 4 * MB_SPIN_LIMIT) return false;
   }
   return true;
 }
@@ -347,7 +348,7 @@ __device__ __forceinline__ void helper_loop(SecT& S, const SharedTables& T, cons
       // The event wave always posts CMD_EXIT before it ends; this bound only
       // keeps a broken protocol from running forever.
       if (++spins > 4 * MB_SPIN_LIMIT) { got = false; break; }
-      __builtin_amdgcn_s_sleep(2);
+      __builtin_amdgcn_s_sleep(1);
     }
     if (!got) return;
     MbCmd& c = mb.q[k & (MB_RING - 1)];
