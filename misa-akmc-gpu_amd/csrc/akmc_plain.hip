@@ -725,9 +725,9 @@ __device__ __forceinline__ void plain_body(const PlainArgs& a, SecT& S, int cap,
       if (uni_i(mbc->q[ka & (MB_RING - 1)].r1.x) != 0) { status = AKMC_RUN_ERR_HELPER; break; }
       if (!stays) nv -= 1;
       if (tail_k >= 0) {
-        // The list helper waited for the repair before its move.
-        int hi = 0;
-        if (!tail_collect(*mb, tail_k, sum, hi)) { status = AKMC_RUN_ERR_HELPER; break; }
+        // The list helper waited for the repair before its move (and its
+        // answer was read after an acquire), so the total is there.
+        sum = uni_d(unpack_d(mb->q[tail_k & (MB_RING - 1)].r0.x, mb->q[tail_k & (MB_RING - 1)].r0.y));
         tail_k = -1;
         st.t += lg / sum;
         if (!(st.t < st.thr)) {
