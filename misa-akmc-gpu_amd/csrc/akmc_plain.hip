@@ -365,10 +365,43 @@ __device__ __forceinline__ void helper_loop(SecT& S, const SharedTables& T, cons
                   ihi, only);
       const int wait_k = uni_i(a2.x);
       int status = 0;
-      if (wait_k >= 0 && !mb_wait(*tmb, wait_k)) {
+      // The entries between move by one (as move_entry does).  Up to 64 of
+      // them are loaded before the wait for the tail repair, which only
+      // reads the rates and tie masks, and stored after it.
+      const int to_i = stays ? knew : -1;
+      const int lo = to_i < 0 || to_i >= vsel ? vsel + 1 : to_i;
+      const int hi = to_i < 0 ? nv - 1 : to_i >= vsel ? to_i : vsel - 1;
+      const int delta = to_i < 0 || to_i >= vsel ? -1 : 1;
+      if (hi - lo + 1 <= 64) {
+        const int v = lo + l;
+        const bool act = v <= hi;
+        const int vc = hi >= lo ? min(v, hi) : 0;  // [lo, hi] lies in [0, nv)
+        const uint32_t kk = S.key[vc];
+        const int32_t sl = S.slot[vc];
+        const uint8_t o = S.own[vc];
+        const uint64_t tm = S.tmask[vc];
+        double rt[8];
+        uint8_t nb[8];
+#pragma unroll
+        for (int b = 0; b < 8; b++) { rt[b] = S.rate[8 * vc + b]; nb[b] = S.nb1[8 * vc + b]; }
+        if (wait_k >= 0 && !mb_wait(*tmb, wait_k)) {
+          status = 1;
+        } else {
+          wave_sync();
+          if (act) {
+            const int w = v + delta;
+            S.key[w] = kk; S.slot[w] = sl; S.own[w] = o; S.tmask[w] = tm;
+#pragma unroll
+            for (int b = 0; b < 8; b++) { S.rate[8 * w + b] = rt[b]; S.nb1[8 * w + b] = nb[b]; }
+          }
+          wave_sync();
+        }
+      } else if (wait_k >= 0 && !mb_wait(*tmb, wait_k)) {
         status = 1;
       } else {
-        move_entry(S, nv, vsel, stays ? knew : -1);
+        move_entry(S, nv, vsel, to_i);
+      }
+      if (status == 0) {
         if (stays && l == 0) {
           S.key[knew] = to;
           S.slot[knew] = uni_i(a2.y);
